@@ -1,0 +1,1184 @@
+// hx3d engine: MI355X (gfx950) HIP kernels for one explicit RK3 step of HORSES3D's
+// compressible Navier-Stokes DGSEM on Gauss-Lobatto nodes, and the C ABI of include/hx3d.h.
+//
+// One stage (ComputeTimeDerivative, SpatialDiscretization.f90:227-320) is three launches:
+//   k_grad    per element: local gradient (HexElement_ComputeLocalGradient,
+//             HexElementClass.f90:427-531) + BR1 lifting (BR1_LiftGradients,
+//             EllipticBR1.f90:168-397).  With GL nodes the face trace is the boundary node
+//             itself (ProlongSolutionToFaces is a copy) and the lifting only touches
+//             boundary nodes, so the neighbour's boundary state is gathered directly.
+//   k_face    per face node: Riemann flux (Roe/LxF) + BR1 viscous average
+//             (computeElementInterfaceFlux, SD.f90:1710-1799), written once per face.
+//   k_volume  per element: viscosity (Sutherland + LES), inviscid/viscous contravariant
+//             fluxes, the standard or split-form volume integral
+//             (TimeDerivative_VolumetricContribution, SD.f90:1602-1682), the surface
+//             integral of the face fluxes, 1/J (SD.f90:483-494), and the Williamson RK3
+//             register update (TakeRK3Step, ExplicitMethods.f90:748-775), fused.
+// Device layout: structure of arrays, one contiguous array per variable:
+//   Q[v*S + g], G[v*S + g], grad[(5d + v)*S + g], Ja[(3d + c)*S + g], invJ[g]
+// with g = e*(N+1)^3 + i + (N+1)(j + (N+1)k) and S the padded node count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hx3d.h"
+#include "physics.hpp"
+
+using hx::NV;
+using hx::Phys;
+
+namespace {
+
+constexpr int MAXN = 7;
+
+struct Ops {  // NodalStorage for GL nodes (NodalStorageClass.f90:170-337), row-major (i,l)
+    double D[64], hatD[64], sharpD[64], w[8];
+};
+
+// element face slot 0..5 = EFRONT, EBACK, EBOTTOM, ERIGHT, ETOP, ELEFT
+constexpr int SLOT_AXIS[6] = {1, 1, 2, 0, 2, 0};
+constexpr int SLOT_PLUS[6] = {0, 1, 0, 1, 1, 0};
+constexpr int SLOT_U[6] = {0, 0, 0, 1, 0, 1};
+constexpr int SLOT_V[6] = {2, 2, 1, 2, 1, 2};
+
+__host__ __device__ inline int xcd_block(int b, int grid)
+{
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+    // elements so that face neighbours (gathered through L2) mostly share an XCD.  The grid
+    // is a multiple of 8, which makes this a bijection of [0, grid); logical blocks past
+    // the last element exit.
+    const int per = grid >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+inline int grid_for(int nb) { return ((nb + 7) / 8) * 8; }
+
+// ------------------------------------------------------------------------------------
+// k_grad
+// ------------------------------------------------------------------------------------
+template <int N, int EPB>
+__global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
+k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
+       const int *__restrict__ nbr, double *__restrict__ grad, int K, long S, Ops ops)
+{
+    constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
+    __shared__ double sQ[EPB][NV][ne];
+    __shared__ double sD[n1 * n1];
+    const int nb = (K + EPB - 1) / EPB;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int t = threadIdx.x;
+    const int el = t / ne, n = t - el * ne;
+    const int e = lb * EPB + el;
+    const bool active = lb < nb && el < EPB && e < K;
+    for (int x = t; x < n1 * n1; x += blockDim.x) sD[x] = ops.D[(x / n1) * 8 + (x % n1)];
+    const long g = (long)e * ne + n;
+    double q[NV];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { q[v] = Q[v * S + g]; sQ[el][v][n] = q[v]; }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int i = n % n1, j = (n / n1) % n1, k = n / nf;
+    double uxi[NV], ueta[NV], uzeta[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { uxi[v] = 0.0; ueta[v] = 0.0; uzeta[v] = 0.0; }
+#pragma unroll
+    for (int l = 0; l < n1; ++l) {
+        const double di = sD[i * n1 + l], dj = sD[j * n1 + l], dk = sD[k * n1 + l];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            uxi[v] += sQ[el][v][l + n1 * (j + n1 * k)] * di;
+            ueta[v] += sQ[el][v][i + n1 * (l + n1 * k)] * dj;
+            uzeta[v] += sQ[el][v][i + n1 * (j + n1 * l)] * dk;
+        }
+    }
+    double ja[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
+    const double iJ = invJ[g];
+    double gr[3][NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        gr[0][v] = (uxi[v] * ja[0] + ueta[v] * ja[3] + uzeta[v] * ja[6]) * iJ;
+        gr[1][v] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
+        gr[2][v] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
+    }
+    // BR1 lifting (BR1_ComputeElementInterfaceAverage + BR1_GradientFaceLoop): on each face
+    // the node lies on, unStar = 1/2 (U_nbr - U_own) |Ja| n_out, the same value for both
+    // sides; the GL surface integral weights it by b = 1/w_0.
+    const bool onface = (i == 0) | (i == N) | (j == 0) | (j == N) | (k == 0) | (k == N);
+    if (onface) {
+        const double b0 = 1.0 / ops.w[0];
+        double fi[3][NV];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fi[d][v] = 0.0;
+        auto lift = [&](int slot, int a, int b, double sgn, int dax) {
+            const int gn = nbr[((long)e * 6 + slot) * nf + a + n1 * b];
+            const double s0 = sgn * ja[3 * dax], s1 = sgn * ja[3 * dax + 1], s2 = sgn * ja[3 * dax + 2];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double us = 0.5 * (Q[v * S + gn] - q[v]) * b0;
+                fi[0][v] += us * s0;
+                fi[1][v] += us * s1;
+                fi[2][v] += us * s2;
+            }
+        };
+        if (i == 0) lift(5, j, k, -1.0, 0);
+        if (i == N) lift(3, j, k, 1.0, 0);
+        if (j == 0) lift(0, i, k, -1.0, 1);
+        if (j == N) lift(1, i, k, 1.0, 1);
+        if (k == 0) lift(2, i, j, -1.0, 2);
+        if (k == N) lift(4, i, j, 1.0, 2);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gr[d][v] += fi[d][v] * iJ;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) grad[(5 * d + v) * S + g] = gr[d][v];
+}
+
+// ------------------------------------------------------------------------------------
+// k_face: one thread per face node
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256)
+k_face(const double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ Ja,
+       const int2 *__restrict__ fnode, const unsigned char *__restrict__ fslot,
+       const double *__restrict__ fdelta, double *__restrict__ fstar, long FS, long S, Phys ph)
+{
+    constexpr int n1 = N + 1, nf = n1 * n1;
+    const long fq = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fq >= FS) return;
+    const long f = fq / nf;
+    const int2 gg = fnode[fq];
+    const int sl = fslot[f];
+    const int dax = sl == 3 || sl == 5 ? 0 : (sl <= 1 ? 1 : 2);
+    const double sgn = (sl == 1 || sl == 3 || sl == 4) ? 1.0 : -1.0;
+    double nrm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nrm[c] = sgn * Ja[(3 * dax + c) * S + gg.x];
+    const double jac = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    const double ijac = 1.0 / jac;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nrm[c] *= ijac;
+    double qL[NV], qR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { qL[v] = Q[v * S + gg.x]; qR[v] = Q[v * S + gg.y]; }
+    double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!ph.euler) {
+        double gL[3][NV], gR[3][NV];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                gL[d][v] = grad[(5 * d + v) * S + gg.x];
+                gR[d][v] = grad[(5 * d + v) * S + gg.y];
+            }
+        double muL, kL, muR, kR;
+        hx::laminar_mu_kappa(ph, qL, muL, kL);
+        hx::laminar_mu_kappa(ph, qR, muR, kR);
+        if (ph.les) {
+            const double dl = fdelta[f];
+            const double mtL = hx::les_viscosity(ph, dl, qL, gL[0], gL[1], gL[2]);
+            const double mtR = hx::les_viscosity(ph, dl, qR, gR[0], gR[1], gR[2]);
+            muL += mtL; kL += mtL * ph.mu_to_kappa;
+            muR += mtR; kR += mtR * ph.mu_to_kappa;
+        }
+        double FL[NV][3], FR[NV][3];
+        hx::viscous_flux(ph, qL, gL[0], gL[1], gL[2], muL, kL, FL);
+        hx::viscous_flux(ph, qR, gR[0], gR[1], gR[2], muR, kR, FR);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            visc[v] = 0.5 * (FL[v][0] + FR[v][0]) * nrm[0] + 0.5 * (FL[v][1] + FR[v][1]) * nrm[1] +
+                      0.5 * (FL[v][2] + FR[v][2]) * nrm[2];
+    }
+    double inv[NV];
+    if (ph.riemann == 1) hx::lxf_flux(ph, qL, qR, nrm, inv);
+    else hx::roe_flux(ph, qL, qR, nrm, inv);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) fstar[v * FS + fq] = (inv[v] - visc[v]) * jac;
+}
+
+// ------------------------------------------------------------------------------------
+// k_volume: volume integral + surface integral + 1/J + RK3 register update
+// ------------------------------------------------------------------------------------
+// Two-point volume fluxes (RiemannSolvers_NS.f90:2085-2438) contracted with the averaged
+// metric row; node data: q[5] conservative, p[4] = (u, v, w, p), ja[3].
+__device__ __forceinline__ void two_point(const Phys &ph, const double *qa, const double *pa,
+                                          const double *ja, const double *qb, const double *pb,
+                                          const double *jb, double fs[NV])
+{
+    switch (ph.averaging) {
+    case 1: {  // Ducros_TwoPointFlux
+        const double J0 = 0.5 * (ja[0] + jb[0]), J1 = 0.5 * (ja[1] + jb[1]), J2 = 0.5 * (ja[2] + jb[2]);
+        const double U = 0.25 * ((pa[0] + pb[0]) * J0 + (pa[1] + pb[1]) * J1 + (pa[2] + pb[2]) * J2);
+        const double P = 0.5 * (pa[3] + pb[3]);
+        fs[0] = (qa[0] + qb[0]) * U;
+        fs[1] = (qa[1] + qb[1]) * U + P * J0;
+        fs[2] = (qa[2] + qb[2]) * U + P * J1;
+        fs[3] = (qa[3] + qb[3]) * U + P * J2;
+        fs[4] = (qa[4] + pa[3] + qb[4] + pb[3]) * U;
+        return; }
+    case 2:    // KennedyGruber_TwoPointFlux
+    case 3: {  // Pirozzoli_TwoPointFlux
+        const double J0 = 0.5 * (ja[0] + jb[0]), J1 = 0.5 * (ja[1] + jb[1]), J2 = 0.5 * (ja[2] + jb[2]);
+        const double rho = 0.5 * (qa[0] + qb[0]);
+        const double u = 0.5 * (pa[0] + pb[0]), v = 0.5 * (pa[1] + pb[1]), w = 0.5 * (pa[2] + pb[2]);
+        const double p = 0.5 * (pa[3] + pb[3]);
+        const double U = u * J0 + v * J1 + w * J2;
+        const double m = rho * U;
+        fs[0] = m;
+        fs[1] = m * u + p * J0;
+        fs[2] = m * v + p * J1;
+        fs[3] = m * w + p * J2;
+        if (ph.averaging == 2)
+            fs[4] = m * 0.5 * (qa[4] / qa[0] + qb[4] / qb[0]) + p * U;
+        else
+            fs[4] = m * 0.5 * ((qa[4] + pa[3]) / qa[0] + (qb[4] + pb[3]) / qb[0]);
+        return; }
+    default: {  // StandardDG_TwoPointFlux (metrics summed, 1/4 applied at the end)
+        const double J0 = ja[0] + jb[0], J1 = ja[1] + jb[1], J2 = ja[2] + jb[2];
+        const double Ua = pa[0] * J0 + pa[1] * J1 + pa[2] * J2;
+        const double Ub = pb[0] * J0 + pb[1] * J1 + pb[2] * J2;
+        const double P = pa[3] + pb[3];
+        fs[0] = 0.25 * (qa[0] * Ua + qb[0] * Ub);
+        fs[1] = 0.25 * (qa[1] * Ua + qb[1] * Ub + P * J0);
+        fs[2] = 0.25 * (qa[2] * Ua + qb[2] * Ub + P * J1);
+        fs[3] = 0.25 * (qa[3] * Ua + qb[3] * Ub + P * J2);
+        fs[4] = 0.25 * ((qa[4] + pa[3]) * Ua + (qb[4] + pb[3]) * Ub);
+        return; }
+    }
+}
+
+template <int N, int EPB, bool SPLIT>
+__global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
+k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ Ja,
+         const double *__restrict__ invJ, const double *__restrict__ edelta,
+         const int *__restrict__ efn, const double *__restrict__ fstar, double *__restrict__ G,
+         double *__restrict__ qdot_out, const double *__restrict__ dt_dev, double dt_host,
+         double a_k, double c_k, int update, int K, long S, long FS, Ops ops, Phys ph)
+{
+    constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
+    // standard: sF[el][d][v][n] total contravariant flux; split: sF holds the viscous
+    // contravariant flux and sP the node data for the two-point fluxes.
+    __shared__ double sF[EPB][3][NV][ne];
+    __shared__ double sP[SPLIT ? EPB : 1][SPLIT ? 18 : 1][SPLIT ? ne : 1];
+    __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
+    const int nb = (K + EPB - 1) / EPB;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int t = threadIdx.x;
+    const int el = t / ne, n = t - el * ne;
+    const int e = lb * EPB + el;
+    const bool active = lb < nb && el < EPB && e < K;
+    for (int x = t; x < n1 * n1; x += blockDim.x) {
+        sHat[x] = ops.hatD[(x / n1) * 8 + (x % n1)];
+        sSharp[x] = ops.sharpD[(x / n1) * 8 + (x % n1)];
+    }
+    const long g = (long)e * ne + n;
+    double q[NV], ja[9], Finv[3][NV];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
+        hx::euler_contravariant(ph, q, ja, Finv);
+        double Fvc[3][NV];
+        if (!ph.euler) {
+            double gr[3][NV];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) gr[d][v] = grad[(5 * d + v) * S + g];
+            double mu, kappa;
+            hx::laminar_mu_kappa(ph, q, mu, kappa);
+            if (ph.les) {
+                const double mt = hx::les_viscosity(ph, edelta[e], q, gr[0], gr[1], gr[2]);
+                mu += mt;
+                kappa += mt * ph.mu_to_kappa;
+            }
+            double Fv[NV][3];
+            hx::viscous_flux(ph, q, gr[0], gr[1], gr[2], mu, kappa, Fv);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    Fvc[d][v] = Fv[v][0] * ja[3 * d] + Fv[v][1] * ja[3 * d + 1] + Fv[v][2] * ja[3 * d + 2];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fvc[d][v] = 0.0;
+        }
+        if (!SPLIT) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sF[el][d][v][n] = Finv[d][v] - Fvc[d][v];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sF[el][d][v][n] = Fvc[d][v];
+            const double ir = 1.0 / q[0];
+            const double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
+            const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
+#pragma unroll
+            for (int c = 0; c < NV; ++c) sP[el][c][n] = q[c];
+            sP[el][5][n] = u; sP[el][6][n] = v; sP[el][7][n] = w; sP[el][8][n] = p;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) sP[el][9 + c][n] = ja[c];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int i = n % n1, j = (n / n1) % n1, k = n / nf;
+    double qd[NV];
+    if (!SPLIT) {
+        // ScalarWeakIntegrals_StdVolumeGreen (DGIntegrals.f90:56-87)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qd[v] = 0.0;
+#pragma unroll
+        for (int l = 0; l < n1; ++l) {
+            const double hi = sHat[i * n1 + l], hj = sHat[j * n1 + l], hk = sHat[k * n1 + l];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                qd[v] += hi * sF[el][0][v][l + n1 * (j + n1 * k)] + hj * sF[el][1][v][i + n1 * (l + n1 * k)] +
+                         hk * sF[el][2][v][i + n1 * (j + n1 * l)];
+        }
+    } else {
+        // ScalarWeakIntegrals_SplitVolumeDivergence (DGIntegrals.f90:92-129), QDot = -volInt,
+        // two-point fluxes (SplitDG_ComputeSplitFormFluxes, HyperbolicSplitForm.f90:64-116)
+        // evaluated on the fly, diagonal = inviscid contravariant flux.
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        double qa[NV], pa[4];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) qa[c] = q[c];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pa[c] = sP[el][5 + c][n];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+#pragma unroll
+            for (int l = 0; l < n1; ++l) {
+                const int own = d == 0 ? i : (d == 1 ? j : k);
+                const int m = d == 0 ? l + n1 * (j + n1 * k) : (d == 1 ? i + n1 * (l + n1 * k) : i + n1 * (j + n1 * l));
+                const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
+                double fs[NV];
+                if (l == own) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) fs[v] = Finv[d][v];
+                } else {
+                    double qb[NV], pb[4], jb[3];
+#pragma unroll
+                    for (int c = 0; c < NV; ++c) qb[c] = sP[el][c][m];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) pb[c] = sP[el][5 + c][m];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) jb[c] = sP[el][9 + 3 * d + c][m];
+                    two_point(ph, qa, pa, &ja[3 * d], qb, pb, jb, fs);
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] += sh * fs[v] + hh * sF[el][d][v][m];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qd[v] = -acc[v];
+    }
+    // Surface integral (TimeDerivative_FacesContribution, SD.f90:1686-1701): for GL nodes only
+    // boundary nodes receive b = 1/w_0 times the stored face flux (negated on the right side).
+    const bool onface = (i == 0) | (i == N) | (j == 0) | (j == N) | (k == 0) | (k == N);
+    if (onface) {
+        const double b0 = 1.0 / ops.w[0];
+        double x[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        auto surf = [&](int slot, int a, int b) {
+            const int code = efn[((long)e * 6 + slot) * nf + a + n1 * b];
+            const long fq = code >> 1;
+            const double sg = (code & 1) ? -b0 : b0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) x[v] += sg * fstar[v * FS + fq];
+        };
+        if (i == 0) surf(5, j, k);
+        if (i == N) surf(3, j, k);
+        if (j == 0) surf(0, i, k);
+        if (j == N) surf(1, i, k);
+        if (k == 0) surf(2, i, j);
+        if (k == N) surf(4, i, j);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qd[v] -= x[v];
+    }
+    const double iJ = invJ[g];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) qd[v] *= iJ;
+    if (qdot_out) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qdot_out[v * S + g] = qd[v];
+    }
+    if (update) {
+        const double dt = dt_dev ? dt_dev[0] : dt_host;
+        const double cdt = c_k * dt;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double gv = a_k * G[v * S + g] + qd[v];
+            G[v * S + g] = gv;
+            Q[v * S + g] = q[v] + cdt * gv;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// reductions and small kernels
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double x)
+{
+    for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x)
+{
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// MaxTimeStep (DGSEMClass.f90:870-1035): per-node convective and viscous limits, block
+// minimum, then an unsigned 64-bit atomic minimum on the bit patterns (order-preserving for
+// non-negative doubles, so the result is exact and independent of scheduling).
+__global__ void __launch_bounds__(256)
+k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
+        long nodes, long S, double dcsi, double cfl, double dcfl, Phys ph,
+        unsigned long long *__restrict__ bits)
+{
+    __shared__ double sc[4], sv[4];
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double tc = INFINITY, tv = INFINITY;
+    if (g < nodes) {
+        double q[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
+        const double ir = 1.0 / q[0];
+        const double u = fabs(q[1] * ir), v = fabs(q[2] * ir), w = fabs(q[3] * ir);
+        const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * ir);
+        const double a = sqrt(ph.gamma * p * ir);
+        const double ev0 = u + a, ev1 = v + a, ev2 = w + a;
+        double ja[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
+        const double jac = 1.0 / invJ[g];
+        const double lx = fabs(ja[0] * ev0 + ja[1] * ev1 + ja[2] * ev2) * dcsi;
+        const double ly = fabs(ja[3] * ev0 + ja[4] * ev1 + ja[5] * ev2) * dcsi;
+        const double lz = fabs(ja[6] * ev0 + ja[7] * ev1 + ja[8] * ev2) * dcsi;
+        tc = cfl * fabs(jac) / (lx + ly + lz);
+        if (!ph.euler) {
+            const double T = ph.gammaM2 * p * ir;
+            const double tT = T * ph.Trenorm;
+            const double mu = (1.0 + ph.S_div_T) / (tT + ph.S_div_T) * tT * sqrt(tT);
+            const double d2 = dcsi * dcsi;
+            const double vx = mu * d2 * fabs(ja[0] + ja[1] + ja[2]);
+            const double vy = mu * d2 * fabs(ja[3] + ja[4] + ja[5]);
+            const double vz = mu * d2 * fabs(ja[6] + ja[7] + ja[8]);
+            tv = dcfl * fabs(jac) / (vx + vy + vz);
+        }
+    }
+    tc = wave_min(tc);
+    tv = wave_min(tv);
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) { sc[w] = tc; sv[w] = tv; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) { tc = fmin(tc, sc[x]); tv = fmin(tv, sv[x]); }
+        atomicMin(&bits[0], (unsigned long long)__double_as_longlong(tc));
+        atomicMin(&bits[1], (unsigned long long)__double_as_longlong(tv));
+    }
+}
+
+__global__ void k_dt_reset(unsigned long long *bits)
+{
+    if (threadIdx.x < 2) bits[threadIdx.x] = 0x7FF0000000000000ull;  // +inf
+}
+
+// dt = min(convective, viscous) (DGSEMClass.f90:1024-1030); with advance != 0 also
+// time += dt (device scalars: [0] dt, [1] t)
+__global__ void k_dt_finish(const unsigned long long *bits, double *sc, int advance)
+{
+    if (threadIdx.x == 0) {
+        const double a = __longlong_as_double((long long)bits[0]);
+        const double b = __longlong_as_double((long long)bits[1]);
+        const double dt = a < b ? a : b;
+        sc[0] = dt;
+        if (advance) sc[1] += dt;
+    }
+}
+
+// Per-block partial results in a fixed order, summed by k_finish_* in a fixed order, so
+// every reduction is deterministic.
+__global__ void __launch_bounds__(256)
+k_absmax(const double *__restrict__ X, long nodes, long S, double *__restrict__ part)
+{
+    __shared__ double sm[4][NV];
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double m[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) m[v] = g < nodes ? fabs(X[v * S + g]) : 0.0;
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        m[v] = wave_max(m[v]);
+        if (ln == 0) sm[w][v] = m[v];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double r = sm[0][threadIdx.x];
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) r = fmax(r, sm[x][threadIdx.x]);
+        part[(long)blockIdx.x * NV + threadIdx.x] = r;
+    }
+}
+
+__global__ void k_finish_max(const double *part, int nparts, double *out)
+{
+    if (threadIdx.x < NV) {
+        double r = 0.0;
+        for (int b = 0; b < nparts; ++b) r = fmax(r, part[(long)b * NV + threadIdx.x]);
+        out[threadIdx.x] = r;
+    }
+}
+
+// Kinetic energy: sum_e sum_n w_i w_j w_k J |rho u|^2 / (2 rho), and the volume
+__global__ void __launch_bounds__(256)
+k_kinetic(const double *__restrict__ Q, const double *__restrict__ invJ, const double *__restrict__ wq,
+          long nodes, long S, double *__restrict__ part)
+{
+    __shared__ double sk[4], sv[4];
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0, vol = 0.0;
+    if (g < nodes) {
+        const double J = 1.0 / invJ[g];
+        const double ww = wq[g];
+        const double r = Q[g], a = Q[S + g], b = Q[2 * S + g], c = Q[3 * S + g];
+        ke = ww * J * (0.5 * (a * a + b * b + c * c) / r);
+        vol = ww * J;
+    }
+    ke = wave_sum(ke);
+    vol = wave_sum(vol);
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) { sk[w] = ke; sv[w] = vol; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) { ke += sk[x]; vol += sv[x]; }
+        part[2 * blockIdx.x] = ke;
+        part[2 * blockIdx.x + 1] = vol;
+    }
+}
+
+__global__ void k_finish_sum2(const double *part, int nparts, double *out)
+{
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int x = 0; x < nparts; ++x) { a += part[2 * x]; b += part[2 * x + 1]; }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_nan(const double *__restrict__ Q, long nodes, long S, int *__restrict__ flag)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nodes) return;
+    bool bad = false;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) bad |= isnan(Q[v * S + g]);
+    if (bad) atomicOr(flag, 1);
+}
+
+// packed (nodes, 5) <-> SoA [5][S]
+__global__ void k_pack_in(const double *__restrict__ in, double *__restrict__ Q, long nodes, long S)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nodes) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Q[v * S + g] = in[NV * g + v];
+}
+__global__ void k_pack_out(const double *__restrict__ Q, double *__restrict__ out, long nodes, long S)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nodes) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[NV * g + v] = Q[v * S + g];
+}
+__global__ void k_fill(double *__restrict__ X, long n, double val)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) X[g] = val;
+}
+
+// ------------------------------------------------------------------------------------
+// host side: spectral operators (restated from NodalStorageClass.f90 for the engine)
+// ------------------------------------------------------------------------------------
+void gl_nodes(int N, double *x, double *w)
+{
+    // LegendreLobattoNodesAndWeights (LegendreAlgorithms.f90:275-355)
+    const double tol = 4.0 * 2.220446049250313e-16;
+    auto qL = [&](double xx, double &q, double &dq, double &LN) {
+        double a2 = 1.0, da2 = 0.0, a1 = xx, da1 = 1.0, Lk = 0, dLk = 0;
+        for (int k = 2; k <= N; ++k) {
+            Lk = ((2 * k - 1) * xx * a1 - (k - 1) * a2) / k;
+            dLk = da2 + (2 * k - 1) * a1;
+            a2 = a1; a1 = Lk; da2 = da1; da1 = dLk;
+        }
+        int k = N + 1;
+        Lk = ((2 * k - 1) * xx * a1 - (k - 1) * a2) / k;
+        dLk = da2 + (2 * k - 1) * a1;
+        q = Lk - a2; dq = dLk - da2; LN = a1;
+    };
+    if (N == 1) { x[0] = -1; x[1] = 1; w[0] = w[1] = 1; return; }
+    x[0] = -1.0; x[N] = 1.0; w[0] = w[N] = 2.0 / (N * (N + 1));
+    for (int j = 1; j <= (N + 1) / 2 - 1; ++j) {
+        double xj = -std::cos((j + 0.25) * M_PI / N - 3.0 / (8 * N * M_PI * (j + 0.25)));
+        double q, dq, LN;
+        for (int k = 0; k <= 10; ++k) {
+            qL(xj, q, dq, LN);
+            double d = -q / dq;
+            xj += d;
+            if (std::fabs(d) <= tol * std::fabs(xj)) break;
+        }
+        qL(xj, q, dq, LN);
+        x[j] = xj; w[j] = 2.0 / (N * (N + 1) * LN * LN);
+        x[N - j] = -xj; w[N - j] = w[j];
+    }
+    if (N % 2 == 0) {
+        double a2 = 1.0, a1 = 0.0, L = (N == 0) ? 1.0 : 0.0;
+        for (int k = 2; k <= N; ++k) { L = ((2 * k - 1) * 0.0 * a1 - (k - 1) * a2) / k; a2 = a1; a1 = L; }
+        x[N / 2] = 0.0; w[N / 2] = 2.0 / (N * (N + 1) * L * L);
+    }
+}
+
+Ops make_ops(int N)
+{
+    Ops o;
+    std::memset(&o, 0, sizeof(o));
+    double x[8], w[8], wb[8];
+    gl_nodes(N, x, w);
+    for (int j = 0; j <= N; ++j) wb[j] = 1.0;
+    for (int j = 1; j <= N; ++j)
+        for (int k = 0; k < j; ++k) { wb[k] *= (x[k] - x[j]); wb[j] *= (x[j] - x[k]); }
+    for (int j = 0; j <= N; ++j) wb[j] = 1.0 / wb[j];
+    double D[8][8];
+    for (int i = 0; i <= N; ++i) {
+        D[i][i] = 0.0;
+        for (int j = 0; j <= N; ++j)
+            if (j != i) { D[i][j] = wb[j] / (wb[i] * (x[i] - x[j])); D[i][i] -= D[i][j]; }
+    }
+    for (int i = 0; i <= N; ++i)
+        for (int j = 0; j <= N; ++j) {
+            o.D[i * 8 + j] = D[i][j];
+            o.hatD[i * 8 + j] = D[j][i] * w[j] / w[i];
+            o.sharpD[i * 8 + j] = 2.0 * D[i][j];
+        }
+    o.sharpD[0] = 2.0 * D[0][0] + 1.0 / w[0];
+    o.sharpD[N * 8 + N] = 2.0 * D[N][N] - 1.0 / w[N];
+    for (int i = 0; i <= N; ++i) o.w[i] = w[i];
+    return o;
+}
+
+void left_to_right(int i, int j, int Nx, int Ny, int rot, int &ii, int &jj)
+{
+    // leftIndexes2Right (MeshTypes.f90:70-108)
+    switch (rot) {
+    case 0: ii = i; jj = j; break;
+    case 1: ii = Ny - j; jj = i; break;
+    case 2: ii = Nx - i; jj = Ny - j; break;
+    case 3: ii = j; jj = Nx - i; break;
+    case 4: ii = j; jj = i; break;
+    case 5: ii = Nx - i; jj = j; break;
+    case 6: ii = Ny - j; jj = Nx - i; break;
+    default: ii = i; jj = Ny - j; break;
+    }
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) return eng->fail(HX3D_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct hx3d_engine {
+    int K = 0, N = 0, n1 = 0, nf = 0, ne = 0, F = 0;
+    long nodes = 0, S = 0, FS = 0;
+    Phys ph{};
+    Ops ops{};
+    int inviscid = 0;
+    double dcsi = 0;
+    hipStream_t stream = nullptr;
+    double *dQ = nullptr, *dG = nullptr, *dGrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
+    double *dFstar = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
+    double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr;
+    unsigned long long *dBits = nullptr;
+    int *dNbr = nullptr, *dEfn = nullptr, *dFlag = nullptr;
+    int2 *dFnode = nullptr;
+    unsigned char *dFslot = nullptr;
+    bool qdot_valid = false, timing = false;
+    double ktime[4] = {0, 0, 0, 0};
+    hipEvent_t ev[8] = {};
+    std::string err;
+
+    int fail(int code, const std::string &m) { err = m; return code; }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(hx3d_engine *eng, T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return eng->fail(HX3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// elements per block for the element kernels
+constexpr int epb_for(int N) { return N == 1 ? 16 : N == 2 ? 8 : N == 3 ? 4 : N == 4 ? 2 : 1; }
+
+template <int N>
+void launch_grad(hx3d_engine *eng)
+{
+    constexpr int EPB = epb_for(N), ne = (N + 1) * (N + 1) * (N + 1);
+    constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
+    const int nb = grid_for((eng->K + EPB - 1) / EPB);
+    hipLaunchKernelGGL((k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
+                       eng->dInvJ, eng->dNbr, eng->dGrad, eng->K, eng->S, eng->ops);
+}
+
+template <int N>
+void launch_face(hx3d_engine *eng)
+{
+    const long nb = (eng->FS + 255) / 256;
+    hipLaunchKernelGGL((k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dGrad, eng->dJa,
+                       eng->dFnode, eng->dFslot, eng->dFdelta, eng->dFstar, eng->FS, eng->S, eng->ph);
+}
+
+template <int N>
+void launch_volume(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev,
+                   double dt_host, double *qdot)
+{
+    constexpr int EPB = epb_for(N), ne = (N + 1) * (N + 1) * (N + 1);
+    constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
+    const int nb = grid_for((eng->K + EPB - 1) / EPB);
+    if (eng->inviscid == HX3D_INVISCID_SPLIT)
+        hipLaunchKernelGGL((k_volume<N, EPB, true>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
+                           eng->dGrad, eng->dJa, eng->dInvJ, eng->dEdelta, eng->dEfn, eng->dFstar,
+                           eng->dG, qdot, dt_dev, dt_host, a_k, c_k, update, eng->K, eng->S, eng->FS,
+                           eng->ops, eng->ph);
+    else
+        hipLaunchKernelGGL((k_volume<N, EPB, false>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
+                           eng->dGrad, eng->dJa, eng->dInvJ, eng->dEdelta, eng->dEfn, eng->dFstar,
+                           eng->dG, qdot, dt_dev, dt_host, a_k, c_k, update, eng->K, eng->S, eng->FS,
+                           eng->ops, eng->ph);
+}
+
+template <int N>
+void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev,
+             double dt_host, double *qdot, bool timed)
+{
+    if (timed) hipEventRecord(eng->ev[0], eng->stream);
+    if (!eng->ph.euler) launch_grad<N>(eng);
+    if (timed) hipEventRecord(eng->ev[1], eng->stream);
+    launch_face<N>(eng);
+    if (timed) hipEventRecord(eng->ev[2], eng->stream);
+    launch_volume<N>(eng, a_k, c_k, update, dt_dev, dt_host, qdot);
+    if (timed) hipEventRecord(eng->ev[3], eng->stream);
+}
+
+void stage(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev, double dt_host,
+           double *qdot, bool timed = false)
+{
+    switch (eng->N) {
+    case 1: stage_n<1>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    case 2: stage_n<2>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    case 3: stage_n<3>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    case 4: stage_n<4>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    case 5: stage_n<5>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    case 6: stage_n<6>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    default: stage_n<7>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    }
+}
+
+void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int advance)
+{
+    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, eng->dBits);
+    const long nb = (eng->nodes + 255) / 256;
+    hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dInvJ,
+                       eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dBits);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, eng->dBits, eng->dScal, advance);
+}
+
+// Williamson RK3 coefficients (ExplicitMethods.f90:697-699)
+const double RK_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+const double RK_C[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+const char *hx3d_last_error(const hx3d_engine *eng) { return eng ? eng->err.c_str() : "null engine"; }
+long hx3d_num_nodes(const hx3d_engine *eng) { return eng ? eng->nodes : -1; }
+
+int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
+{
+    if (!d || !out) return HX3D_EINVAL;
+    *out = nullptr;
+    hx3d_engine *eng = new hx3d_engine();
+    auto bail = [&](int code) { *out = eng; return code; };
+    if (d->K <= 0 || !d->N || !d->faceID || !d->faceSide || !d->fElem || !d->fESide || !d->fRot ||
+        !d->Ja || !d->J)
+        return bail(eng->fail(HX3D_EINVAL, "descriptor: missing arrays or K <= 0"));
+    const int N = d->N[0];
+    for (int e = 0; e < d->K; ++e)
+        for (int c = 0; c < 3; ++c)
+            if (d->N[3 * e + c] != N)
+                return bail(eng->fail(HX3D_EUNSUPPORTED, "plain engine needs one uniform order; mixed orders go to the batched engine"));
+    if (N < 1 || N > MAXN) return bail(eng->fail(HX3D_EUNSUPPORTED, "order must be 1..7"));
+    if (2L * d->F != 6L * d->K) return bail(eng->fail(HX3D_EUNSUPPORTED, "every element face must be an interior (periodic) face: need F = 3K"));
+    if (d->riemann < 0 || d->riemann > 1 || d->inviscid < 0 || d->inviscid > 1 || d->averaging < 0 ||
+        d->averaging > 4 || d->les < 0 || d->les > 2)
+        return bail(eng->fail(HX3D_EINVAL, "descriptor: selector out of range"));
+    if (d->inviscid == HX3D_INVISCID_SPLIT && d->averaging == HX3D_AVG_MORINISHI)
+        return bail(eng->fail(HX3D_EUNSUPPORTED, "split form with Morinishi averaging is not implemented"));
+    if (d->mach <= 0 || (!d->euler && d->reynolds <= 0))
+        return bail(eng->fail(HX3D_EINVAL, "descriptor: mach and reynolds must be positive"));
+    eng->K = d->K; eng->N = N; eng->n1 = N + 1; eng->nf = eng->n1 * eng->n1; eng->ne = eng->nf * eng->n1;
+    eng->F = d->F;
+    eng->nodes = (long)eng->K * eng->ne;
+    eng->S = ((eng->nodes + 63) / 64) * 64;
+    eng->FS = (long)eng->F * eng->nf;
+    eng->inviscid = d->inviscid;
+    eng->ops = make_ops(N);
+    {
+        double x[8], w[8];
+        gl_nodes(N, x, w);
+        eng->dcsi = 1.0 / std::fabs(x[1] - x[0]);
+    }
+    // physics constants (PhysicsStorage_NS.f90:145-439, ThermodynamicsAir :115-132)
+    Phys &ph = eng->ph;
+    ph.gamma = 1.4; ph.gm1 = 1.4 - 1.0;
+    ph.gammaM2 = ph.gamma * d->mach * d->mach;
+    ph.euler = d->euler ? 1 : 0;
+    ph.mu = d->euler ? 0.0 : 1.0 / d->reynolds;
+    ph.mu_to_kappa = d->euler ? 0.0 : 1.0 / (ph.gm1 * d->mach * d->mach * d->prandtl);
+    const double Tref = 520.0 * 5.0 / 9.0, Ssu = 198.6 * 5.0 / 9.0;
+    ph.S_div_T = Ssu / Tref;
+    ph.Trenorm = Tref / Tref;
+    ph.lambda_stab = d->lambda_stab;
+    ph.les_c = d->les_c;
+    ph.riemann = d->riemann;
+    ph.averaging = d->averaging;
+    ph.les = d->euler ? 0 : d->les;
+
+    const int K = eng->K, n1 = eng->n1, nf = eng->nf, ne = eng->ne, F = eng->F;
+    const long S = eng->S, nodes = eng->nodes, FS = eng->FS;
+    // ---- host tables --------------------------------------------------------------
+    auto node_of = [&](int e, int slot, int a, int b) -> long {
+        int idx[3];
+        idx[SLOT_AXIS[slot]] = SLOT_PLUS[slot] ? N : 0;
+        idx[SLOT_U[slot]] = a;
+        idx[SLOT_V[slot]] = b;
+        return (long)e * ne + idx[0] + n1 * (idx[1] + n1 * idx[2]);
+    };
+    std::vector<int> nbr((size_t)K * 6 * nf, -1), efn((size_t)K * 6 * nf, -1);
+    std::vector<int2> fnode((size_t)FS);
+    std::vector<unsigned char> fslot(F);
+    for (int f = 0; f < F; ++f) {
+        const int eL = d->fElem[2 * f], eR = d->fElem[2 * f + 1];
+        const int sL = d->fESide[2 * f] - 1, sR = d->fESide[2 * f + 1] - 1, rot = d->fRot[f];
+        if (eL < 0 || eL >= K || eR < 0 || eR >= K || sL < 0 || sL > 5 || sR < 0 || sR > 5 || rot < 0 || rot > 7)
+            return bail(eng->fail(HX3D_EINVAL, "face " + std::to_string(f) + ": bad element/side/rotation"));
+        if (d->faceID[6 * eL + sL] != f || d->faceSide[6 * eL + sL] != 1 || d->faceID[6 * eR + sR] != f ||
+            d->faceSide[6 * eR + sR] != 2)
+            return bail(eng->fail(HX3D_EINVAL, "face " + std::to_string(f) + ": inconsistent with element faceID/faceSide"));
+        fslot[f] = (unsigned char)sL;
+        for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= N; ++i) {
+                int ii, jj;
+                left_to_right(i, j, N, N, rot, ii, jj);
+                const long gL = node_of(eL, sL, i, j), gR = node_of(eR, sR, ii, jj);
+                const long q = (long)f * nf + i + n1 * j;
+                fnode[q] = make_int2((int)gL, (int)gR);
+                nbr[((size_t)eL * 6 + sL) * nf + i + n1 * j] = (int)gR;
+                nbr[((size_t)eR * 6 + sR) * nf + ii + n1 * jj] = (int)gL;
+                efn[((size_t)eL * 6 + sL) * nf + i + n1 * j] = (int)(2 * q);
+                efn[((size_t)eR * 6 + sR) * nf + ii + n1 * jj] = (int)(2 * q + 1);
+            }
+    }
+    for (size_t x = 0; x < nbr.size(); ++x)
+        if (nbr[x] < 0 || efn[x] < 0) return bail(eng->fail(HX3D_EINVAL, "an element face node has no neighbour"));
+    if (nodes > 0x7fffffffL || 2 * FS > 0x7fffffffL)
+        return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit node indices"));
+    // geometry SoA, 1/J, quadrature weights, LES filter widths
+    std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), wq((size_t)S, 0.0), edelta(K), fdelta(F);
+    double xg[8], wg[8];
+    gl_nodes(N, xg, wg);
+    for (long g = 0; g < nodes; ++g) {
+        for (int c = 0; c < 9; ++c) ja[(size_t)c * S + g] = d->Ja[9 * g + c];
+        ij[g] = 1.0 / d->J[g];
+        const int n = (int)(g % ne);
+        wq[g] = wg[n % n1] * wg[(n / n1) % n1] * wg[n / nf];
+    }
+    for (int e = 0; e < K; ++e) {
+        double vol = 0.0;
+        for (int n = 0; n < ne; ++n) vol += wq[(long)e * ne + n] * d->J[(long)e * ne + n];
+        edelta[e] = std::pow(vol / ne, 1.0 / 3.0);   // SD.f90:420
+    }
+    for (int f = 0; f < F; ++f) {
+        const int sL = fslot[f];
+        const int dax = SLOT_AXIS[sL];
+        double surf = 0.0;
+        for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= N; ++i) {
+                const long g = fnode[(long)f * nf + i + n1 * j].x;
+                const double a = d->Ja[9 * g + 3 * dax], b = d->Ja[9 * g + 3 * dax + 1], c = d->Ja[9 * g + 3 * dax + 2];
+                surf += wg[i] * wg[j] * std::sqrt(a * a + b * b + c * c);
+            }
+        fdelta[f] = std::sqrt(surf / nf);   // SD.f90:1378
+    }
+    // ---- device -------------------------------------------------------------------
+    int rc;
+    if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
+        (rc = dalloc(eng, &eng->dGrad, (size_t)15 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
+        (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
+        (rc = dalloc(eng, &eng->dFstar, (size_t)NV * FS)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
+        (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) || (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) ||
+        (rc = dalloc(eng, &eng->dScal, 4)) || (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
+        (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2)) ||
+        (rc = dalloc(eng, &eng->dNbr, nbr.size())) || (rc = dalloc(eng, &eng->dEfn, efn.size())) ||
+        (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())) ||
+        (rc = dalloc(eng, &eng->dFslot, fslot.size())))
+        return bail(rc);
+    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(eng->fail(HX3D_EDEVICE, "hipStreamCreate failed"));
+    for (auto &e : eng->ev) hipEventCreate(&e);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(eng->dJa, ja.data(), ja.size() * 8) || !up(eng->dInvJ, ij.data(), ij.size() * 8) ||
+        !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
+        !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dNbr, nbr.data(), nbr.size() * 4) ||
+        !up(eng->dEfn, efn.data(), efn.size() * 4) || !up(eng->dFnode, fnode.data(), fnode.size() * 8) ||
+        !up(eng->dFslot, fslot.data(), fslot.size()))
+        return bail(eng->fail(HX3D_EDEVICE, "upload failed"));
+    // zero state, RK register and time
+    const long nb = (NV * S + 255) / 256;
+    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, (long)NV * S, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dG, (long)NV * S, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, eng->stream, eng->dScal, 4L, 0.0);
+    if (hipStreamSynchronize(eng->stream) != hipSuccess)
+        return bail(eng->fail(HX3D_EDEVICE, "initialisation failed"));
+    *out = eng;
+    return HX3D_OK;
+}
+
+int hx3d_engine_destroy(hx3d_engine *eng)
+{
+    if (!eng) return HX3D_OK;
+    if (eng->stream) hipStreamSynchronize(eng->stream);
+    void *ptrs[] = {eng->dQ, eng->dG, eng->dGrad, eng->dJa, eng->dInvJ, eng->dFstar, eng->dQDot,
+                    eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
+                    eng->dBits, eng->dNbr, eng->dEfn, eng->dFlag, eng->dFnode, eng->dFslot};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto &e : eng->ev)
+        if (e) hipEventDestroy(e);
+    if (eng->stream) hipStreamDestroy(eng->stream);
+    delete eng;
+    return HX3D_OK;
+}
+
+int hx3d_set_state_device(hx3d_engine *eng, const double *Qd)
+{
+    if (!eng || !Qd) return HX3D_EINVAL;
+    const long nb = (eng->nodes + 255) / 256;
+    hipLaunchKernelGGL(k_pack_in, dim3(nb), dim3(256), 0, eng->stream, Qd, eng->dQ, eng->nodes, eng->S);
+    // a new state restarts the RK register (the reference allocates G_NS zeroed)
+    const long nb2 = (NV * eng->S + 255) / 256;
+    hipLaunchKernelGGL(k_fill, dim3(nb2), dim3(256), 0, eng->stream, eng->dG, (long)NV * eng->S, 0.0);
+    eng->qdot_valid = false;
+    HIPCHK(hipGetLastError());
+    return HX3D_OK;
+}
+
+int hx3d_get_state_device(hx3d_engine *eng, double *Qd)
+{
+    if (!eng || !Qd) return HX3D_EINVAL;
+    const long nb = (eng->nodes + 255) / 256;
+    hipLaunchKernelGGL(k_pack_out, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, Qd, eng->nodes, eng->S);
+    HIPCHK(hipGetLastError());
+    return HX3D_OK;
+}
+
+int hx3d_set_state(hx3d_engine *eng, const double *Qh)
+{
+    if (!eng || !Qh) return HX3D_EINVAL;
+    HIPCHK(hipMemcpyAsync(eng->dStage, Qh, sizeof(double) * NV * eng->nodes, hipMemcpyHostToDevice, eng->stream));
+    int rc = hx3d_set_state_device(eng, eng->dStage);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_get_state(hx3d_engine *eng, double *Qh)
+{
+    if (!eng || !Qh) return HX3D_EINVAL;
+    int rc = hx3d_get_state_device(eng, eng->dStage);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(Qh, eng->dStage, sizeof(double) * NV * eng->nodes, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host)
+{
+    (void)t;   // no source terms on this path (UserDefinedSourceTermNS is zero)
+    if (!eng) return HX3D_EINVAL;
+    stage(eng, 0.0, 0.0, 0, nullptr, 0.0, eng->dQDot);
+    HIPCHK(hipGetLastError());
+    eng->qdot_valid = true;
+    if (QDot_host) {
+        const long nb = (eng->nodes + 255) / 256;
+        hipLaunchKernelGGL(k_pack_out, dim3(nb), dim3(256), 0, eng->stream, eng->dQDot, eng->dStage, eng->nodes, eng->S);
+        HIPCHK(hipMemcpyAsync(QDot_host, eng->dStage, sizeof(double) * NV * eng->nodes, hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+    }
+    return HX3D_OK;
+}
+
+int hx3d_rk3_step(hx3d_engine *eng, double t, double dt)
+{
+    (void)t;
+    if (!eng) return HX3D_EINVAL;
+    for (int k = 0; k < 3; ++k) stage(eng, RK_A[k], RK_C[k], 1, nullptr, dt, k == 2 ? eng->dQDot : nullptr);
+    eng->qdot_valid = false;
+    HIPCHK(hipGetLastError());
+    return HX3D_OK;
+}
+
+int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt)
+{
+    if (!eng || !dt) return HX3D_EINVAL;
+    launch_maxdt(eng, cfl, dcfl, 0);
+    HIPCHK(hipMemcpyAsync(dt, eng->dScal, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *t_inout)
+{
+    if (!eng || nsteps < 0) return HX3D_EINVAL;
+    if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
+    double acc[3] = {0, 0, 0};
+    for (int s = 0; s < nsteps; ++s) {
+        launch_maxdt(eng, cfl, dcfl, 1);
+        for (int k = 0; k < 3; ++k) {
+            const bool timed = eng->timing;
+            stage(eng, RK_A[k], RK_C[k], 1, eng->dScal, 0.0, nullptr, timed);
+            if (timed) {
+                HIPCHK(hipEventSynchronize(eng->ev[3]));
+                float a, b, c;
+                hipEventElapsedTime(&a, eng->ev[0], eng->ev[1]);
+                hipEventElapsedTime(&b, eng->ev[1], eng->ev[2]);
+                hipEventElapsedTime(&c, eng->ev[2], eng->ev[3]);
+                acc[0] += a; acc[1] += b; acc[2] += c;
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    if (eng->timing && nsteps > 0)
+        for (int x = 0; x < 3; ++x) eng->ktime[x] = acc[x] / (3.0 * nsteps);
+    eng->qdot_valid = false;
+    if (t_inout) {
+        HIPCHK(hipMemcpyAsync(t_inout, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+    }
+    return HX3D_OK;
+}
+
+int hx3d_residuals(hx3d_engine *eng, double res[5])
+{
+    if (!eng || !res) return HX3D_EINVAL;
+    if (!eng->qdot_valid) {
+        int rc = hx3d_compute_qdot(eng, 0.0, nullptr);
+        if (rc) return rc;
+    }
+    const int nb = (int)((eng->nodes + 255) / 256);
+    hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(256), 0, eng->stream, eng->dQDot, eng->nodes, eng->S, eng->dPart);
+    hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(64), 0, eng->stream, eng->dPart, nb, eng->dStage);
+    HIPCHK(hipMemcpyAsync(res, eng->dStage, sizeof(double) * NV, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_kinetic_energy(hx3d_engine *eng, double *ke)
+{
+    if (!eng || !ke) return HX3D_EINVAL;
+    const int nb = (int)((eng->nodes + 255) / 256);
+    hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dInvJ, eng->dWq, eng->nodes, eng->S, eng->dPart);
+    hipLaunchKernelGGL(k_finish_sum2, dim3(1), dim3(64), 0, eng->stream, eng->dPart, nb, eng->dStage);
+    double r[2];
+    HIPCHK(hipMemcpyAsync(r, eng->dStage, sizeof(r), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    *ke = r[0] / r[1];
+    return HX3D_OK;
+}
+
+int hx3d_check_nan(hx3d_engine *eng, int *has_nan)
+{
+    if (!eng || !has_nan) return HX3D_EINVAL;
+    int zero = 0;
+    HIPCHK(hipMemcpyAsync(eng->dFlag, &zero, sizeof(int), hipMemcpyHostToDevice, eng->stream));
+    const long nb = (eng->nodes + 255) / 256;
+    hipLaunchKernelGGL(k_nan, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->nodes, eng->S, eng->dFlag);
+    HIPCHK(hipMemcpyAsync(has_nan, eng->dFlag, sizeof(int), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_synchronize(hx3d_engine *eng)
+{
+    if (!eng) return HX3D_EINVAL;
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_set_timing(hx3d_engine *eng, int on)
+{
+    if (!eng) return HX3D_EINVAL;
+    eng->timing = on != 0;
+    return HX3D_OK;
+}
+
+int hx3d_kernel_times(hx3d_engine *eng, double out_ms[4])
+{
+    if (!eng || !out_ms) return HX3D_EINVAL;
+    for (int x = 0; x < 4; ++x) out_ms[x] = eng->ktime[x];
+    return HX3D_OK;
+}
+
+}  // extern "C"

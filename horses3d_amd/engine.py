@@ -1,0 +1,183 @@
+"""Python host binding of the hx3d C ABI (include/hx3d.h, horses3d_amd/libhx3d.so).
+
+The engine is the drop-in for the reference's ComputeTimeDerivative / TakeRK3Step
+(Solver/src/NavierStokesSolver/SpatialDiscretization.f90:227-320,
+Solver/src/libs/timeintegrator/ExplicitMethods.f90:667-788).  A *case* is a dict of numpy
+arrays in the reference's HexMesh element/face storage contract (see include/hx3d.h); the
+case builders in horses3d_amd.mesh produce one.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is visible, every call
+raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhx3d.so")
+
+RIEMANN = {"roe": 0, "lax-friedrichs": 1, "lxf": 1}
+INVISCID = {"standard": 0, "split-form": 1}
+AVERAGING = {"standard": 0, "ducros": 1, "kennedy-gruber": 2, "pirozzoli": 3, "morinishi": 4}
+LES = {None: 0, "none": 0, "smagorinsky": 1, "wale": 2}
+LES_DEFAULT_C = {0: 0.0, 1: 0.2, 2: 0.325}   # LESModels.f90:248, :352
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [("K", C.c_int), ("N", C.c_void_p), ("faceID", C.c_void_p), ("faceSide", C.c_void_p),
+                ("F", C.c_int), ("fElem", C.c_void_p), ("fESide", C.c_void_p), ("fRot", C.c_void_p),
+                ("Ja", C.c_void_p), ("J", C.c_void_p),
+                ("mach", C.c_double), ("reynolds", C.c_double), ("prandtl", C.c_double),
+                ("euler", C.c_int), ("riemann", C.c_int), ("inviscid", C.c_int),
+                ("averaging", C.c_int), ("lambda_stab", C.c_double), ("les", C.c_int),
+                ("les_c", C.c_double)]
+
+
+class HX3DError(RuntimeError):
+    pass
+
+
+_lib = None
+
+EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
+           "hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device",
+           "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_max_dt", "hx3d_advance", "hx3d_residuals",
+           "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_set_timing",
+           "hx3d_kernel_times"]
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP engine library (does not touch the GPU)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HX3DError("hx3d HIP library not built: %s (run __graft_entry__.build())" % path)
+    L = C.CDLL(path)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    L.hx3d_engine_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)]
+    L.hx3d_engine_destroy.argtypes = [P]
+    L.hx3d_last_error.argtypes = [P]
+    L.hx3d_last_error.restype = C.c_char_p
+    L.hx3d_num_nodes.argtypes = [P]
+    L.hx3d_num_nodes.restype = C.c_long
+    for fn in ("hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device"):
+        getattr(L, fn).argtypes = [P, P]
+    L.hx3d_compute_qdot.argtypes = [P, D, P]
+    L.hx3d_rk3_step.argtypes = [P, D, D]
+    L.hx3d_max_dt.argtypes = [P, D, D, C.POINTER(D)]
+    L.hx3d_advance.argtypes = [P, I, D, D, C.POINTER(D)]
+    L.hx3d_residuals.argtypes = [P, P]
+    L.hx3d_kinetic_energy.argtypes = [P, C.POINTER(D)]
+    L.hx3d_check_nan.argtypes = [P, C.POINTER(I)]
+    L.hx3d_synchronize.argtypes = [P]
+    L.hx3d_set_timing.argtypes = [P, I]
+    L.hx3d_kernel_times.argtypes = [P, P]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Engine:
+    """Device-resident DGSEM case.  State arrays on the host are (nodes, 5) float64."""
+
+    def __init__(self, case, mach, reynolds, prandtl=0.72, riemann="roe", inviscid="standard",
+                 averaging="standard", les=None, les_c=None, lambda_stab=1.0, euler=False):
+        L = load_library()
+        self._keep = dict(
+            N=np.ascontiguousarray(case["N"], dtype=np.int32),
+            faceID=np.ascontiguousarray(case["faceID"], dtype=np.int32),
+            faceSide=np.ascontiguousarray(case["faceSide"], dtype=np.int32),
+            fElem=np.ascontiguousarray(case["fElem"], dtype=np.int32),
+            fESide=np.ascontiguousarray(case["fESide"], dtype=np.int32),
+            fRot=np.ascontiguousarray(case["fRot"], dtype=np.int32),
+            Ja=np.ascontiguousarray(case["Ja"], dtype=np.float64),
+            J=np.ascontiguousarray(case["J"], dtype=np.float64))
+        k = self._keep
+        les_id = LES[les]
+        d = CaseDesc(int(case["K"]), _p(k["N"]), _p(k["faceID"]), _p(k["faceSide"]), int(case["F"]),
+                     _p(k["fElem"]), _p(k["fESide"]), _p(k["fRot"]), _p(k["Ja"]), _p(k["J"]),
+                     mach, reynolds, prandtl, 1 if euler else 0, RIEMANN[riemann],
+                     INVISCID[inviscid], AVERAGING[averaging], lambda_stab, les_id,
+                     LES_DEFAULT_C[les_id] if les_c is None else les_c)
+        h = C.c_void_p()
+        rc = L.hx3d_engine_create(C.byref(d), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = L.hx3d_last_error(h).decode() if h else "create failed"
+            L.hx3d_engine_destroy(h)
+            self.h = None
+            raise HX3DError("hx3d_engine_create: %s (code %d)" % (msg, rc))
+        self.nodes = int(L.hx3d_num_nodes(self.h))
+        self.t = 0.0
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise HX3DError("%s: %s (code %d)" % (what, load_library().hx3d_last_error(self.h).decode(), rc))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().hx3d_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        assert Q.shape == (self.nodes, 5), Q.shape
+        self._chk(load_library().hx3d_set_state(self.h, _p(Q)), "hx3d_set_state")
+
+    def get_state(self):
+        out = np.empty((self.nodes, 5))
+        self._chk(load_library().hx3d_get_state(self.h, _p(out)), "hx3d_get_state")
+        return out
+
+    def qdot(self, t=0.0):
+        out = np.empty((self.nodes, 5))
+        self._chk(load_library().hx3d_compute_qdot(self.h, t, _p(out)), "hx3d_compute_qdot")
+        return out
+
+    def rk3_step(self, t, dt):
+        self._chk(load_library().hx3d_rk3_step(self.h, t, dt), "hx3d_rk3_step")
+
+    def max_dt(self, cfl, dcfl):
+        dt = C.c_double()
+        self._chk(load_library().hx3d_max_dt(self.h, cfl, dcfl, C.byref(dt)), "hx3d_max_dt")
+        return dt.value
+
+    def advance(self, nsteps, cfl, dcfl, t=None):
+        tt = C.c_double(self.t if t is None else t)
+        self._chk(load_library().hx3d_advance(self.h, nsteps, cfl, dcfl, C.byref(tt)), "hx3d_advance")
+        self.t = tt.value
+        return self.t
+
+    def residuals(self):
+        r = np.zeros(5)
+        self._chk(load_library().hx3d_residuals(self.h, _p(r)), "hx3d_residuals")
+        return r
+
+    def kinetic_energy(self):
+        ke = C.c_double()
+        self._chk(load_library().hx3d_kinetic_energy(self.h, C.byref(ke)), "hx3d_kinetic_energy")
+        return ke.value
+
+    def has_nan(self):
+        f = C.c_int()
+        self._chk(load_library().hx3d_check_nan(self.h, C.byref(f)), "hx3d_check_nan")
+        return bool(f.value)
+
+    def synchronize(self):
+        self._chk(load_library().hx3d_synchronize(self.h), "hx3d_synchronize")
+
+    def set_timing(self, on):
+        self._chk(load_library().hx3d_set_timing(self.h, 1 if on else 0), "hx3d_set_timing")
+
+    def kernel_times(self):
+        out = np.zeros(4)
+        self._chk(load_library().hx3d_kernel_times(self.h, _p(out)), "hx3d_kernel_times")
+        return out

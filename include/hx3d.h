@@ -1,0 +1,122 @@
+/*
+ * hx3d -- MI355X-native DGSEM time-derivative engine for HORSES3D's compressible
+ * Navier-Stokes solver.  C ABI (plain pointers and sizes; no C++ or torch types).
+ *
+ * The engine replaces two reference procedures:
+ *   - ComputeTimeDerivative(mesh, particles, time, mode)
+ *       Solver/src/NavierStokesSolver/SpatialDiscretization.f90:227-320, interface
+ *       ComputeTimeDerivative_f in Solver/src/libs/discretization/DGSEMClass.f90:77-97
+ *       -> hx3d_compute_qdot
+ *   - TakeRK3Step(mesh, particles, t, deltaT, ComputeTimeDerivative)
+ *       Solver/src/libs/timeintegrator/ExplicitMethods.f90:667-788 (RKStepFcn,
+ *       TimeIntegrator.f90:77) -> hx3d_rk3_step
+ * together with the step-loop helpers the time integrator calls around them:
+ *   - MaxTimeStep (DGSEMClass.f90:870-1035)               -> hx3d_max_dt
+ *   - ComputeMaxResiduals (DGSEMClass.f90:770-860)        -> hx3d_residuals
+ *   - checkForNan (ExplicitMethods.f90:1856-1905)         -> hx3d_check_nan
+ *   - kinetic-energy volume monitor (VolumeIntegrals.f90:179-195,
+ *     VolumeMonitor.f90:300)                              -> hx3d_kinetic_energy
+ *   - the IntegrateInTime step loop (TimeIntegrator.f90:737-812) for CFL-bounded
+ *     explicit RK3                                        -> hx3d_advance
+ *
+ * Storage contract (the reference's HexMesh element/face storage, StorageClass.f90):
+ *   element e (0-based) has orders N[3e..3e+2]; its node n = i + (Nx+1)*(j + (Ny+1)*k)
+ *   is global node g = sum_{e'<e} (Nx'+1)(Ny'+1)(Nz'+1) + n; host state arrays are packed
+ *   per node, variable fastest, Q[5*g + v] (the reference's QNS layout,
+ *   StorageClass.f90:138-173).  Element face slots 0..5 are EFRONT, EBACK, EBOTTOM,
+ *   ERIGHT, ETOP, ELEFT (MeshTypes.f90:17-18).  Faces follow Face (FaceClass.f90:63-101):
+ *   left/right element, the element face slot (1..6) each side sits on, and the
+ *   rotation 0..7 (leftIndexes2Right, MeshTypes.f90:70-108).
+ *
+ * Status codes: 0 = OK, > 0 = error (text from hx3d_last_error).  No exceptions or
+ * process exits cross the ABI.  One host thread per engine at a time; engines own their
+ * device memory and their HIP stream.
+ */
+#ifndef HX3D_H
+#define HX3D_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HX3D_OK 0
+#define HX3D_EINVAL 1
+#define HX3D_EUNSUPPORTED 2
+#define HX3D_EDEVICE 3
+#define HX3D_ENOMEM 4
+#define HX3D_ENAN 5
+
+/* selectors (keys of the reference's .control file) */
+enum { HX3D_RIEMANN_ROE = 0, HX3D_RIEMANN_LXF = 1 };                    /* "riemann solver" */
+enum { HX3D_INVISCID_STANDARD = 0, HX3D_INVISCID_SPLIT = 1 };           /* "inviscid discretization" */
+enum { HX3D_AVG_STANDARD = 0, HX3D_AVG_DUCROS = 1, HX3D_AVG_KENNEDYGRUBER = 2,
+       HX3D_AVG_PIROZZOLI = 3, HX3D_AVG_MORINISHI = 4 };                 /* "averaging" */
+enum { HX3D_LES_NONE = 0, HX3D_LES_SMAGORINSKY = 1, HX3D_LES_WALE = 2 }; /* "les model" */
+
+typedef struct {
+    /* mesh (ConstructMesh_FromSpecMeshFile_ / HexMesh, HexMesh.f90:71-180) */
+    int K;                  /* number of elements */
+    const int *N;           /* [3K] polynomial orders (Nx, Ny, Nz) per element */
+    const int *faceID;      /* [6K] 0-based face of each element face slot */
+    const int *faceSide;    /* [6K] 1 = element is the face's left side, 2 = right */
+    int F;                  /* number of (interior, periodic-paired) faces */
+    const int *fElem;       /* [2F] 0-based left, right element */
+    const int *fESide;      /* [2F] element face slot 1..6 of the left, right element */
+    const int *fRot;        /* [F] rotation 0..7 */
+    /* geometry (MappedGeometry.f90:64-386), per global node g */
+    const double *Ja;       /* [9*nodes] Ja^d_c at 9g + 3d + c (d = xi, eta, zeta) */
+    const double *J;        /* [nodes] Jacobian */
+    /* physics (ConstructPhysicsStorage_NS, PhysicsStorage_NS.f90:145-439) */
+    double mach, reynolds, prandtl;
+    int euler;              /* 1: "flow equations" = Euler (no viscous terms) */
+    /* discretisation (Initialize_SpaceAndTimeMethods, SpatialDiscretization.f90:44-210) */
+    int riemann, inviscid, averaging;
+    double lambda_stab;     /* "lambda stabilization" (LxF), default 1 */
+    int les;                /* "les model" */
+    double les_c;           /* "les model intensity" (Smagorinsky 0.2, WALE 0.325) */
+} hx3d_case_desc;
+
+typedef struct hx3d_engine hx3d_engine;
+
+/* Creates an engine on the current HIP device: uploads geometry, builds the
+ * element/face gather tables.  Supported: Gauss-Lobatto nodes, uniform order
+ * 1 <= N <= 7 in all elements and directions, conforming faces. */
+int hx3d_engine_create(const hx3d_case_desc *desc, hx3d_engine **out);
+int hx3d_engine_destroy(hx3d_engine *eng);
+const char *hx3d_last_error(const hx3d_engine *eng);
+long hx3d_num_nodes(const hx3d_engine *eng);
+
+/* State transfer in the packed host layout Q[5g+v] (LoadSolutionForRestart / the QNS copy). */
+int hx3d_set_state(hx3d_engine *eng, const double *Q_host);
+int hx3d_get_state(hx3d_engine *eng, double *Q_host);
+/* The same on device memory (packed layout, stream-ordered on the engine stream). */
+int hx3d_set_state_device(hx3d_engine *eng, const double *Q_dev);
+int hx3d_get_state_device(hx3d_engine *eng, double *Q_dev);
+
+/* Drop-in for ComputeTimeDerivative: QDot(Q, t) of the resident state.  QDot_host may be
+ * NULL (the derivative then stays on the device for hx3d_residuals). */
+int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host);
+/* Drop-in for TakeRK3Step: one Williamson RK3 step of size dt on the resident state. */
+int hx3d_rk3_step(hx3d_engine *eng, double t, double dt);
+/* MaxTimeStep(cfl, dcfl) of the resident state. */
+int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt);
+/* The step loop: nsteps RK3 steps, each with dt = MaxTimeStep(cfl, dcfl) of the state at
+ * the start of the step (computed on the device; no host round trip).  t_inout advances. */
+int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *t_inout);
+/* ComputeMaxResiduals: max |QDot| per variable of the last computed derivative. */
+int hx3d_residuals(hx3d_engine *eng, double res[5]);
+/* Kinetic-energy monitor of the resident state. */
+int hx3d_kinetic_energy(hx3d_engine *eng, double *ke);
+/* checkForNan: *has_nan = 1 if any state value is NaN. */
+int hx3d_check_nan(hx3d_engine *eng, int *has_nan);
+/* Blocks until the engine's queued work is done. */
+int hx3d_synchronize(hx3d_engine *eng);
+/* Average duration (ms) of each stage kernel over the last hx3d_advance call when timing
+ * was enabled with hx3d_set_timing(eng, 1): out[0..2] = gradient, face flux, volume+RK. */
+int hx3d_set_timing(hx3d_engine *eng, int on);
+int hx3d_kernel_times(hx3d_engine *eng, double out_ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,39 @@
+import ast
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (REPO, os.path.join(REPO, "oracle"), os.path.join(REPO, "tests", "golden")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+GOLDEN = sorted(glob.glob(os.path.join(REPO, "tests", "golden", "*.npz")))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
+
+
+def load_golden(path):
+    z = dict(np.load(path, allow_pickle=False))
+    meta = ast.literal_eval(str(z.pop("meta")))
+    case = {k: (int(v) if k in ("K", "F") else v) for k, v in z.items()}
+    return case, meta
+
+
+def physics_kwargs(meta):
+    """Engine/oracle keyword arguments from a fixture's control settings."""
+    les = meta.get("les")
+    return dict(mach=meta["mach"], reynolds=meta["re"], riemann=meta.get("riemann", "roe"),
+                inviscid=meta.get("inviscid", "standard"),
+                averaging=meta.get("averaging", "standard").lower(),
+                les=les.lower() if les else None, les_c=meta.get("les_intensity"))
+
+
+@pytest.fixture(params=GOLDEN, ids=lambda p: os.path.basename(p)[:-4])
+def golden(request):
+    return load_golden(request.param)

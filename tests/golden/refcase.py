@@ -228,8 +228,8 @@ def read_dt_log(path):
 
 
 def read_volume_monitor(case_dir):
-    """Kinetic energy monitor written by the reference (RESULTS/case.KinEn.volume)."""
-    p = os.path.join(case_dir, "RESULTS", "case.KinEn.volume")
+    """Kinetic energy monitor written by the reference (RESULTS/case.kinen.volume)."""
+    p = os.path.join(case_dir, "RESULTS", "case.kinen.volume")
     rows = []
     for l in open(p):
         t = l.split()

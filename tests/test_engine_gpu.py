@@ -1,0 +1,109 @@
+"""HIP engine parity (through the C ABI) against the reference's golden vectors and the
+oracle.  Tolerance: max relative error <= 1e-12 per conserved variable, normwise
+(max_i |a - b| / max_i |b|), per stage; kinetic energy after 100 steps within 1e-10."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import load_golden, physics_kwargs, GOLDEN
+from horses3d_amd import mesh
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def engine_for(case, kw):
+    from horses3d_amd.engine import Engine
+    return Engine(case, **kw)
+
+
+def oracle_for(case, kw):
+    return po.Oracle(case, po.make_params(kw["mach"], kw["reynolds"], riemann=kw["riemann"],
+                                          inviscid=kw["inviscid"], averaging=kw["averaging"],
+                                          les=kw["les"], les_c=kw["les_c"]))
+
+
+def test_qdot_matches_reference(golden):
+    case, meta = golden
+    eng = engine_for(case, physics_kwargs(meta))
+    for tag in [k[2:] for k in case if k.startswith("Q_step") or k == "Q_final"]:
+        eng.set_state(case["Q_" + tag])
+        err = po.normwise_error(eng.qdot(), case["QDot_" + tag])
+        print(tag, err, flush=True)
+        assert err.max() <= TOL, (tag, err)
+    eng.close()
+
+
+def test_rk3_step_and_dt_match_oracle(golden):
+    case, meta = golden
+    kw = physics_kwargs(meta)
+    eng, ora = engine_for(case, kw), oracle_for(case, kw)
+    Q0 = case["Q_step0"]
+    eng.set_state(Q0)
+    dt = eng.max_dt(0.4, 0.4)
+    assert dt == pytest.approx(case["dt_log"][0, 2], rel=1e-13)
+    eng.rk3_step(0.0, dt)
+    Qo, G = Q0.copy(), np.zeros_like(Q0)
+    ora.rk3_step(Qo, G, 0.0, dt)
+    err = po.normwise_error(eng.get_state() - Q0, Qo - Q0)
+    assert err.max() <= TOL, err
+    eng.close()
+
+
+def test_config1_100_steps_kinetic_energy():
+    """config 1: 4^3, N=3, Roe/BR1/standard, CFL 0.4, 100 device-side dt steps."""
+    case, meta = load_golden([p for p in GOLDEN if "c1_tgv" in p][0])
+    eng = engine_for(case, physics_kwargs(meta))
+    eng.set_state(case["Q_step0"])
+    t = eng.advance(10, 0.4, 0.4, t=0.0)
+    assert po.normwise_error(eng.get_state(), case["Q_step10"]).max() < 1e-12
+    t = eng.advance(90, 0.4, 0.4)
+    assert t == pytest.approx(float(case["t_final"]), rel=1e-13)
+    Q = eng.get_state()
+    assert po.normwise_error(Q, case["Q_final"]).max() < 1e-11
+    ke = eng.kinetic_energy()
+    assert abs(ke / case["ke_monitor"][-1, 2] - 1.0) < 1e-10
+    res = eng.residuals()
+    assert np.allclose(res, np.abs(case["QDot_final"]).max(axis=0), rtol=1e-11, atol=0)
+    assert not eng.has_nan()
+    eng.close()
+
+
+@pytest.mark.parametrize("inviscid,averaging,riemann,les", [
+    ("split-form", "pirozzoli", "lxf", None),
+    ("standard", "standard", "roe", None),
+    ("split-form", "kennedy-gruber", "roe", "wale"),
+])
+def test_config2_size_matches_oracle(inviscid, averaging, riemann, les):
+    """config 2 size (16^3 x N=4, curvilinear) with a perturbed Taylor-Green state."""
+    case = mesh.periodic_box(16, 4, amplitude=0.1, phase_seed=2)
+    Q = mesh.taylor_green(case, 0.1, noise=1e-3, seed=22)
+    kw = dict(mach=0.1, reynolds=1600.0, riemann=riemann, inviscid=inviscid, averaging=averaging,
+              les=les, les_c=None)
+    eng, ora = engine_for(case, kw), oracle_for(case, kw)
+    eng.set_state(Q)
+    err = po.normwise_error(eng.qdot(), ora.qdot(Q))
+    print(err, flush=True)
+    assert err.max() <= TOL
+    eng.close()
+
+
+def test_conservation_and_nan_flag():
+    """Periodic telescoping: sum_e sum_n w J QDot = 0 per variable; a planted NaN is found."""
+    case = mesh.periodic_box(8, 5, amplitude=0.15, phase_seed=5)
+    Q = mesh.taylor_green(case, 0.2, noise=1e-2, seed=3)
+    eng = engine_for(case, dict(mach=0.2, reynolds=1600.0, riemann="roe", inviscid="split-form",
+                                averaging="pirozzoli", les="wale", les_c=None))
+    eng.set_state(Q)
+    qd = eng.qdot()
+    w = mesh.gl_nodes(5)[1]
+    W = np.einsum("k,j,i->kji", w, w, w).ravel()
+    wJ = (case["J"].reshape(case["K"], -1) * W).ravel()
+    tot = (wJ[:, None] * qd).sum(axis=0)
+    scale = (wJ[:, None] * np.abs(qd)).sum(axis=0)
+    assert np.all(np.abs(tot) <= 1e-12 * scale), tot / scale
+    assert not eng.has_nan()
+    Q[1234, 2] = np.nan
+    eng.set_state(Q)
+    assert eng.has_nan()
+    eng.close()

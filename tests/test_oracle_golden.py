@@ -1,0 +1,61 @@
+"""The oracle (C restatement) against the reference's own golden vectors.
+
+The fixtures were written by the reference solver built from its own sources
+(tests/golden/make_golden.py); they pin the composed time derivative (every operator of
+ComputeTimeDerivative, SpatialDiscretization.f90:227-320), MaxTimeStep
+(DGSEMClass.f90:870-1035), the RK3 step (ExplicitMethods.f90:667-788) and the kinetic-energy
+monitor (VolumeIntegrals.f90:179-195).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import load_golden, physics_kwargs, GOLDEN
+
+
+def make_oracle(case, meta):
+    kw = physics_kwargs(meta)
+    return po.Oracle(case, po.make_params(kw["mach"], kw["reynolds"], riemann=kw["riemann"],
+                                          inviscid=kw["inviscid"], averaging=kw["averaging"],
+                                          les=kw["les"], les_c=kw["les_c"]))
+
+
+def test_qdot_matches_reference(golden):
+    case, meta = golden
+    o = make_oracle(case, meta)
+    tags = [k[2:] for k in case if k.startswith("Q_step") or k == "Q_final"]
+    assert tags
+    for tag in tags:
+        err = po.normwise_error(o.qdot(case["Q_" + tag]), case["QDot_" + tag])
+        assert err.max() <= 1e-13, (tag, err)
+
+
+def test_max_dt_matches_reference(golden):
+    case, meta = golden
+    o = make_oracle(case, meta)
+    # dt_log row k: (step, time, dt) printed before step k from the state after k steps
+    dt0 = o.max_dt(case["Q_step0"], 0.4, 0.4)
+    assert dt0 == pytest.approx(case["dt_log"][0, 2], rel=1e-14)
+
+
+def test_rk3_trajectory_matches_reference():
+    """config 1 (4^3, N=3, Roe, BR1, standard form): 100 CFL-bounded RK3 steps."""
+    path = [p for p in GOLDEN if "c1_tgv" in p][0]
+    case, meta = load_golden(path)
+    o = make_oracle(case, meta)
+    Q = case["Q_step0"].copy()
+    G = np.zeros_like(Q)
+    t = 0.0
+    dts = case["dt_log"]
+    for s in range(int(case["steps"])):
+        dt = o.max_dt(Q, 0.4, 0.4)
+        assert dt == pytest.approx(dts[s, 2], rel=1e-12), s
+        o.rk3_step(Q, G, t, dt)
+        t += dt
+        if s + 1 == 10:
+            assert po.normwise_error(Q, case["Q_step10"]).max() < 1e-13
+    assert po.normwise_error(Q, case["Q_final"]).max() < 1e-12
+    ke = o.kinetic_energy(Q)
+    ke_ref = case["ke_monitor"][-1, 2]
+    assert abs(ke / ke_ref - 1.0) < 1e-10
+    assert abs(t / case["t_final"] - 1.0) < 1e-13
