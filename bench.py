@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark: DOF-updates/s (FP64) of the plain GPU engine on one MI355X.
+
+Workload (default): the single-GPU case the metric is quoted on -- periodic box
+[-pi,pi]^3 of 16^3 curvilinear hexahedra (x' = x + 0.1 prod_k sin(x_k + phi_dk), phases
+from seed 2), Gauss-Lobatto N=4 (512,000 nodes), Taylor-Green vortex at Mach 0.1, Re 1600,
+split-form kinetic-energy-preserving volume flux (Pirozzoli averaging), Lax-Friedrichs face
+flux, BR1 viscous fluxes, low-storage RK3 with dt from CFL 0.4 every step.  One "step" is one
+full RK3 step (three time-derivative stages plus MaxTimeStep); one DOF-update is one node
+advanced one step (the reference's convention, main.f90:404-409).
+
+Prints ONE JSON line.  `roofline` is for the dominant kernel (the fused volume/surface/RK
+kernel) from its HIP-event-timed average launch duration; `cpu_baseline` times the reference
+OpenMP build (oracle/_ref, built from the reference's sources) on a bounded sample when it is
+present, otherwise the C restatement.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6
+
+
+def alg_bytes(N):
+    """Compulsory FP64 traffic of the two-sweep DGSEM+BR1 stage (bytes):
+    per node  - gradient sweep: Q(5) Ja(9) 1/J(1) in, grad(15) out  = 240 B
+              - flux sweep (volume kernel): Q(5) grad(15) Ja(9) J(1) G(5) in, G(5) Q(5) out = 360 B
+    per face node - gradient sweep: both traces of Q (10), n and |Ja| (4) = 112 B
+                  - flux sweep (face kernel): both traces of Q and grad (40), n t1 t2 |Ja| (10) = 400 B
+    with 3/(N+1) face nodes per node on a conforming hexahedral mesh."""
+    fpn = 3.0 / (N + 1)
+    return dict(grad=240.0 + 112.0 * fpn, face=400.0 * fpn, volume=360.0,
+                stage=600.0 + 512.0 * fpn)
+
+
+def cpu_baseline_reference(n, N, steps, threads):
+    """Time the reference solver (oracle/_ref/bin/horses3d.ns) on the same box, its own
+    'Solver efficiency' line (seconds per million DOF and iteration, main.f90:404-409)."""
+    exe = os.path.join(REPO, "oracle", "_ref", "bin", "horses3d.ns")
+    if not os.path.exists(exe):
+        return None
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+    import refcase as rc
+    work = tempfile.mkdtemp(prefix="hx3d_cpu_")
+    # straight-sided box: the per-step work of the reference does not depend on the mapping
+    rc.write_specmesh(os.path.join(work, "box.mesh"), n)
+    rc.write_control(os.path.join(work, "case.control"), "box.mesh", N, steps, 0.1, 1600.0,
+                     riemann="lax-friedrichs", inviscid="split-form", averaging="Pirozzoli",
+                     ctd_after=False)
+    t0 = time.time()
+    out = rc.run_reference(work, threads=threads, timeout=900)
+    wall = time.time() - t0
+    m = re.search(r"Solver efficiency:\s*([0-9.Ee+-]+)", out)
+    if not m:
+        return None
+    eff = float(m.group(1))   # seconds / (1e6 DOF * iteration)
+    return dict(value=1e6 / eff, unit="DOF-updates/s", cores=threads, kind="reference",
+                sample="reference OpenMP build (amdflang -O2 from the reference sources), "
+                       "%d^3 straight periodic box, N=%d, split-form Pirozzoli + LxF + BR1, %d RK3 steps, "
+                       "OMP_NUM_THREADS=%d; its own 'Solver efficiency' line (%.3e s/(MDOF*iter)), "
+                       "wall %.1f s incl. setup" % (n, N, steps, threads, eff, wall))
+
+
+def cpu_baseline_port(case, kw, steps):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import pyoracle as po
+    from horses3d_amd import mesh
+    o = po.Oracle(case, po.make_params(kw["mach"], kw["reynolds"], riemann=kw["riemann"],
+                                       inviscid=kw["inviscid"], averaging=kw["averaging"]))
+    Q = mesh.taylor_green(case, kw["mach"])
+    G = np.zeros_like(Q)
+    t0 = time.time()
+    for _ in range(steps):
+        dt = o.max_dt(Q, 0.4, 0.4)
+        o.rk3_step(Q, G, 0.0, dt)
+    el = time.time() - t0
+    nd = mesh.ndof(case)
+    return dict(value=nd * steps / el, unit="DOF-updates/s", cores=1, kind="port",
+                sample="C restatement (oracle), %d nodes, %d RK3 steps, 1 thread" % (nd, steps))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=16, help="elements per box side")
+    ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--amplitude", type=float, default=0.1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X and the "
+              "multi-GPU path is not part of this build (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from horses3d_amd import mesh
+    from horses3d_amd.engine import Engine
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    torch.cuda.set_device(0)
+    print("[bench] building case %d^3 N=%d" % (args.n, args.order), file=sys.stderr, flush=True)
+    case = mesh.periodic_box(args.n, args.order, amplitude=args.amplitude, phase_seed=2)
+    kw = dict(mach=0.1, reynolds=1600.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli")
+    Q0 = mesh.taylor_green(case, kw["mach"])
+    nd = mesh.ndof(case)
+    eng = Engine(case, **kw)
+    eng.set_state(Q0)
+    print("[bench] warmup %d steps" % args.warmup, file=sys.stderr, flush=True)
+    eng.advance(args.warmup, 0.4, 0.4, t=0.0)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.advance(args.steps, 0.4, 0.4)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = nd * args.steps / el
+    print("[bench] timed %d steps in %.4f s" % (args.steps, el), file=sys.stderr, flush=True)
+    assert not eng.has_nan()
+
+    # per-kernel HIP-event timing over a second pass of the same steps
+    eng.set_timing(True)
+    eng.advance(args.steps, 0.4, 0.4)
+    kt = eng.kernel_times()   # ms per launch: grad, face, volume, stage
+    eng.set_timing(False)
+    B = alg_bytes(args.order)
+    FS = case["F"] * (args.order + 1) ** 2
+    kern = {
+        "grad": dict(ms=kt[0], bytes=B["grad"] * nd),
+        "face": dict(ms=kt[1], bytes=400.0 * FS),
+        "volume": dict(ms=kt[2], bytes=B["volume"] * nd),
+    }
+    for k in kern.values():
+        k["GBs"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic_config2.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        traffic = tj.get("k_volume_bytes_per_launch") if dom == "volume" else tj.get("k_%s_bytes_per_launch" % dom)
+    roof = dict(bound="hbm", kernel="k_" + dom, achieved=round(kern[dom]["GBs"], 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(kern[dom]["GBs"] / HBM_PEAK_GBS, 4), traffic=traffic,
+                bytes_per_launch=kern[dom]["bytes"], avg_launch_ms=round(kern[dom]["ms"], 5),
+                stage_ms=round(kt[3], 5),
+                stage_achieved_GBs=round(B["stage"] * nd / (kt[3] * 1e-3) / 1e9, 1),
+                stage_frac=round(B["stage"] * nd / (kt[3] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                kernels={k: dict(avg_launch_ms=round(v["ms"], 5), GBs=round(v["GBs"], 1)) for k, v in kern.items()})
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        try:
+            cpu = cpu_baseline_reference(args.n, args.order, args.cpu_steps, args.cpu_threads)
+        except Exception as e:   # the reference build is optional on a box
+            print("[bench] reference CPU baseline failed: %s" % e, file=sys.stderr, flush=True)
+        if cpu is None:
+            small = mesh.periodic_box(8, args.order, amplitude=args.amplitude, phase_seed=2)
+            cpu = cpu_baseline_port(small, kw, 3)
+
+    line = {
+        "metric": "DOF-updates/s (FP64)",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green vortex on a seeded curvilinear periodic box)",
+        "config": {"workload": "plain DGSEM compressible NS, %d^3 curvilinear hexes x N=%d (%d nodes), "
+                               "split-form Pirozzoli + LxF + BR1, LS-RK3, CFL 0.4"
+                               % (args.n, args.order, nd),
+                   "elements": case["K"], "order": args.order, "ndof": nd, "mach": 0.1,
+                   "reynolds": 1600.0, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

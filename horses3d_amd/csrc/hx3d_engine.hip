@@ -64,7 +64,8 @@ inline int grid_for(int nb) { return ((nb + 7) / 8) * 8; }
 template <int N, int EPB>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
 k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
-       const int *__restrict__ nbr, double *__restrict__ grad, int K, long S, Ops ops)
+       const int *__restrict__ nbr, const int *__restrict__ efn, const double *__restrict__ fgeo,
+       double *__restrict__ grad, int K, long S, long FS, Ops ops)
 {
     constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
     __shared__ double sQ[EPB][NV][ne];
@@ -109,9 +110,10 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
         gr[1][v] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
         gr[2][v] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
     }
-    // BR1 lifting (BR1_ComputeElementInterfaceAverage + BR1_GradientFaceLoop): on each face
-    // the node lies on, unStar = 1/2 (U_nbr - U_own) |Ja| n_out, the same value for both
-    // sides; the GL surface integral weights it by b = 1/w_0.
+    // BR1 lifting (BR1_ComputeElementInterfaceAverage, EllipticBR1.f90:571-627, and
+    // BR1_GradientFaceLoop, :531-569): on each face the node lies on,
+    // unStar = 1/2 (U_R - U_L) |Ja| n with the face frame of the left element, the same value
+    // for both sides; the GL surface integral weights it by b = 1/w_0.
     const bool onface = (i == 0) | (i == N) | (j == 0) | (j == N) | (k == 0) | (k == N);
     if (onface) {
         const double b0 = 1.0 / ops.w[0];
@@ -120,27 +122,33 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
         for (int d = 0; d < 3; ++d)
 #pragma unroll
             for (int v = 0; v < NV; ++v) fi[d][v] = 0.0;
-        auto lift = [&](int slot, int a, int b, double sgn, int dax) {
-            const int gn = nbr[((long)e * 6 + slot) * nf + a + n1 * b];
-            const double s0 = sgn * ja[3 * dax], s1 = sgn * ja[3 * dax + 1], s2 = sgn * ja[3 * dax + 2];
+        auto lift = [&](int slot, int a, int b) {
+            const long x = ((long)e * 6 + slot) * nf + a + n1 * b;
+            const int gn = nbr[x];
+            const int code = efn[x];
+            const long fq = code >> 1;
+            const bool right = code & 1;
+            const double jac = fgeo[9 * FS + fq];
+            const double n0 = fgeo[fq], n1_ = fgeo[FS + fq], n2 = fgeo[2 * FS + fq];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const double us = 0.5 * (Q[v * S + gn] - q[v]) * b0;
-                fi[0][v] += us * s0;
-                fi[1][v] += us * s1;
-                fi[2][v] += us * s2;
+                const double qn = Q[v * S + gn];
+                const double uS = 0.5 * (right ? q[v] - qn : qn - q[v]) * jac;
+                fi[0][v] = fi[0][v] + (uS * n0) * b0;
+                fi[1][v] = fi[1][v] + (uS * n1_) * b0;
+                fi[2][v] = fi[2][v] + (uS * n2) * b0;
             }
         };
-        if (i == 0) lift(5, j, k, -1.0, 0);
-        if (i == N) lift(3, j, k, 1.0, 0);
-        if (j == 0) lift(0, i, k, -1.0, 1);
-        if (j == N) lift(1, i, k, 1.0, 1);
-        if (k == 0) lift(2, i, j, -1.0, 2);
-        if (k == N) lift(4, i, j, 1.0, 2);
+        if (i == 0) lift(5, j, k);
+        if (i == N) lift(3, j, k);
+        if (j == 0) lift(0, i, k);
+        if (j == N) lift(1, i, k);
+        if (k == 0) lift(2, i, j);
+        if (k == N) lift(4, i, j);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) gr[d][v] += fi[d][v] * iJ;
+            for (int v = 0; v < NV; ++v) gr[d][v] = gr[d][v] + fi[d][v] * iJ;
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -153,30 +161,31 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
 // ------------------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(256)
-k_face(const double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ Ja,
-       const int2 *__restrict__ fnode, const unsigned char *__restrict__ fslot,
-       const double *__restrict__ fdelta, double *__restrict__ fstar, long FS, long S, Phys ph)
+k_face(const double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ fgeo,
+       const int2 *__restrict__ fnode, const double *__restrict__ fdelta, double *__restrict__ fstar,
+       long FS, long S, Phys ph)
 {
     constexpr int n1 = N + 1, nf = n1 * n1;
     const long fq = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (fq >= FS) return;
     const long f = fq / nf;
     const int2 gg = fnode[fq];
-    const int sl = fslot[f];
-    const int dax = sl == 3 || sl == 5 ? 0 : (sl <= 1 ? 1 : 2);
-    const double sgn = (sl == 1 || sl == 3 || sl == 4) ? 1.0 : -1.0;
-    double nrm[3];
+    // face frame (unit normal, tangents, |Ja|): fgeo[c*FS + fq], c = n(0..2), t1(3..5), t2(6..8), jac(9)
+    double nrm[3], t1[3], t2[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) nrm[c] = sgn * Ja[(3 * dax + c) * S + gg.x];
-    const double jac = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
-    const double ijac = 1.0 / jac;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) nrm[c] *= ijac;
+    for (int c = 0; c < 3; ++c) {
+        nrm[c] = fgeo[c * FS + fq];
+        t1[c] = fgeo[(3 + c) * FS + fq];
+        t2[c] = fgeo[(6 + c) * FS + fq];
+    }
+    const double jac = fgeo[9 * FS + fq];
     double qL[NV], qR[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { qL[v] = Q[v * S + gg.x]; qR[v] = Q[v * S + gg.y]; }
     double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (!ph.euler) {
+        // BR1_RiemannSolver (EllipticBR1.f90:816-868) with the face viscosities of
+        // compute_viscosity_at_faces (SpatialDiscretization.f90:1345-1398)
         double gL[3][NV], gR[3][NV];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -192,8 +201,8 @@ k_face(const double *__restrict__ Q, const double *__restrict__ grad, const doub
             const double dl = fdelta[f];
             const double mtL = hx::les_viscosity(ph, dl, qL, gL[0], gL[1], gL[2]);
             const double mtR = hx::les_viscosity(ph, dl, qR, gR[0], gR[1], gR[2]);
-            muL += mtL; kL += mtL * ph.mu_to_kappa;
-            muR += mtR; kR += mtR * ph.mu_to_kappa;
+            muL = muL + mtL; kL = kL + mtL * ph.mu_to_kappa;
+            muR = muR + mtR; kR = kR + mtR * ph.mu_to_kappa;
         }
         double FL[NV][3], FR[NV][3];
         hx::viscous_flux(ph, qL, gL[0], gL[1], gL[2], muL, kL, FL);
@@ -204,7 +213,7 @@ k_face(const double *__restrict__ Q, const double *__restrict__ grad, const doub
                       0.5 * (FL[v][2] + FR[v][2]) * nrm[2];
     }
     double inv[NV];
-    if (ph.riemann == 1) hx::lxf_flux(ph, qL, qR, nrm, inv);
+    if (ph.riemann == 1) hx::lxf_flux(ph, qL, qR, nrm, t1, t2, inv);
     else hx::roe_flux(ph, qL, qR, nrm, inv);
 #pragma unroll
     for (int v = 0; v < NV; ++v) fstar[v * FS + fq] = (inv[v] - visc[v]) * jac;
@@ -213,58 +222,10 @@ k_face(const double *__restrict__ Q, const double *__restrict__ grad, const doub
 // ------------------------------------------------------------------------------------
 // k_volume: volume integral + surface integral + 1/J + RK3 register update
 // ------------------------------------------------------------------------------------
-// Two-point volume fluxes (RiemannSolvers_NS.f90:2085-2438) contracted with the averaged
-// metric row; node data: q[5] conservative, p[4] = (u, v, w, p), ja[3].
-__device__ __forceinline__ void two_point(const Phys &ph, const double *qa, const double *pa,
-                                          const double *ja, const double *qb, const double *pb,
-                                          const double *jb, double fs[NV])
-{
-    switch (ph.averaging) {
-    case 1: {  // Ducros_TwoPointFlux
-        const double J0 = 0.5 * (ja[0] + jb[0]), J1 = 0.5 * (ja[1] + jb[1]), J2 = 0.5 * (ja[2] + jb[2]);
-        const double U = 0.25 * ((pa[0] + pb[0]) * J0 + (pa[1] + pb[1]) * J1 + (pa[2] + pb[2]) * J2);
-        const double P = 0.5 * (pa[3] + pb[3]);
-        fs[0] = (qa[0] + qb[0]) * U;
-        fs[1] = (qa[1] + qb[1]) * U + P * J0;
-        fs[2] = (qa[2] + qb[2]) * U + P * J1;
-        fs[3] = (qa[3] + qb[3]) * U + P * J2;
-        fs[4] = (qa[4] + pa[3] + qb[4] + pb[3]) * U;
-        return; }
-    case 2:    // KennedyGruber_TwoPointFlux
-    case 3: {  // Pirozzoli_TwoPointFlux
-        const double J0 = 0.5 * (ja[0] + jb[0]), J1 = 0.5 * (ja[1] + jb[1]), J2 = 0.5 * (ja[2] + jb[2]);
-        const double rho = 0.5 * (qa[0] + qb[0]);
-        const double u = 0.5 * (pa[0] + pb[0]), v = 0.5 * (pa[1] + pb[1]), w = 0.5 * (pa[2] + pb[2]);
-        const double p = 0.5 * (pa[3] + pb[3]);
-        const double U = u * J0 + v * J1 + w * J2;
-        const double m = rho * U;
-        fs[0] = m;
-        fs[1] = m * u + p * J0;
-        fs[2] = m * v + p * J1;
-        fs[3] = m * w + p * J2;
-        if (ph.averaging == 2)
-            fs[4] = m * 0.5 * (qa[4] / qa[0] + qb[4] / qb[0]) + p * U;
-        else
-            fs[4] = m * 0.5 * ((qa[4] + pa[3]) / qa[0] + (qb[4] + pb[3]) / qb[0]);
-        return; }
-    default: {  // StandardDG_TwoPointFlux (metrics summed, 1/4 applied at the end)
-        const double J0 = ja[0] + jb[0], J1 = ja[1] + jb[1], J2 = ja[2] + jb[2];
-        const double Ua = pa[0] * J0 + pa[1] * J1 + pa[2] * J2;
-        const double Ub = pb[0] * J0 + pb[1] * J1 + pb[2] * J2;
-        const double P = pa[3] + pb[3];
-        fs[0] = 0.25 * (qa[0] * Ua + qb[0] * Ub);
-        fs[1] = 0.25 * (qa[1] * Ua + qb[1] * Ub + P * J0);
-        fs[2] = 0.25 * (qa[2] * Ua + qb[2] * Ub + P * J1);
-        fs[3] = 0.25 * (qa[3] * Ua + qb[3] * Ub + P * J2);
-        fs[4] = 0.25 * ((qa[4] + pa[3]) * Ua + (qb[4] + pb[3]) * Ub);
-        return; }
-    }
-}
-
 template <int N, int EPB, bool SPLIT>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
 k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ Ja,
-         const double *__restrict__ invJ, const double *__restrict__ edelta,
+         const double *__restrict__ Jac, const double *__restrict__ edelta,
          const int *__restrict__ efn, const double *__restrict__ fstar, double *__restrict__ G,
          double *__restrict__ qdot_out, const double *__restrict__ dt_dev, double dt_host,
          double a_k, double c_k, int update, int K, long S, long FS, Ops ops, Phys ph)
@@ -273,7 +234,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *
     // standard: sF[el][d][v][n] total contravariant flux; split: sF holds the viscous
     // contravariant flux and sP the node data for the two-point fluxes.
     __shared__ double sF[EPB][3][NV][ne];
-    __shared__ double sP[SPLIT ? EPB : 1][SPLIT ? 18 : 1][SPLIT ? ne : 1];
+    __shared__ double sP[SPLIT ? EPB : 1][SPLIT ? 19 : 1][SPLIT ? ne : 1];
     __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
     const int nb = (K + EPB - 1) / EPB;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -330,14 +291,15 @@ k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) sF[el][d][v][n] = Fvc[d][v];
+            // node data of the two-point fluxes (invRho, u, v, w, p as RiemannSolvers_NS.f90:2323-2329)
             const double ir = 1.0 / q[0];
-            const double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
+            const double u = ir * q[1], v = ir * q[2], w = ir * q[3];
             const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
 #pragma unroll
             for (int c = 0; c < NV; ++c) sP[el][c][n] = q[c];
-            sP[el][5][n] = u; sP[el][6][n] = v; sP[el][7][n] = w; sP[el][8][n] = p;
+            sP[el][5][n] = u; sP[el][6][n] = v; sP[el][7][n] = w; sP[el][8][n] = p; sP[el][9][n] = ir;
 #pragma unroll
-            for (int c = 0; c < 9; ++c) sP[el][9 + c][n] = ja[c];
+            for (int c = 0; c < 9; ++c) sP[el][10 + c][n] = ja[c];
         }
     }
     __syncthreads();
@@ -345,29 +307,37 @@ k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *
     const int i = n % n1, j = (n / n1) % n1, k = n / nf;
     double qd[NV];
     if (!SPLIT) {
-        // ScalarWeakIntegrals_StdVolumeGreen (DGIntegrals.f90:56-87)
+        // ScalarWeakIntegrals_StdVolumeGreen (DGIntegrals.f90:56-87): xi, then eta, then zeta sums
 #pragma unroll
         for (int v = 0; v < NV; ++v) qd[v] = 0.0;
 #pragma unroll
         for (int l = 0; l < n1; ++l) {
-            const double hi = sHat[i * n1 + l], hj = sHat[j * n1 + l], hk = sHat[k * n1 + l];
+            const double hi = sHat[i * n1 + l];
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                qd[v] += hi * sF[el][0][v][l + n1 * (j + n1 * k)] + hj * sF[el][1][v][i + n1 * (l + n1 * k)] +
-                         hk * sF[el][2][v][i + n1 * (j + n1 * l)];
+            for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hi * sF[el][0][v][l + n1 * (j + n1 * k)];
+        }
+#pragma unroll
+        for (int l = 0; l < n1; ++l) {
+            const double hj = sHat[j * n1 + l];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hj * sF[el][1][v][i + n1 * (l + n1 * k)];
+        }
+#pragma unroll
+        for (int l = 0; l < n1; ++l) {
+            const double hk = sHat[k * n1 + l];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hk * sF[el][2][v][i + n1 * (j + n1 * l)];
         }
     } else {
         // ScalarWeakIntegrals_SplitVolumeDivergence (DGIntegrals.f90:92-129), QDot = -volInt,
         // two-point fluxes (SplitDG_ComputeSplitFormFluxes, HyperbolicSplitForm.f90:64-116)
-        // evaluated on the fly, diagonal = inviscid contravariant flux.
+        // evaluated on the fly; the diagonal is the inviscid contravariant flux.
         double acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-        double qa[NV], pa[4];
+        double pa[5];
 #pragma unroll
-        for (int c = 0; c < NV; ++c) qa[c] = q[c];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) pa[c] = sP[el][5 + c][n];
+        for (int c = 0; c < 5; ++c) pa[c] = sP[el][5 + c][n];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
 #pragma unroll
@@ -380,17 +350,17 @@ k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *
 #pragma unroll
                     for (int v = 0; v < NV; ++v) fs[v] = Finv[d][v];
                 } else {
-                    double qb[NV], pb[4], jb[3];
+                    double qb[NV], pb[5], jb[3];
 #pragma unroll
                     for (int c = 0; c < NV; ++c) qb[c] = sP[el][c][m];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) pb[c] = sP[el][5 + c][m];
+                    for (int c = 0; c < 5; ++c) pb[c] = sP[el][5 + c][m];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) jb[c] = sP[el][9 + 3 * d + c][m];
-                    two_point(ph, qa, pa, &ja[3 * d], qb, pb, jb, fs);
+                    for (int c = 0; c < 3; ++c) jb[c] = sP[el][10 + 3 * d + c][m];
+                    hx::two_point_flux(ph, q, pa, &ja[3 * d], qb, pb, jb, fs);
                 }
 #pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v] += sh * fs[v] + hh * sF[el][d][v][m];
+                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + sh * fs[v] + hh * sF[el][d][v][m];
             }
         }
 #pragma unroll
@@ -407,7 +377,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *
             const long fq = code >> 1;
             const double sg = (code & 1) ? -b0 : b0;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) x[v] += sg * fstar[v * FS + fq];
+            for (int v = 0; v < NV; ++v) x[v] = x[v] + fstar[v * FS + fq] * sg;
         };
         if (i == 0) surf(5, j, k);
         if (i == N) surf(3, j, k);
@@ -416,11 +386,11 @@ k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *
         if (k == 0) surf(2, i, j);
         if (k == N) surf(4, i, j);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) qd[v] -= x[v];
+        for (int v = 0; v < NV; ++v) qd[v] = qd[v] - x[v];
     }
-    const double iJ = invJ[g];
+    const double jg = Jac[g];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) qd[v] *= iJ;
+    for (int v = 0; v < NV; ++v) qd[v] = qd[v] / jg;
     if (qdot_out) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) qdot_out[v * S + g] = qd[v];
@@ -460,7 +430,7 @@ __device__ __forceinline__ double wave_sum(double x)
 // minimum, then an unsigned 64-bit atomic minimum on the bit patterns (order-preserving for
 // non-negative doubles, so the result is exact and independent of scheduling).
 __global__ void __launch_bounds__(256)
-k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
+k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ Jac,
         long nodes, long S, double dcsi, double cfl, double dcfl, Phys ph,
         unsigned long long *__restrict__ bits)
 {
@@ -471,23 +441,22 @@ k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doubl
         double q[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
-        const double ir = 1.0 / q[0];
-        const double u = fabs(q[1] * ir), v = fabs(q[2] * ir), w = fabs(q[3] * ir);
-        const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * ir);
-        const double a = sqrt(ph.gamma * p * ir);
+        // ComputeEigenvaluesForState (Physics_NS.f90:927-962)
+        const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
+        const double p = hx::pressure(ph, q);
+        const double a = sqrt(ph.gamma * p / q[0]);
         const double ev0 = u + a, ev1 = v + a, ev2 = w + a;
         double ja[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
-        const double jac = 1.0 / invJ[g];
+        const double jac = Jac[g];
         const double lx = fabs(ja[0] * ev0 + ja[1] * ev1 + ja[2] * ev2) * dcsi;
         const double ly = fabs(ja[3] * ev0 + ja[4] * ev1 + ja[5] * ev2) * dcsi;
         const double lz = fabs(ja[6] * ev0 + ja[7] * ev1 + ja[8] * ev2) * dcsi;
         tc = cfl * fabs(jac) / (lx + ly + lz);
         if (!ph.euler) {
-            const double T = ph.gammaM2 * p * ir;
-            const double tT = T * ph.Trenorm;
-            const double mu = (1.0 + ph.S_div_T) / (tT + ph.S_div_T) * tT * sqrt(tT);
+            const double T = ph.gammaM2 * p / q[0];
+            const double mu = hx::sutherland(ph, T);
             const double d2 = dcsi * dcsi;
             const double vx = mu * d2 * fabs(ja[0] + ja[1] + ja[2]);
             const double vy = mu * d2 * fabs(ja[3] + ja[4] + ja[5]);
@@ -560,17 +529,17 @@ __global__ void k_finish_max(const double *part, int nparts, double *out)
 
 // Kinetic energy: sum_e sum_n w_i w_j w_k J |rho u|^2 / (2 rho), and the volume
 __global__ void __launch_bounds__(256)
-k_kinetic(const double *__restrict__ Q, const double *__restrict__ invJ, const double *__restrict__ wq,
+k_kinetic(const double *__restrict__ Q, const double *__restrict__ Jac, const double *__restrict__ wq,
           long nodes, long S, double *__restrict__ part)
 {
     __shared__ double sk[4], sv[4];
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0, vol = 0.0;
     if (g < nodes) {
-        const double J = 1.0 / invJ[g];
+        const double J = Jac[g];
         const double ww = wq[g];
         const double r = Q[g], a = Q[S + g], b = Q[2 * S + g], c = Q[3 * S + g];
-        ke = ww * J * (0.5 * (a * a + b * b + c * c) / r);
+        ke = ww * J * (0.5 * (a * a + b * b + c * c) / r);   // VolumeIntegrals.f90:185-193
         vol = ww * J;
     }
     ke = wave_sum(ke);
@@ -729,14 +698,14 @@ struct hx3d_engine {
     hipStream_t stream = nullptr;
     double *dQ = nullptr, *dG = nullptr, *dGrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
     double *dFstar = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
+    double *dJ = nullptr, *dFgeo = nullptr;
     double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr;
     unsigned long long *dBits = nullptr;
     int *dNbr = nullptr, *dEfn = nullptr, *dFlag = nullptr;
     int2 *dFnode = nullptr;
-    unsigned char *dFslot = nullptr;
     bool qdot_valid = false, timing = false;
     double ktime[4] = {0, 0, 0, 0};
-    hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> evpool;   // timing: 4 events per stage of the last hx3d_advance
     std::string err;
 
     int fail(int code, const std::string &m) { err = m; return code; }
@@ -763,15 +732,16 @@ void launch_grad(hx3d_engine *eng)
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
     hipLaunchKernelGGL((k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
-                       eng->dInvJ, eng->dNbr, eng->dGrad, eng->K, eng->S, eng->ops);
+                       eng->dInvJ, eng->dNbr, eng->dEfn, eng->dFgeo, eng->dGrad, eng->K, eng->S,
+                       eng->FS, eng->ops);
 }
 
 template <int N>
 void launch_face(hx3d_engine *eng)
 {
     const long nb = (eng->FS + 255) / 256;
-    hipLaunchKernelGGL((k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dGrad, eng->dJa,
-                       eng->dFnode, eng->dFslot, eng->dFdelta, eng->dFstar, eng->FS, eng->S, eng->ph);
+    hipLaunchKernelGGL((k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dGrad, eng->dFgeo,
+                       eng->dFnode, eng->dFdelta, eng->dFstar, eng->FS, eng->S, eng->ph);
 }
 
 template <int N>
@@ -783,40 +753,41 @@ void launch_volume(hx3d_engine *eng, double a_k, double c_k, int update, const d
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
     if (eng->inviscid == HX3D_INVISCID_SPLIT)
         hipLaunchKernelGGL((k_volume<N, EPB, true>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
-                           eng->dGrad, eng->dJa, eng->dInvJ, eng->dEdelta, eng->dEfn, eng->dFstar,
+                           eng->dGrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dEfn, eng->dFstar,
                            eng->dG, qdot, dt_dev, dt_host, a_k, c_k, update, eng->K, eng->S, eng->FS,
                            eng->ops, eng->ph);
     else
         hipLaunchKernelGGL((k_volume<N, EPB, false>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
-                           eng->dGrad, eng->dJa, eng->dInvJ, eng->dEdelta, eng->dEfn, eng->dFstar,
+                           eng->dGrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dEfn, eng->dFstar,
                            eng->dG, qdot, dt_dev, dt_host, a_k, c_k, update, eng->K, eng->S, eng->FS,
                            eng->ops, eng->ph);
 }
 
 template <int N>
 void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev,
-             double dt_host, double *qdot, bool timed)
+             double dt_host, double *qdot, hipEvent_t *ev)
 {
-    if (timed) hipEventRecord(eng->ev[0], eng->stream);
+    if (ev) hipEventRecord(ev[0], eng->stream);
     if (!eng->ph.euler) launch_grad<N>(eng);
-    if (timed) hipEventRecord(eng->ev[1], eng->stream);
+    if (ev) hipEventRecord(ev[1], eng->stream);
     launch_face<N>(eng);
-    if (timed) hipEventRecord(eng->ev[2], eng->stream);
+    if (ev) hipEventRecord(ev[2], eng->stream);
     launch_volume<N>(eng, a_k, c_k, update, dt_dev, dt_host, qdot);
-    if (timed) hipEventRecord(eng->ev[3], eng->stream);
+    if (ev) hipEventRecord(ev[3], eng->stream);
 }
 
+// ev: NULL, or 4 events recorded around the three launches of the stage
 void stage(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev, double dt_host,
-           double *qdot, bool timed = false)
+           double *qdot, hipEvent_t *ev = nullptr)
 {
     switch (eng->N) {
-    case 1: stage_n<1>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
-    case 2: stage_n<2>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
-    case 3: stage_n<3>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
-    case 4: stage_n<4>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
-    case 5: stage_n<5>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
-    case 6: stage_n<6>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
-    default: stage_n<7>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, timed); break;
+    case 1: stage_n<1>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    case 2: stage_n<2>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    case 3: stage_n<3>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    case 4: stage_n<4>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    case 5: stage_n<5>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    case 6: stage_n<6>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    default: stage_n<7>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
     }
 }
 
@@ -824,7 +795,7 @@ void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int advance)
 {
     hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, eng->dBits);
     const long nb = (eng->nodes + 255) / 256;
-    hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dInvJ,
+    hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
                        eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dBits);
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, eng->dBits, eng->dScal, advance);
 }
@@ -891,6 +862,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     ph.lambda_stab = d->lambda_stab;
     ph.les_c = d->les_c;
     ph.riemann = d->riemann;
+    ph.cp = ph.gamma * (1.0 / (1.4 - 1.0));
     ph.averaging = d->averaging;
     ph.les = d->euler ? 0 : d->les;
 
@@ -933,13 +905,15 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         if (nbr[x] < 0 || efn[x] < 0) return bail(eng->fail(HX3D_EINVAL, "an element face node has no neighbour"));
     if (nodes > 0x7fffffffL || 2 * FS > 0x7fffffffL)
         return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit node indices"));
-    // geometry SoA, 1/J, quadrature weights, LES filter widths
-    std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), wq((size_t)S, 0.0), edelta(K), fdelta(F);
+    // geometry SoA, J, 1/J, quadrature weights, LES filter widths, face frames
+    std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), jj((size_t)S, 1.0), wq((size_t)S, 0.0),
+        edelta(K), fdelta(F), fgeo((size_t)10 * FS, 0.0);
     double xg[8], wg[8];
     gl_nodes(N, xg, wg);
     for (long g = 0; g < nodes; ++g) {
         for (int c = 0; c < 9; ++c) ja[(size_t)c * S + g] = d->Ja[9 * g + c];
         ij[g] = 1.0 / d->J[g];
+        jj[g] = d->J[g];
         const int n = (int)(g % ne);
         wq[g] = wg[n % n1] * wg[(n / n1) % n1] * wg[n / nf];
     }
@@ -948,15 +922,47 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         for (int n = 0; n < ne; ++n) vol += wq[(long)e * ne + n] * d->J[(long)e * ne + n];
         edelta[e] = std::pow(vol / ne, 1.0 / 3.0);   // SD.f90:420
     }
+    // face frames: from the descriptor (the reference's MappedGeometryFace), or, with GL nodes,
+    // dS = +-Ja^d of the left element at the face node, |Ja| = |dS|, n = dS/|Ja|
+    // (ConstructMappedGeometryFace, MappedGeometry.f90:482-760) and a fixed orthonormal
+    // completion t1, t2 of n (the Riemann fluxes do not depend on the tangent choice).
+    const bool have_fg = d->fNormal && d->fT1 && d->fT2 && d->fJac;
+    if (!have_fg && (d->fNormal || d->fT1 || d->fT2 || d->fJac))
+        return bail(eng->fail(HX3D_EINVAL, "descriptor: give all four face geometry arrays or none"));
     for (int f = 0; f < F; ++f) {
         const int sL = fslot[f];
         const int dax = SLOT_AXIS[sL];
+        const double sg = SLOT_PLUS[sL] ? 1.0 : -1.0;
         double surf = 0.0;
         for (int j = 0; j <= N; ++j)
             for (int i = 0; i <= N; ++i) {
-                const long g = fnode[(long)f * nf + i + n1 * j].x;
-                const double a = d->Ja[9 * g + 3 * dax], b = d->Ja[9 * g + 3 * dax + 1], c = d->Ja[9 * g + 3 * dax + 2];
-                surf += wg[i] * wg[j] * std::sqrt(a * a + b * b + c * c);
+                const long q = (long)f * nf + i + n1 * j;
+                double nr[3], t1[3], t2[3], jac;
+                if (have_fg) {
+                    for (int c = 0; c < 3; ++c) { nr[c] = d->fNormal[3 * q + c]; t1[c] = d->fT1[3 * q + c]; t2[c] = d->fT2[3 * q + c]; }
+                    jac = d->fJac[q];
+                } else {
+                    const long g = fnode[q].x;
+                    for (int c = 0; c < 3; ++c) nr[c] = sg * d->Ja[9 * g + 3 * dax + c];
+                    jac = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+                    for (int c = 0; c < 3; ++c) nr[c] /= jac;
+                    // t1 = e_m - (e_m . n) n for the axis e_m least aligned with n, normalised
+                    int m = 0;
+                    for (int c = 1; c < 3; ++c) if (std::fabs(nr[c]) < std::fabs(nr[m])) m = c;
+                    for (int c = 0; c < 3; ++c) t1[c] = (c == m ? 1.0 : 0.0) - nr[m] * nr[c];
+                    const double tn = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+                    for (int c = 0; c < 3; ++c) t1[c] /= tn;
+                    t2[0] = nr[1] * t1[2] - nr[2] * t1[1];
+                    t2[1] = nr[2] * t1[0] - nr[0] * t1[2];
+                    t2[2] = nr[0] * t1[1] - nr[1] * t1[0];
+                }
+                for (int c = 0; c < 3; ++c) {
+                    fgeo[(size_t)c * FS + q] = nr[c];
+                    fgeo[(size_t)(3 + c) * FS + q] = t1[c];
+                    fgeo[(size_t)(6 + c) * FS + q] = t2[c];
+                }
+                fgeo[(size_t)9 * FS + q] = jac;
+                surf += wg[i] * wg[j] * jac;
             }
         fdelta[f] = std::sqrt(surf / nf);   // SD.f90:1378
     }
@@ -965,25 +971,24 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dGrad, (size_t)15 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
         (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
+        (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * FS)) ||
         (rc = dalloc(eng, &eng->dFstar, (size_t)NV * FS)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) || (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) ||
         (rc = dalloc(eng, &eng->dScal, 4)) || (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
         (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2)) ||
         (rc = dalloc(eng, &eng->dNbr, nbr.size())) || (rc = dalloc(eng, &eng->dEfn, efn.size())) ||
-        (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())) ||
-        (rc = dalloc(eng, &eng->dFslot, fslot.size())))
+        (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
         return bail(rc);
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(eng->fail(HX3D_EDEVICE, "hipStreamCreate failed"));
-    for (auto &e : eng->ev) hipEventCreate(&e);
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
     if (!up(eng->dJa, ja.data(), ja.size() * 8) || !up(eng->dInvJ, ij.data(), ij.size() * 8) ||
+        !up(eng->dJ, jj.data(), jj.size() * 8) || !up(eng->dFgeo, fgeo.data(), fgeo.size() * 8) ||
         !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
         !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dNbr, nbr.data(), nbr.size() * 4) ||
-        !up(eng->dEfn, efn.data(), efn.size() * 4) || !up(eng->dFnode, fnode.data(), fnode.size() * 8) ||
-        !up(eng->dFslot, fslot.data(), fslot.size()))
+        !up(eng->dEfn, efn.data(), efn.size() * 4) || !up(eng->dFnode, fnode.data(), fnode.size() * 8))
         return bail(eng->fail(HX3D_EDEVICE, "upload failed"));
     // zero state, RK register and time
     const long nb = (NV * S + 255) / 256;
@@ -1000,13 +1005,12 @@ int hx3d_engine_destroy(hx3d_engine *eng)
 {
     if (!eng) return HX3D_OK;
     if (eng->stream) hipStreamSynchronize(eng->stream);
-    void *ptrs[] = {eng->dQ, eng->dG, eng->dGrad, eng->dJa, eng->dInvJ, eng->dFstar, eng->dQDot,
+    void *ptrs[] = {eng->dQ, eng->dG, eng->dGrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dFstar, eng->dQDot,
                     eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
-                    eng->dBits, eng->dNbr, eng->dEfn, eng->dFlag, eng->dFnode, eng->dFslot};
+                    eng->dBits, eng->dNbr, eng->dEfn, eng->dFlag, eng->dFnode};
     for (void *p : ptrs)
         if (p) hipFree(p);
-    for (auto &e : eng->ev)
-        if (e) hipEventDestroy(e);
+    for (auto &e : eng->evpool) hipEventDestroy(e);
     if (eng->stream) hipStreamDestroy(eng->stream);
     delete eng;
     return HX3D_OK;
@@ -1093,25 +1097,32 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
 {
     if (!eng || nsteps < 0) return HX3D_EINVAL;
     if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
-    double acc[3] = {0, 0, 0};
+    const size_t nev = eng->timing ? (size_t)12 * nsteps : 0;
+    while (eng->evpool.size() < nev) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        eng->evpool.push_back(e);
+    }
     for (int s = 0; s < nsteps; ++s) {
         launch_maxdt(eng, cfl, dcfl, 1);
-        for (int k = 0; k < 3; ++k) {
-            const bool timed = eng->timing;
-            stage(eng, RK_A[k], RK_C[k], 1, eng->dScal, 0.0, nullptr, timed);
-            if (timed) {
-                HIPCHK(hipEventSynchronize(eng->ev[3]));
-                float a, b, c;
-                hipEventElapsedTime(&a, eng->ev[0], eng->ev[1]);
-                hipEventElapsedTime(&b, eng->ev[1], eng->ev[2]);
-                hipEventElapsedTime(&c, eng->ev[2], eng->ev[3]);
-                acc[0] += a; acc[1] += b; acc[2] += c;
-            }
-        }
+        for (int k = 0; k < 3; ++k)
+            stage(eng, RK_A[k], RK_C[k], 1, eng->dScal, 0.0, nullptr,
+                  eng->timing ? &eng->evpool[(size_t)12 * s + 4 * k] : nullptr);
     }
     HIPCHK(hipGetLastError());
-    if (eng->timing && nsteps > 0)
+    if (eng->timing && nsteps > 0) {
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        double acc[3] = {0, 0, 0};
+        for (size_t x = 0; x < (size_t)3 * nsteps; ++x) {
+            for (int kk = 0; kk < 3; ++kk) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, eng->evpool[4 * x + kk], eng->evpool[4 * x + kk + 1]));
+                acc[kk] += ms;
+            }
+        }
         for (int x = 0; x < 3; ++x) eng->ktime[x] = acc[x] / (3.0 * nsteps);
+        eng->ktime[3] = (eng->ktime[0] + eng->ktime[1] + eng->ktime[2]);
+    }
     eng->qdot_valid = false;
     if (t_inout) {
         HIPCHK(hipMemcpyAsync(t_inout, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
@@ -1139,7 +1150,7 @@ int hx3d_kinetic_energy(hx3d_engine *eng, double *ke)
 {
     if (!eng || !ke) return HX3D_EINVAL;
     const int nb = (int)((eng->nodes + 255) / 256);
-    hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dInvJ, eng->dWq, eng->nodes, eng->S, eng->dPart);
+    hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJ, eng->dWq, eng->nodes, eng->S, eng->dPart);
     hipLaunchKernelGGL(k_finish_sum2, dim3(1), dim3(64), 0, eng->stream, eng->dPart, nb, eng->dStage);
     double r[2];
     HIPCHK(hipMemcpyAsync(r, eng->dStage, sizeof(r), hipMemcpyDeviceToHost, eng->stream));

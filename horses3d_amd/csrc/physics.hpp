@@ -1,7 +1,9 @@
 // Point-wise compressible Navier-Stokes physics for the hx3d kernels (FP64, device + host).
 // Each function restates the reference routine cited beside it (paths relative to
-// Solver/src/libs/physics/).  Where the reference works in a rotated face frame, the
-// kernels use the equivalent rotation-invariant Cartesian form (noted per routine).
+// Solver/src/libs/physics/) with the reference's operation order: the Taylor-Green cases run
+// at Mach 0.1, where the background pressure makes the volume and surface terms cancel by
+// three orders of magnitude, so the rounding sequence of the large terms is kept identical
+// (the library is compiled without floating-point contraction).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,28 +14,37 @@ namespace hx {
 constexpr int NV = 5;
 
 struct Phys {
-    double gamma, gm1, gammaM2, mu, mu_to_kappa, S_div_T, Trenorm, lambda_stab, les_c;
+    double gamma, gm1, gammaM2, mu, mu_to_kappa, S_div_T, Trenorm, lambda_stab, les_c, cp;
     int riemann, averaging, les, euler;
 };
 
-// navierstokes/VariableConversion_NS.f90:146-186 get_laminar_mu_kappa + SutherlandsLaw
+// navierstokes/VariableConversion_NS.f90:64 Pressure, :99-112 Temperature,
+// :146-186 get_laminar_mu_kappa + SutherlandsLaw
+HX_HD double pressure(const Phys &ph, const double q[NV])
+{
+    return ph.gm1 * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0]);
+}
+
+HX_HD double sutherland(const Phys &ph, double T)
+{
+    const double tT = T * ph.Trenorm;
+    return (1.0 + ph.S_div_T) / (tT + ph.S_div_T) * tT * sqrt(tT);
+}
+
 HX_HD void laminar_mu_kappa(const Phys &ph, const double q[NV], double &mu, double &kappa)
 {
-    double p = ph.gm1 * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0]);
-    double T = ph.gammaM2 * p / q[0];
-    double tT = T * ph.Trenorm;
-    double s = (1.0 + ph.S_div_T) / (tT + ph.S_div_T) * tT * sqrt(tT);
-    mu = ph.mu * s;
+    const double T = ph.gammaM2 * pressure(ph, q) / q[0];
+    mu = ph.mu * sutherland(ph, T);
     kappa = mu * ph.mu_to_kappa;
 }
 
-// navierstokes/Physics_NS.f90:51-97 EulerFlux, contracted with the three metric rows:
-// out[d][v] = sum_c F_c[v] * Ja^d_c
+// navierstokes/Physics_NS.f90:51-97 EulerFlux, then the contravariant projection of
+// BaseClass_ComputeInnerFluxes (discretization/HyperbolicDiscretizationClass.f90:83-152):
+// out[d][v] = F_x[v] Ja^d_x + F_y[v] Ja^d_y + F_z[v] Ja^d_z
 HX_HD void euler_contravariant(const Phys &ph, const double q[NV], const double ja[9], double out[3][NV])
 {
-    double ir = 1.0 / q[0];
-    double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
-    double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
+    const double u = q[1] / q[0], v = q[2] / q[0], w = q[3] / q[0];
+    const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
     double F[NV][3];
     F[0][0] = q[1]; F[1][0] = q[1] * u + p; F[2][0] = q[1] * v; F[3][0] = q[1] * w; F[4][0] = (q[4] + p) * u;
     F[0][1] = q[2]; F[1][1] = F[2][0]; F[2][1] = q[2] * v + p; F[3][1] = q[2] * w; F[4][1] = (q[4] + p) * v;
@@ -49,9 +60,9 @@ HX_HD void euler_contravariant(const Phys &ph, const double q[NV], const double 
 HX_HD void viscous_flux(const Phys &ph, const double q[NV], const double gx[NV], const double gy[NV],
                         const double gz[NV], double mu, double kappa, double F[NV][3])
 {
-    double ir = 1.0 / q[0];
-    double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
-    double ud[3] = {u * ir, v * ir, w * ir};
+    const double ir = 1.0 / q[0];
+    const double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
+    const double ud[3] = {u * ir, v * ir, w * ir};
     double ux[3], uy[3], uz[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -59,73 +70,94 @@ HX_HD void viscous_flux(const Phys &ph, const double q[NV], const double gx[NV],
         uy[d] = ir * gy[1 + d] - ud[d] * gy[0];
         uz[d] = ir * gz[1 + d] - ud[d] * gz[0];
     }
-    double g = ph.gm1 * ph.gammaM2;
-    double eir2 = q[4] * ir * ir;
-    double nTx = g * (ir * gx[4] - eir2 * gx[0] - u * ux[0] - v * ux[1] - w * ux[2]);
-    double nTy = g * (ir * gy[4] - eir2 * gy[0] - u * uy[0] - v * uy[1] - w * uy[2]);
-    double nTz = g * (ir * gz[4] - eir2 * gz[0] - u * uz[0] - v * uz[1] - w * uz[2]);
-    double divV = ux[0] + uy[1] + uz[2];
-    double t11 = mu * (2.0 * ux[0] - 2.0 / 3.0 * divV);
-    double t22 = mu * (2.0 * uy[1] - 2.0 / 3.0 * divV);
-    double t33 = mu * (2.0 * uz[2] - 2.0 / 3.0 * divV);
-    double t12 = mu * (ux[1] + uy[0]);
-    double t13 = mu * (ux[2] + uz[0]);
-    double t23 = mu * (uy[2] + uz[1]);
-    F[0][0] = 0.0; F[1][0] = t11; F[2][0] = t12; F[3][0] = t13;
-    F[4][0] = t11 * u + t12 * v + t13 * w + kappa * nTx;
-    F[0][1] = 0.0; F[1][1] = t12; F[2][1] = t22; F[3][1] = t23;
-    F[4][1] = t12 * u + t22 * v + t23 * w + kappa * nTy;
-    F[0][2] = 0.0; F[1][2] = t13; F[2][2] = t23; F[3][2] = t33;
-    F[4][2] = t13 * u + t23 * v + t33 * w + kappa * nTz;
+    const double g = ph.gm1 * ph.gammaM2;
+    const double nTx = g * (ir * gx[4] - q[4] * ir * ir * gx[0] - u * ux[0] - v * ux[1] - w * ux[2]);
+    const double nTy = g * (ir * gy[4] - q[4] * ir * ir * gy[0] - u * uy[0] - v * uy[1] - w * uy[2]);
+    const double nTz = g * (ir * gz[4] - q[4] * ir * ir * gz[0] - u * uz[0] - v * uz[1] - w * uz[2]);
+    const double divV = ux[0] + uy[1] + uz[2];
+    F[0][0] = 0.0;
+    F[1][0] = mu * (2.0 * ux[0] - 2.0 / 3.0 * divV);
+    F[2][0] = mu * (ux[1] + uy[0]);
+    F[3][0] = mu * (ux[2] + uz[0]);
+    F[4][0] = F[1][0] * u + F[2][0] * v + F[3][0] * w + kappa * nTx;
+    F[0][1] = 0.0;
+    F[1][1] = F[2][0];
+    F[2][1] = mu * (2.0 * uy[1] - 2.0 / 3.0 * divV);
+    F[3][1] = mu * (uy[2] + uz[1]);
+    F[4][1] = F[1][1] * u + F[2][1] * v + F[3][1] * w + kappa * nTy;
+    F[0][2] = 0.0;
+    F[1][2] = F[3][0];
+    F[2][2] = F[3][1];
+    F[3][2] = mu * (2.0 * uz[2] - 2.0 / 3.0 * divV);
+    F[4][2] = F[1][2] * u + F[2][2] * v + F[3][2] * w + kappa * nTz;
 }
 
 // common/LESModels.f90:256-307 Smagorinsky and :358-~440 WALE eddy viscosity (no wall
-// model), with the velocity gradients of getVelocityGradients_State
+// model), velocity gradients of getVelocityGradients_State
 // (navierstokes/VariableConversion_NS.f90:373-392).
 HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], const double gx[NV],
                            const double gy[NV], const double gz[NV])
 {
-    double ir = 1.0 / q[0], ir2 = ir * ir;
-    double ud[3] = {q[1] * ir2, q[2] * ir2, q[3] * ir2};
-    double G[3][3];   // G[a][b] = d u_b / d x_a
+    const double ir = 1.0 / q[0], ir2 = ir * ir;
+    const double ud[3] = {q[1] * ir2, q[2] * ir2, q[3] * ir2};
+    double ux[3], uy[3], uz[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-        G[0][b] = ir * gx[1 + b] - ud[b] * gx[0];
-        G[1][b] = ir * gy[1 + b] - ud[b] * gy[0];
-        G[2][b] = ir * gz[1 + b] - ud[b] * gz[0];
+        ux[b] = ir * gx[1 + b] - ud[b] * gx[0];
+        uy[b] = ir * gy[1 + b] - ud[b] * gy[0];
+        uz[b] = ir * gz[1 + b] - ud[b] * gz[0];
     }
-    double LS = ph.les_c * delta;
     if (ph.les == 1) {
+        // S(:,1) = U_x, S(:,2) = U_y, S(:,3) = U_z; S(1,:) += U_x ...; S = S/2; |S| = sqrt(2 sum S*S)
+        double S[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { S[a][0] = ux[a]; S[a][1] = uy[a]; S[a][2] = uz[a]; }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) { S[0][b] = S[0][b] + ux[b]; S[1][b] = S[1][b] + uy[b]; S[2][b] = S[2][b] + uz[b]; }
         double ss = 0.0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                double s = 0.5 * (G[b][a] + G[a][b]);
-                ss += s * s;
-            }
-        return q[0] * LS * LS * sqrt(2.0 * ss);
+            for (int a = 0; a < 3; ++a) { const double s = 0.5 * S[a][b]; ss += s * s; }
+        const double normS = sqrt(2.0 * ss);
+        const double LS = ph.les_c * delta;
+        return q[0] * (LS * LS) * normS;
     }
     if (ph.les == 2) {
-        double G2[3][3], normS = 0.0, normSd = 0.0;
+        double G[3][3], S[3][3], G2[3][3], Sd[3][3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) { G[0][b] = ux[b]; G[1][b] = uy[b]; G[2][b] = uz[b]; }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                double s = 0.5 * (G[i][j] + G[j][i]);
-                normS += s * s;
-                G2[i][j] = G[i][0] * G[0][j] + G[i][1] * G[1][j] + G[i][2] * G[2][j];
+                S[i][j] = 0.5 * (G[i][j] + G[j][i]);
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc = acc + G[i][k] * G[k][j];
+                G2[i][j] = acc;
             }
-        double div2 = G2[0][0] + G2[1][1] + G2[2][2];
+        const double div2 = G2[0][0] + G2[1][1] + G2[2][2];
+        double normS = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) normS += S[i][j] * S[i][j];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                double sd = 0.5 * (G2[i][j] + G2[j][i]) - (i == j ? div2 / 3.0 : 0.0);
-                normSd += sd * sd;
-            }
-        if (normS < 1.0e-8 && normSd < 1.0e-8) return 0.0;
-        return q[0] * LS * LS * (pow(normSd, 1.5) / (pow(normS, 2.5) + pow(normSd, 1.25)));
+            for (int j = 0; j < 3; ++j) Sd[i][j] = 0.5 * (G2[i][j] + G2[j][i]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Sd[i][i] = Sd[i][i] - 1.0 / 3.0 * div2;
+        double normSd = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) normSd += Sd[i][j] * Sd[i][j];
+        const double LS = ph.les_c * delta;
+        double mu = q[0] * (LS * LS) * (pow(normSd, 1.5) / (pow(normS, 2.5) + pow(normSd, 1.25)));
+        if (normS < 1.0e-8 && normSd < 1.0e-8) mu = 0.0;
+        return mu;
     }
     return 0.0;
 }
@@ -133,110 +165,206 @@ HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], con
 // navierstokes/RiemannSolvers_NS.f90:1541-1656 RoeRiemannSolver
 HX_HD void roe_flux(const Phys &ph, const double L[NV], const double R[NV], const double n[3], double f[NV])
 {
-    const double g = ph.gamma, gm1 = ph.gm1;
-    double irL = 1.0 / L[0], irR = 1.0 / R[0];
-    double ul = L[1] * irL, vl = L[2] * irL, wl = L[3] * irL;
-    double ur = R[1] * irR, vr = R[2] * irR, wr = R[3] * irR;
-    double pl = gm1 * (L[4] - 0.5 * irL * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]));
-    double pr = gm1 * (R[4] - 0.5 * irR * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]));
-    double ql = n[0] * ul + n[1] * vl + n[2] * wl;
-    double qr = n[0] * ur + n[1] * vr + n[2] * wr;
-    double gg = g / gm1;
-    double hl = 0.5 * (ul * ul + vl * vl + wl * wl) + gg * pl * irL;
-    double hr = 0.5 * (ur * ur + vr * vr + wr * wr) + gg * pr * irR;
-    double rtd = sqrt(L[0] * R[0]);
-    double bl = L[0] / (L[0] + rtd), br = 1.0 - bl;
-    double utd = bl * ul + br * ur, vtd = bl * vl + br * vr, wtd = bl * wl + br * wr;
-    double htd = bl * hl + br * hr;
-    double atd2 = gm1 * (htd - 0.5 * (utd * utd + vtd * vtd + wtd * wtd));
-    double atd = sqrt(atd2);
-    double qtd = utd * n[0] + vtd * n[1] + wtd * n[2];
-    double k = (g + 1.0) / 4.0 * atd / rtd;
+    const double g = ph.gamma;
+    const double rho = L[0], rhou = L[1], rhov = L[2], rhow = L[3], rhoe = L[4];
+    const double rhon = R[0], rhoun = R[1], rhovn = R[2], rhown = R[3], rhoen = R[4];
+    const double ul = rhou / rho, vl = rhov / rho, wl = rhow / rho;
+    const double pleft = (g - 1.0) * (rhoe - 0.5 / rho * (rhou * rhou + rhov * rhov + rhow * rhow));
+    const double ur = rhoun / rhon, vr = rhovn / rhon, wr = rhown / rhon;
+    const double pright = (g - 1.0) * (rhoen - 0.5 / rhon * (rhoun * rhoun + rhovn * rhovn + rhown * rhown));
+    const double ql = n[0] * ul + n[1] * vl + n[2] * wl;
+    const double qr = n[0] * ur + n[1] * vr + n[2] * wr;
+    const double hl = 0.5 * (ul * ul + vl * vl + wl * wl) + g / (g - 1.0) * pleft / rho;
+    const double hr = 0.5 * (ur * ur + vr * vr + wr * wr) + g / (g - 1.0) * pright / rhon;
+    const double rtd = sqrt(rho * rhon);
+    const double betal = rho / (rho + rtd), betar = 1.0 - betal;
+    const double utd = betal * ul + betar * ur, vtd = betal * vl + betar * vr, wtd = betal * wl + betar * wr;
+    const double htd = betal * hl + betar * hr;
+    const double atd2 = (g - 1.0) * (htd - 0.5 * (utd * utd + vtd * vtd + wtd * wtd));
+    const double atd = sqrt(atd2);
+    const double qtd = utd * n[0] + vtd * n[1] + wtd * n[2];
     if (qtd >= 0.0) {
-        double dw = 0.5 * ((pr - pl) / atd2 - (qr - ql) * rtd / atd);
-        double sp = qtd - atd;
-        double spm = sp < 0.0 ? sp : 0.0;
-        double e = -fabs(sp) - k * dw;
-        double eta = e > 0.0 ? e : 0.0;
-        double udw = dw * (spm - 0.5 * eta);
-        double rq = L[0] * ql;
-        f[0] = rq + udw;
-        f[1] = rq * ul + pl * n[0] + udw * (utd - atd * n[0]);
-        f[2] = rq * vl + pl * n[1] + udw * (vtd - atd * n[1]);
-        f[3] = rq * wl + pl * n[2] + udw * (wtd - atd * n[2]);
-        f[4] = rq * hl + udw * (htd - qtd * atd);
+        const double dw1 = 0.5 * ((pright - pleft) / atd2 - (qr - ql) * rtd / atd);
+        const double sp1 = qtd - atd;
+        const double sp1m = sp1 < 0.0 ? sp1 : 0.0;
+        const double hd1m = ((g + 1.0) / 4.0 * atd / rtd) * dw1;
+        const double e1 = -fabs(sp1) - hd1m;
+        const double eta1 = e1 > 0.0 ? e1 : 0.0;
+        const double udw1 = dw1 * (sp1m - 0.5 * eta1);
+        const double rql = rho * ql;
+        f[0] = rql + udw1;
+        f[1] = rql * ul + pleft * n[0] + udw1 * (utd - atd * n[0]);
+        f[2] = rql * vl + pleft * n[1] + udw1 * (vtd - atd * n[1]);
+        f[3] = rql * wl + pleft * n[2] + udw1 * (wtd - atd * n[2]);
+        f[4] = rql * hl + udw1 * (htd - qtd * atd);
     } else {
-        double dw = 0.5 * ((pr - pl) / atd2 + (qr - ql) * rtd / atd);
-        double sp = qtd + atd;
-        double spp = sp > 0.0 ? sp : 0.0;
-        double e = -fabs(sp) + k * dw;
-        double eta = e > 0.0 ? e : 0.0;
-        double udw = dw * (spp + 0.5 * eta);
-        double rq = R[0] * qr;
-        f[0] = rq - udw;
-        f[1] = rq * ur + pr * n[0] - udw * (utd + atd * n[0]);
-        f[2] = rq * vr + pr * n[1] - udw * (vtd + atd * n[1]);
-        f[3] = rq * wr + pr * n[2] - udw * (wtd + atd * n[2]);
-        f[4] = rq * hr - udw * (htd + qtd * atd);
+        const double dw4 = 0.5 * ((pright - pleft) / atd2 + (qr - ql) * rtd / atd);
+        const double sp4 = qtd + atd;
+        const double sp4p = sp4 > 0.0 ? sp4 : 0.0;
+        const double hd4 = ((g + 1.0) / 4.0 * atd / rtd) * dw4;
+        const double e4 = -fabs(sp4) + hd4;
+        const double eta4 = e4 > 0.0 ? e4 : 0.0;
+        const double udw4 = dw4 * (sp4p + 0.5 * eta4);
+        const double rqr = rhon * qr;
+        f[0] = rqr - udw4;
+        f[1] = rqr * ur + pright * n[0] - udw4 * (utd + atd * n[0]);
+        f[2] = rqr * vr + pright * n[1] - udw4 * (vtd + atd * n[1]);
+        f[3] = rqr * wr + pright * n[2] - udw4 * (wtd + atd * n[2]);
+        f[4] = rqr * hr - udw4 * (htd + qtd * atd);
     }
 }
 
-// navierstokes/RiemannSolvers_NS.f90:1251-1334 LxFRiemannSolver with AveragedStates
-// (:1784-2077).  The reference rotates into (n, t1, t2), averages, and rotates the momentum
-// back; every averaging is rotation-covariant, so the flux is evaluated directly in
-// Cartesian components along n (identical in exact arithmetic, no t1/t2 needed).
-HX_HD void lxf_flux(const Phys &ph, const double L[NV], const double R[NV], const double n[3], double f[NV])
+// AveragedStates (RiemannSolvers_NS.f90:1784-2077) of rotated states (rho, rho u_n, rho u_t1,
+// rho u_t2, rho e)
+HX_HD void averaged_states(const Phys &ph, const double QL[NV], const double QR[NV], double pL,
+                           double pR, double iL, double iR, double f[NV])
 {
-    double irL = 1.0 / L[0], irR = 1.0 / R[0];
-    double uL[3] = {L[1] * irL, L[2] * irL, L[3] * irL};
-    double uR[3] = {R[1] * irR, R[2] * irR, R[3] * irR};
-    double mnL = L[1] * n[0] + L[2] * n[1] + L[3] * n[2];   // rho u_n
-    double mnR = R[1] * n[0] + R[2] * n[1] + R[3] * n[2];
-    double pL = ph.gm1 * (L[4] - 0.5 * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) * irL);
-    double pR = ph.gm1 * (R[4] - 0.5 * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) * irR);
-    double aL = sqrt(ph.gamma * pL * irL), aR = sqrt(ph.gamma * pR * irR);
-    double unL = mnL * irL, unR = mnR * irR;
-    double l1 = fabs(unL) + aL, l2 = fabs(unR) + aR;
-    double lam = l1 > l2 ? l1 : l2;
+    const double uL = iL * QL[1], uR = iR * QR[1], vL = iL * QL[2], vR = iR * QR[2];
+    const double wL = iL * QL[3], wR = iR * QR[3];
     switch (ph.averaging) {
-    case 1: {  // Ducros
-        double us = 0.25 * (unL + unR);
-        double ps = 0.5 * (pL + pR);
-        f[0] = (L[0] + R[0]) * us;
-        for (int d = 0; d < 3; ++d) f[1 + d] = (L[1 + d] + R[1 + d]) * us + ps * n[d];
-        f[4] = (L[4] + pL + R[4] + pR) * us;
+    case 1:
+        f[0] = 0.25 * (QL[0] + QR[0]) * (uL + uR);
+        f[1] = 0.25 * (QL[1] + QR[1]) * (uL + uR) + 0.5 * (pL + pR);
+        f[2] = 0.25 * (QL[2] + QR[2]) * (uL + uR);
+        f[3] = 0.25 * (QL[3] + QR[3]) * (uL + uR);
+        f[4] = 0.25 * (QL[4] + pL + QR[4] + pR) * (uL + uR);
+        break;
+    case 2: {
+        const double rho = 0.5 * (QL[0] + QR[0]), u = 0.5 * (uL + uR), v = 0.5 * (vL + vR);
+        const double w = 0.5 * (wL + wR), p = 0.5 * (pL + pR), e = 0.5 * (QL[4] * iL + QR[4] * iR);
+        f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * v; f[3] = rho * u * w;
+        f[4] = rho * u * e + p * u;
         break; }
-    case 2:    // Kennedy-Gruber
-    case 3: {  // Pirozzoli
-        double rho = 0.5 * (L[0] + R[0]), un = 0.5 * (unL + unR), p = 0.5 * (pL + pR);
-        double m = rho * un;
-        f[0] = m;
-        for (int d = 0; d < 3; ++d) f[1 + d] = m * 0.5 * (uL[d] + uR[d]) + p * n[d];
-        if (ph.averaging == 2) f[4] = m * 0.5 * (L[4] * irL + R[4] * irR) + p * un;
-        else f[4] = m * 0.5 * ((L[4] + pL) * irL + (R[4] + pR) * irR);
+    case 3: {
+        const double rho = 0.5 * (QL[0] + QR[0]), u = 0.5 * (uL + uR), v = 0.5 * (vL + vR);
+        const double w = 0.5 * (wL + wR), p = 0.5 * (pL + pR);
+        const double h = 0.5 * ((QL[4] + pL) * iL + (QR[4] + pR) * iR);
+        f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * v; f[3] = rho * u * w;
+        f[4] = rho * u * h;
         break; }
-    case 4: {  // Morinishi
-        double cp = ph.gamma / ph.gm1;
-        double s = 0.25 * (mnL + mnR);
-        f[0] = 0.5 * (mnL + mnR);
-        for (int d = 0; d < 3; ++d) f[1 + d] = s * (uL[d] + uR[d]) + 0.5 * (pL + pR) * n[d];
-        double uLuL = uL[0] * uL[0] + uL[1] * uL[1] + uL[2] * uL[2];
-        double uRuR = uR[0] * uR[0] + uR[1] * uR[1] + uR[2] * uR[2];
-        double uLuR = uL[0] * uR[0] + uL[1] * uR[1] + uL[2] * uR[2];
-        // sum_c (mL uL_c + mR uR_c)(uL_c + uR_c)/4 - (mL uL_c^2 + mR uR_c^2)/4
-        f[4] = 0.5 * (unL * cp * pL + unR * cp * pR) + 0.25 * ((mnL + mnR) * uLuR);
-        (void)uLuL; (void)uRuR;
+    case 4: {
+        const double hL = ph.cp * pL, hR = ph.cp * pR;
+        f[0] = 0.5 * (QL[1] + QR[1]);
+        f[1] = 0.25 * (QL[1] + QR[1]) * (uL + uR) + 0.5 * (pL + pR);
+        f[2] = 0.25 * (QL[1] + QR[1]) * (vL + vR);
+        f[3] = 0.25 * (QL[1] + QR[1]) * (wL + wR);
+        f[4] = 0.5 * (uL * hL + uR * hR) + 0.25 * (QL[1] * uL + QR[1] * uR) * (uL + uR)
+             + 0.25 * (QL[1] * vL + QR[1] * vR) * (vL + vR)
+             + 0.25 * (QL[1] * wL + QR[1] * wR) * (wL + wR)
+             - 0.25 * (QL[1] * (uL * uL) + QR[1] * (uR * uR))
+             - 0.25 * (QL[1] * (vL * vL) + QR[1] * (vR * vR))
+             - 0.25 * (QL[1] * (wL * wL) + QR[1] * (wR * wR));
         break; }
-    default: { // Standard
-        double ps = 0.5 * (pL + pR);
-        f[0] = 0.5 * (mnL + mnR);
-        for (int d = 0; d < 3; ++d) f[1 + d] = 0.5 * (mnL * uL[d] + mnR * uR[d]) + ps * n[d];
-        f[4] = 0.5 * (unL * (L[4] + pL) + unR * (R[4] + pR));
+    default:
+        f[0] = 0.5 * (QL[1] + QR[1]);
+        f[1] = 0.5 * (QL[1] * uL + QR[1] * uR + pL + pR);
+        f[2] = 0.5 * (QL[1] * vL + QR[1] * vR);
+        f[3] = 0.5 * (QL[1] * wL + QR[1] * wR);
+        f[4] = 0.5 * (uL * (QL[4] + pL) + uR * (QR[4] + pR));
     }
-    }
-    double s = ph.lambda_stab * 0.5 * lam;
+}
+
+// navierstokes/RiemannSolvers_NS.f90:1251-1334 LxFRiemannSolver (rotated face frame)
+HX_HD void lxf_flux(const Phys &ph, const double QL[NV], const double QR[NV], const double n[3],
+                    const double t1[3], const double t2[3], double f[NV])
+{
+    const double rhoL = QL[0], rhoR = QR[0];
+    const double iL = 1.0 / rhoL, iR = 1.0 / rhoR;
+    const double ruL = QL[1] * n[0] + QL[2] * n[1] + QL[3] * n[2];
+    const double ruR = QR[1] * n[0] + QR[2] * n[1] + QR[3] * n[2];
+    const double rvL = QL[1] * t1[0] + QL[2] * t1[1] + QL[3] * t1[2];
+    const double rvR = QR[1] * t1[0] + QR[2] * t1[1] + QR[3] * t1[2];
+    const double rwL = QL[1] * t2[0] + QL[2] * t2[1] + QL[3] * t2[2];
+    const double rwR = QR[1] * t2[0] + QR[2] * t2[1] + QR[3] * t2[2];
+    const double V2L = (ruL * ruL + rvL * rvL + rwL * rwL) * iL;
+    const double V2R = (ruR * ruR + rvR * rvR + rwR * rwR) * iR;
+    const double pL = ph.gm1 * (QL[4] - 0.5 * V2L), pR = ph.gm1 * (QR[4] - 0.5 * V2R);
+    const double aL = sqrt(ph.gamma * pL * iL), aR = sqrt(ph.gamma * pR * iR);
+    const double l1 = fabs(ruL * iL) + aL, l2 = fabs(ruR * iR) + aR;
+    const double lam = l1 > l2 ? l1 : l2;
+    const double Lr[NV] = {rhoL, ruL, rvL, rwL, QL[4]}, Rr[NV] = {rhoR, ruR, rvR, rwR, QR[4]};
+    averaged_states(ph, Lr, Rr, pL, pR, iL, iR, f);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) f[v] -= s * (R[v] - L[v]);
+    for (int v = 0; v < NV; ++v) f[v] = f[v] - ph.lambda_stab * (0.5 * lam * (Rr[v] - Lr[v]));
+    const double f1 = f[1], f2 = f[2], f3 = f[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[1 + d] = n[d] * f1 + t1[d] * f2 + t2[d] * f3;
+}
+
+// Two-point volume fluxes, RiemannSolvers_NS.f90:2085-2438 (Standard, Ducros, Kennedy-Gruber,
+// Pirozzoli): fSharp = f Ja_x + g Ja_y + h Ja_z of the averaged states and metrics.
+// q: conservative state; p: (u, v, w, pressure, 1/rho) as the reference computes them there.
+HX_HD void two_point_flux(const Phys &ph, const double *qa, const double *pa, const double *ja,
+                          const double *qb, const double *pb, const double *jb, double fs[NV])
+{
+    const double uL = pa[0], vL = pa[1], wL = pa[2], pL = pa[3];
+    const double uR = pb[0], vR = pb[1], wR = pb[2], pR = pb[3];
+    double f[NV], g[NV], h[NV], J[3];
+    switch (ph.averaging) {
+    case 1:
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[d] = 0.5 * (ja[d] + jb[d]);
+        f[0] = 0.25 * (qa[0] + qb[0]) * (uL + uR);
+        f[1] = 0.25 * (qa[1] + qb[1]) * (uL + uR) + 0.5 * (pL + pR);
+        f[2] = 0.25 * (qa[2] + qb[2]) * (uL + uR);
+        f[3] = 0.25 * (qa[3] + qb[3]) * (uL + uR);
+        f[4] = 0.25 * (qa[4] + pL + qb[4] + pR) * (uL + uR);
+        g[0] = 0.25 * (qa[0] + qb[0]) * (vL + vR);
+        g[1] = 0.25 * (qa[1] + qb[1]) * (vL + vR);
+        g[2] = 0.25 * (qa[2] + qb[2]) * (vL + vR) + 0.5 * (pL + pR);
+        g[3] = 0.25 * (qa[3] + qb[3]) * (vL + vR);
+        g[4] = 0.25 * (qa[4] + pL + qb[4] + pR) * (vL + vR);
+        h[0] = 0.25 * (qa[0] + qb[0]) * (wL + wR);
+        h[1] = 0.25 * (qa[1] + qb[1]) * (wL + wR);
+        h[2] = 0.25 * (qa[2] + qb[2]) * (wL + wR);
+        h[3] = 0.25 * (qa[3] + qb[3]) * (wL + wR) + 0.5 * (pL + pR);
+        h[4] = 0.25 * (qa[4] + pL + qb[4] + pR) * (wL + wR);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fs[v] = f[v] * J[0] + g[v] * J[1] + h[v] * J[2];
+        return;
+    case 2:
+    case 3: {
+        const double rho = 0.5 * (qa[0] + qb[0]);
+        const double u = 0.5 * (uL + uR), v = 0.5 * (vL + vR), w = 0.5 * (wL + wR);
+        const double p = 0.5 * (pL + pR);
+        double e;
+        if (ph.averaging == 2) e = 0.5 * (qa[4] * pa[4] + qb[4] * pb[4]);
+        else e = 0.5 * ((qa[4] + pL) * pa[4] + (qb[4] + pR) * pb[4]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[d] = 0.5 * (ja[d] + jb[d]);
+        f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * v; f[3] = rho * u * w;
+        g[0] = rho * v; g[1] = rho * v * u; g[2] = rho * v * v + p; g[3] = rho * v * w;
+        h[0] = rho * w; h[1] = rho * w * u; h[2] = rho * w * v; h[3] = rho * w * w + p;
+        if (ph.averaging == 2) {
+            f[4] = rho * u * e + p * u; g[4] = rho * v * e + p * v; h[4] = rho * w * e + p * w;
+        } else {
+            f[4] = rho * u * e; g[4] = rho * v * e; h[4] = rho * w * e;
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) fs[q] = f[q] * J[0] + g[q] * J[1] + h[q] * J[2];
+        return; }
+    default:
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[d] = ja[d] + jb[d];
+        f[0] = (qa[1] + qb[1]);
+        f[1] = (qa[1] * uL + qb[1] * uR + pL + pR);
+        f[2] = (qa[1] * vL + qb[1] * vR);
+        f[3] = (qa[1] * wL + qb[1] * wR);
+        f[4] = (uL * (qa[4] + pL) + uR * (qb[4] + pR));
+        g[0] = (qa[2] + qb[2]);
+        g[1] = (qa[2] * uL + qb[2] * uR);
+        g[2] = (qa[2] * vL + qb[2] * vR + pL + pR);
+        g[3] = (qa[2] * wL + qb[2] * wR);
+        g[4] = (vL * (qa[4] + pL) + vR * (qb[4] + pR));
+        h[0] = (qa[3] + qb[3]);
+        h[1] = (qa[3] * uL + qb[3] * uR);
+        h[2] = (qa[3] * vL + qb[3] * vR);
+        h[3] = (qa[3] * wL + qb[3] * wR + pL + pR);
+        h[4] = (wL * (qa[4] + pL) + wR * (qb[4] + pR));
+#pragma unroll
+        for (int q = 0; q < NV; ++q) fs[q] = 0.25 * (f[q] * J[0] + g[q] * J[1] + h[q] * J[2]);
+        return;
+    }
 }
 
 }  // namespace hx

@@ -28,6 +28,8 @@ class CaseDesc(C.Structure):
     _fields_ = [("K", C.c_int), ("N", C.c_void_p), ("faceID", C.c_void_p), ("faceSide", C.c_void_p),
                 ("F", C.c_int), ("fElem", C.c_void_p), ("fESide", C.c_void_p), ("fRot", C.c_void_p),
                 ("Ja", C.c_void_p), ("J", C.c_void_p),
+                ("fNormal", C.c_void_p), ("fT1", C.c_void_p), ("fT2", C.c_void_p),
+                ("fJac", C.c_void_p),
                 ("mach", C.c_double), ("reynolds", C.c_double), ("prandtl", C.c_double),
                 ("euler", C.c_int), ("riemann", C.c_int), ("inviscid", C.c_int),
                 ("averaging", C.c_int), ("lambda_stab", C.c_double), ("les", C.c_int),
@@ -97,11 +99,16 @@ class Engine:
             fRot=np.ascontiguousarray(case["fRot"], dtype=np.int32),
             Ja=np.ascontiguousarray(case["Ja"], dtype=np.float64),
             J=np.ascontiguousarray(case["J"], dtype=np.float64))
+        face_geo = all(x in case for x in ("normal", "t1", "t2", "fjac"))
+        if face_geo:
+            for x in ("normal", "t1", "t2", "fjac"):
+                self._keep[x] = np.ascontiguousarray(case[x], dtype=np.float64)
         k = self._keep
+        fg = (lambda x: _p(k[x])) if face_geo else (lambda x: None)
         les_id = LES[les]
         d = CaseDesc(int(case["K"]), _p(k["N"]), _p(k["faceID"]), _p(k["faceSide"]), int(case["F"]),
                      _p(k["fElem"]), _p(k["fESide"]), _p(k["fRot"]), _p(k["Ja"]), _p(k["J"]),
-                     mach, reynolds, prandtl, 1 if euler else 0, RIEMANN[riemann],
+                     fg("normal"), fg("t1"), fg("t2"), fg("fjac"), mach, reynolds, prandtl, 1 if euler else 0, RIEMANN[riemann],
                      INVISCID[inviscid], AVERAGING[averaging], lambda_stab, les_id,
                      LES_DEFAULT_C[les_id] if les_c is None else les_c)
         h = C.c_void_p()

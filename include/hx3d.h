@@ -66,6 +66,15 @@ typedef struct {
     /* geometry (MappedGeometry.f90:64-386), per global node g */
     const double *Ja;       /* [9*nodes] Ja^d_c at 9g + 3d + c (d = xi, eta, zeta) */
     const double *J;        /* [nodes] Jacobian */
+    /* face geometry (MappedGeometryFace, MappedGeometry.f90:482-760) per face node
+     * q = f*(N+1)^2 + i + (N+1)*j in the left element's face coordinates: unit normal,
+     * tangents and surface Jacobian |Ja|.  Optional (all NULL): then the normal and |Ja| are
+     * taken from the left element's metrics at the face node (identical for GL nodes) and the
+     * tangents from a fixed orthonormal completion of the normal. */
+    const double *fNormal;  /* [3*F*(N+1)^2] */
+    const double *fT1;      /* [3*F*(N+1)^2] */
+    const double *fT2;      /* [3*F*(N+1)^2] */
+    const double *fJac;     /* [F*(N+1)^2] */
     /* physics (ConstructPhysicsStorage_NS, PhysicsStorage_NS.f90:145-439) */
     double mach, reynolds, prandtl;
     int euler;              /* 1: "flow equations" = Euler (no viscous terms) */
@@ -111,8 +120,10 @@ int hx3d_kinetic_energy(hx3d_engine *eng, double *ke);
 int hx3d_check_nan(hx3d_engine *eng, int *has_nan);
 /* Blocks until the engine's queued work is done. */
 int hx3d_synchronize(hx3d_engine *eng);
-/* Average duration (ms) of each stage kernel over the last hx3d_advance call when timing
- * was enabled with hx3d_set_timing(eng, 1): out[0..2] = gradient, face flux, volume+RK. */
+/* Average duration (ms) per launch of each stage kernel over the last hx3d_advance call,
+ * measured with HIP events on the engine stream when timing was enabled with
+ * hx3d_set_timing(eng, 1): out[0..2] = gradient+lifting, face flux, volume+surface+RK;
+ * out[3] = their sum (one stage). */
 int hx3d_set_timing(hx3d_engine *eng, int on);
 int hx3d_kernel_times(hx3d_engine *eng, double out_ms[4]);
 

@@ -45,8 +45,13 @@ def test_rk3_step_and_dt_match_oracle(golden):
     eng.rk3_step(0.0, dt)
     Qo, G = Q0.copy(), np.zeros_like(Q0)
     ora.rk3_step(Qo, G, 0.0, dt)
-    err = po.normwise_error(eng.get_state() - Q0, Qo - Q0)
-    assert err.max() <= TOL, err
+    Qg = eng.get_state()
+    # per variable: |dQ_gpu - dQ_oracle| <= 1e-12 max|dQ| + a few ulp of the state itself
+    # (the increment of a nearly constant variable such as rho is ~1e-4 of the state)
+    dq = np.abs(Qo - Q0).max(axis=0)
+    floor = 4.0 * np.finfo(float).eps * np.abs(Qo).max(axis=0)
+    diff = np.abs(Qg - Qo).max(axis=0)
+    assert np.all(diff <= TOL * dq + floor), (diff, dq, floor)
     eng.close()
 
 
