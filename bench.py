@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy kernel 5 times (the "
+                         "known-size reference for calibrating FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the "
@@ -113,6 +116,14 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU visible"
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8}))
+        return
     print("[bench] building case %d^3 N=%d" % (args.n, args.order), file=sys.stderr, flush=True)
     case = mesh.periodic_box(args.n, args.order, amplitude=args.amplitude, phase_seed=2)
     kw = dict(mach=0.1, reynolds=1600.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli")

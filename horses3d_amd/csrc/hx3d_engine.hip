@@ -22,23 +22,23 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/hx3d.h"
 #include "physics.hpp"
+#include "stage_kernels.hpp"
 
 using hx::NV;
 using hx::Phys;
+using hxk::DtArgs;
+using hxk::Ops;
 
 namespace {
 
 constexpr int MAXN = 7;
-
-struct Ops {  // NodalStorage for GL nodes (NodalStorageClass.f90:170-337), row-major (i,l)
-    double D[64], hatD[64], sharpD[64], w[8];
-};
 
 // element face slot 0..5 = EFRONT, EBACK, EBOTTOM, ERIGHT, ETOP, ELEFT
 constexpr int SLOT_AXIS[6] = {1, 1, 2, 0, 2, 0};
@@ -46,366 +46,7 @@ constexpr int SLOT_PLUS[6] = {0, 1, 0, 1, 1, 0};
 constexpr int SLOT_U[6] = {0, 0, 0, 1, 0, 1};
 constexpr int SLOT_V[6] = {2, 2, 1, 2, 1, 2};
 
-__host__ __device__ inline int xcd_block(int b, int grid)
-{
-    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
-    // elements so that face neighbours (gathered through L2) mostly share an XCD.  The grid
-    // is a multiple of 8, which makes this a bijection of [0, grid); logical blocks past
-    // the last element exit.
-    const int per = grid >> 3;
-    return (b & 7) * per + (b >> 3);
-}
-
 inline int grid_for(int nb) { return ((nb + 7) / 8) * 8; }
-
-// ------------------------------------------------------------------------------------
-// k_grad
-// ------------------------------------------------------------------------------------
-template <int N, int EPB>
-__global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
-k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
-       const int *__restrict__ nbr, const int *__restrict__ efn, const double *__restrict__ fgeo,
-       double *__restrict__ grad, int K, long S, long FS, Ops ops)
-{
-    constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
-    __shared__ double sQ[EPB][NV][ne];
-    __shared__ double sD[n1 * n1];
-    const int nb = (K + EPB - 1) / EPB;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int t = threadIdx.x;
-    const int el = t / ne, n = t - el * ne;
-    const int e = lb * EPB + el;
-    const bool active = lb < nb && el < EPB && e < K;
-    for (int x = t; x < n1 * n1; x += blockDim.x) sD[x] = ops.D[(x / n1) * 8 + (x % n1)];
-    const long g = (long)e * ne + n;
-    double q[NV];
-    if (active) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) { q[v] = Q[v * S + g]; sQ[el][v][n] = q[v]; }
-    }
-    __syncthreads();
-    if (!active) return;
-    const int i = n % n1, j = (n / n1) % n1, k = n / nf;
-    double uxi[NV], ueta[NV], uzeta[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) { uxi[v] = 0.0; ueta[v] = 0.0; uzeta[v] = 0.0; }
-#pragma unroll
-    for (int l = 0; l < n1; ++l) {
-        const double di = sD[i * n1 + l], dj = sD[j * n1 + l], dk = sD[k * n1 + l];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            uxi[v] += sQ[el][v][l + n1 * (j + n1 * k)] * di;
-            ueta[v] += sQ[el][v][i + n1 * (l + n1 * k)] * dj;
-            uzeta[v] += sQ[el][v][i + n1 * (j + n1 * l)] * dk;
-        }
-    }
-    double ja[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
-    const double iJ = invJ[g];
-    double gr[3][NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        gr[0][v] = (uxi[v] * ja[0] + ueta[v] * ja[3] + uzeta[v] * ja[6]) * iJ;
-        gr[1][v] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
-        gr[2][v] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
-    }
-    // BR1 lifting (BR1_ComputeElementInterfaceAverage, EllipticBR1.f90:571-627, and
-    // BR1_GradientFaceLoop, :531-569): on each face the node lies on,
-    // unStar = 1/2 (U_R - U_L) |Ja| n with the face frame of the left element, the same value
-    // for both sides; the GL surface integral weights it by b = 1/w_0.
-    const bool onface = (i == 0) | (i == N) | (j == 0) | (j == N) | (k == 0) | (k == N);
-    if (onface) {
-        const double b0 = 1.0 / ops.w[0];
-        double fi[3][NV];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) fi[d][v] = 0.0;
-        auto lift = [&](int slot, int a, int b) {
-            const long x = ((long)e * 6 + slot) * nf + a + n1 * b;
-            const int gn = nbr[x];
-            const int code = efn[x];
-            const long fq = code >> 1;
-            const bool right = code & 1;
-            const double jac = fgeo[9 * FS + fq];
-            const double n0 = fgeo[fq], n1_ = fgeo[FS + fq], n2 = fgeo[2 * FS + fq];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const double qn = Q[v * S + gn];
-                const double uS = 0.5 * (right ? q[v] - qn : qn - q[v]) * jac;
-                fi[0][v] = fi[0][v] + (uS * n0) * b0;
-                fi[1][v] = fi[1][v] + (uS * n1_) * b0;
-                fi[2][v] = fi[2][v] + (uS * n2) * b0;
-            }
-        };
-        if (i == 0) lift(5, j, k);
-        if (i == N) lift(3, j, k);
-        if (j == 0) lift(0, i, k);
-        if (j == N) lift(1, i, k);
-        if (k == 0) lift(2, i, j);
-        if (k == N) lift(4, i, j);
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) gr[d][v] = gr[d][v] + fi[d][v] * iJ;
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) grad[(5 * d + v) * S + g] = gr[d][v];
-}
-
-// ------------------------------------------------------------------------------------
-// k_face: one thread per face node
-// ------------------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(256)
-k_face(const double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ fgeo,
-       const int2 *__restrict__ fnode, const double *__restrict__ fdelta, double *__restrict__ fstar,
-       long FS, long S, Phys ph)
-{
-    constexpr int n1 = N + 1, nf = n1 * n1;
-    const long fq = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (fq >= FS) return;
-    const long f = fq / nf;
-    const int2 gg = fnode[fq];
-    // face frame (unit normal, tangents, |Ja|): fgeo[c*FS + fq], c = n(0..2), t1(3..5), t2(6..8), jac(9)
-    double nrm[3], t1[3], t2[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        nrm[c] = fgeo[c * FS + fq];
-        t1[c] = fgeo[(3 + c) * FS + fq];
-        t2[c] = fgeo[(6 + c) * FS + fq];
-    }
-    const double jac = fgeo[9 * FS + fq];
-    double qL[NV], qR[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) { qL[v] = Q[v * S + gg.x]; qR[v] = Q[v * S + gg.y]; }
-    double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (!ph.euler) {
-        // BR1_RiemannSolver (EllipticBR1.f90:816-868) with the face viscosities of
-        // compute_viscosity_at_faces (SpatialDiscretization.f90:1345-1398)
-        double gL[3][NV], gR[3][NV];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                gL[d][v] = grad[(5 * d + v) * S + gg.x];
-                gR[d][v] = grad[(5 * d + v) * S + gg.y];
-            }
-        double muL, kL, muR, kR;
-        hx::laminar_mu_kappa(ph, qL, muL, kL);
-        hx::laminar_mu_kappa(ph, qR, muR, kR);
-        if (ph.les) {
-            const double dl = fdelta[f];
-            const double mtL = hx::les_viscosity(ph, dl, qL, gL[0], gL[1], gL[2]);
-            const double mtR = hx::les_viscosity(ph, dl, qR, gR[0], gR[1], gR[2]);
-            muL = muL + mtL; kL = kL + mtL * ph.mu_to_kappa;
-            muR = muR + mtR; kR = kR + mtR * ph.mu_to_kappa;
-        }
-        double FL[NV][3], FR[NV][3];
-        hx::viscous_flux(ph, qL, gL[0], gL[1], gL[2], muL, kL, FL);
-        hx::viscous_flux(ph, qR, gR[0], gR[1], gR[2], muR, kR, FR);
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-            visc[v] = 0.5 * (FL[v][0] + FR[v][0]) * nrm[0] + 0.5 * (FL[v][1] + FR[v][1]) * nrm[1] +
-                      0.5 * (FL[v][2] + FR[v][2]) * nrm[2];
-    }
-    double inv[NV];
-    if (ph.riemann == 1) hx::lxf_flux(ph, qL, qR, nrm, t1, t2, inv);
-    else hx::roe_flux(ph, qL, qR, nrm, inv);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) fstar[v * FS + fq] = (inv[v] - visc[v]) * jac;
-}
-
-// ------------------------------------------------------------------------------------
-// k_volume: volume integral + surface integral + 1/J + RK3 register update
-// ------------------------------------------------------------------------------------
-template <int N, int EPB, bool SPLIT>
-__global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
-k_volume(double *__restrict__ Q, const double *__restrict__ grad, const double *__restrict__ Ja,
-         const double *__restrict__ Jac, const double *__restrict__ edelta,
-         const int *__restrict__ efn, const double *__restrict__ fstar, double *__restrict__ G,
-         double *__restrict__ qdot_out, const double *__restrict__ dt_dev, double dt_host,
-         double a_k, double c_k, int update, int K, long S, long FS, Ops ops, Phys ph)
-{
-    constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
-    // standard: sF[el][d][v][n] total contravariant flux; split: sF holds the viscous
-    // contravariant flux and sP the node data for the two-point fluxes.
-    __shared__ double sF[EPB][3][NV][ne];
-    __shared__ double sP[SPLIT ? EPB : 1][SPLIT ? 19 : 1][SPLIT ? ne : 1];
-    __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
-    const int nb = (K + EPB - 1) / EPB;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int t = threadIdx.x;
-    const int el = t / ne, n = t - el * ne;
-    const int e = lb * EPB + el;
-    const bool active = lb < nb && el < EPB && e < K;
-    for (int x = t; x < n1 * n1; x += blockDim.x) {
-        sHat[x] = ops.hatD[(x / n1) * 8 + (x % n1)];
-        sSharp[x] = ops.sharpD[(x / n1) * 8 + (x % n1)];
-    }
-    const long g = (long)e * ne + n;
-    double q[NV], ja[9], Finv[3][NV];
-    if (active) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
-        hx::euler_contravariant(ph, q, ja, Finv);
-        double Fvc[3][NV];
-        if (!ph.euler) {
-            double gr[3][NV];
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) gr[d][v] = grad[(5 * d + v) * S + g];
-            double mu, kappa;
-            hx::laminar_mu_kappa(ph, q, mu, kappa);
-            if (ph.les) {
-                const double mt = hx::les_viscosity(ph, edelta[e], q, gr[0], gr[1], gr[2]);
-                mu += mt;
-                kappa += mt * ph.mu_to_kappa;
-            }
-            double Fv[NV][3];
-            hx::viscous_flux(ph, q, gr[0], gr[1], gr[2], mu, kappa, Fv);
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    Fvc[d][v] = Fv[v][0] * ja[3 * d] + Fv[v][1] * ja[3 * d + 1] + Fv[v][2] * ja[3 * d + 2];
-        } else {
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) Fvc[d][v] = 0.0;
-        }
-        if (!SPLIT) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) sF[el][d][v][n] = Finv[d][v] - Fvc[d][v];
-        } else {
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) sF[el][d][v][n] = Fvc[d][v];
-            // node data of the two-point fluxes (invRho, u, v, w, p as RiemannSolvers_NS.f90:2323-2329)
-            const double ir = 1.0 / q[0];
-            const double u = ir * q[1], v = ir * q[2], w = ir * q[3];
-            const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
-#pragma unroll
-            for (int c = 0; c < NV; ++c) sP[el][c][n] = q[c];
-            sP[el][5][n] = u; sP[el][6][n] = v; sP[el][7][n] = w; sP[el][8][n] = p; sP[el][9][n] = ir;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) sP[el][10 + c][n] = ja[c];
-        }
-    }
-    __syncthreads();
-    if (!active) return;
-    const int i = n % n1, j = (n / n1) % n1, k = n / nf;
-    double qd[NV];
-    if (!SPLIT) {
-        // ScalarWeakIntegrals_StdVolumeGreen (DGIntegrals.f90:56-87): xi, then eta, then zeta sums
-#pragma unroll
-        for (int v = 0; v < NV; ++v) qd[v] = 0.0;
-#pragma unroll
-        for (int l = 0; l < n1; ++l) {
-            const double hi = sHat[i * n1 + l];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hi * sF[el][0][v][l + n1 * (j + n1 * k)];
-        }
-#pragma unroll
-        for (int l = 0; l < n1; ++l) {
-            const double hj = sHat[j * n1 + l];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hj * sF[el][1][v][i + n1 * (l + n1 * k)];
-        }
-#pragma unroll
-        for (int l = 0; l < n1; ++l) {
-            const double hk = sHat[k * n1 + l];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hk * sF[el][2][v][i + n1 * (j + n1 * l)];
-        }
-    } else {
-        // ScalarWeakIntegrals_SplitVolumeDivergence (DGIntegrals.f90:92-129), QDot = -volInt,
-        // two-point fluxes (SplitDG_ComputeSplitFormFluxes, HyperbolicSplitForm.f90:64-116)
-        // evaluated on the fly; the diagonal is the inviscid contravariant flux.
-        double acc[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-        double pa[5];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) pa[c] = sP[el][5 + c][n];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-#pragma unroll
-            for (int l = 0; l < n1; ++l) {
-                const int own = d == 0 ? i : (d == 1 ? j : k);
-                const int m = d == 0 ? l + n1 * (j + n1 * k) : (d == 1 ? i + n1 * (l + n1 * k) : i + n1 * (j + n1 * l));
-                const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
-                double fs[NV];
-                if (l == own) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) fs[v] = Finv[d][v];
-                } else {
-                    double qb[NV], pb[5], jb[3];
-#pragma unroll
-                    for (int c = 0; c < NV; ++c) qb[c] = sP[el][c][m];
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) pb[c] = sP[el][5 + c][m];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) jb[c] = sP[el][10 + 3 * d + c][m];
-                    hx::two_point_flux(ph, q, pa, &ja[3 * d], qb, pb, jb, fs);
-                }
-#pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + sh * fs[v] + hh * sF[el][d][v][m];
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) qd[v] = -acc[v];
-    }
-    // Surface integral (TimeDerivative_FacesContribution, SD.f90:1686-1701): for GL nodes only
-    // boundary nodes receive b = 1/w_0 times the stored face flux (negated on the right side).
-    const bool onface = (i == 0) | (i == N) | (j == 0) | (j == N) | (k == 0) | (k == N);
-    if (onface) {
-        const double b0 = 1.0 / ops.w[0];
-        double x[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        auto surf = [&](int slot, int a, int b) {
-            const int code = efn[((long)e * 6 + slot) * nf + a + n1 * b];
-            const long fq = code >> 1;
-            const double sg = (code & 1) ? -b0 : b0;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) x[v] = x[v] + fstar[v * FS + fq] * sg;
-        };
-        if (i == 0) surf(5, j, k);
-        if (i == N) surf(3, j, k);
-        if (j == 0) surf(0, i, k);
-        if (j == N) surf(1, i, k);
-        if (k == 0) surf(2, i, j);
-        if (k == N) surf(4, i, j);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) qd[v] = qd[v] - x[v];
-    }
-    const double jg = Jac[g];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) qd[v] = qd[v] / jg;
-    if (qdot_out) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) qdot_out[v * S + g] = qd[v];
-    }
-    if (update) {
-        const double dt = dt_dev ? dt_dev[0] : dt_host;
-        const double cdt = c_k * dt;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double gv = a_k * G[v * S + g] + qd[v];
-            G[v * S + g] = gv;
-            Q[v * S + g] = q[v] + cdt * gv;
-        }
-    }
-}
 
 // ------------------------------------------------------------------------------------
 // reductions and small kernels
@@ -476,9 +117,9 @@ k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doubl
     }
 }
 
-__global__ void k_dt_reset(unsigned long long *bits)
+__global__ void k_dt_reset(unsigned long long *bits, int n)
 {
-    if (threadIdx.x < 2) bits[threadIdx.x] = 0x7FF0000000000000ull;  // +inf
+    if ((int)threadIdx.x < n) bits[threadIdx.x] = 0x7FF0000000000000ull;  // +inf
 }
 
 // dt = min(convective, viscous) (DGSEMClass.f90:1024-1030); with advance != 0 also
@@ -694,9 +335,10 @@ struct hx3d_engine {
     Phys ph{};
     Ops ops{};
     int inviscid = 0;
+    int vol_tile = 0;   // 0: default elements per block of the volume kernel, 1: half
     double dcsi = 0;
     hipStream_t stream = nullptr;
-    double *dQ = nullptr, *dG = nullptr, *dGrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
+    double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
     double *dFstar = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
     double *dJ = nullptr, *dFgeo = nullptr;
     double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr;
@@ -731,73 +373,97 @@ void launch_grad(hx3d_engine *eng)
     constexpr int EPB = epb_for(N), ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
-    hipLaunchKernelGGL((k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
-                       eng->dInvJ, eng->dNbr, eng->dEfn, eng->dFgeo, eng->dGrad, eng->K, eng->S,
-                       eng->FS, eng->ops);
+    hipLaunchKernelGGL((hxk::k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
+                       eng->dInvJ, eng->dNbr, eng->dEfn, eng->dFgeo, eng->dVgrad, eng->K, eng->S,
+                       eng->FS, eng->ops, eng->ph);
 }
 
 template <int N>
 void launch_face(hx3d_engine *eng)
 {
     const long nb = (eng->FS + 255) / 256;
-    hipLaunchKernelGGL((k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dGrad, eng->dFgeo,
-                       eng->dFnode, eng->dFdelta, eng->dFstar, eng->FS, eng->S, eng->ph);
+    hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dVgrad,
+                       eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFstar, eng->FS, eng->S, eng->ph);
 }
 
-template <int N>
-void launch_volume(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev,
-                   double dt_host, double *qdot)
+template <int N, int FORM, int EPB>
+void launch_volume_e(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
-    constexpr int EPB = epb_for(N), ne = (N + 1) * (N + 1) * (N + 1);
+    constexpr int ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
-    if (eng->inviscid == HX3D_INVISCID_SPLIT)
-        hipLaunchKernelGGL((k_volume<N, EPB, true>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
-                           eng->dGrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dEfn, eng->dFstar,
-                           eng->dG, qdot, dt_dev, dt_host, a_k, c_k, update, eng->K, eng->S, eng->FS,
-                           eng->ops, eng->ph);
-    else
-        hipLaunchKernelGGL((k_volume<N, EPB, false>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
-                           eng->dGrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dEfn, eng->dFstar,
-                           eng->dG, qdot, dt_dev, dt_host, a_k, c_k, update, eng->K, eng->S, eng->FS,
-                           eng->ops, eng->ph);
+    hipLaunchKernelGGL((hxk::k_volume<N, EPB, FORM>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
+                       eng->dVgrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dEfn, eng->dFstar, eng->dG,
+                       qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->FS, eng->ops, eng->ph);
+}
+
+// Two compiled tilings of the volume kernel per order: the default elements per block
+// (epb_for) and a half-size block (vol_tile = 1), selectable per engine.
+template <int N, int FORM>
+void launch_volume_f(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
+{
+    constexpr int E0 = epb_for(N), E1 = E0 > 1 ? E0 / 2 : 1;
+    if (eng->vol_tile == 1) launch_volume_e<N, FORM, E1>(eng, a_k, c_k, update, dta, qdot);
+    else launch_volume_e<N, FORM, E0>(eng, a_k, c_k, update, dta, qdot);
 }
 
 template <int N>
-void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev,
-             double dt_host, double *qdot, hipEvent_t *ev)
+void launch_volume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
+{
+    if (eng->inviscid != HX3D_INVISCID_SPLIT) return launch_volume_f<N, hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
+    switch (eng->ph.averaging) {
+    case HX3D_AVG_DUCROS: return launch_volume_f<N, hxk::AVG_DUCROS>(eng, a_k, c_k, update, dta, qdot);
+    case HX3D_AVG_KENNEDYGRUBER: return launch_volume_f<N, hxk::AVG_KG>(eng, a_k, c_k, update, dta, qdot);
+    case HX3D_AVG_PIROZZOLI: return launch_volume_f<N, hxk::AVG_PIROZZOLI>(eng, a_k, c_k, update, dta, qdot);
+    default: return launch_volume_f<N, hxk::AVG_STANDARD>(eng, a_k, c_k, update, dta, qdot);
+    }
+}
+
+template <int N>
+void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
+             hipEvent_t *ev)
 {
     if (ev) hipEventRecord(ev[0], eng->stream);
     if (!eng->ph.euler) launch_grad<N>(eng);
     if (ev) hipEventRecord(ev[1], eng->stream);
     launch_face<N>(eng);
     if (ev) hipEventRecord(ev[2], eng->stream);
-    launch_volume<N>(eng, a_k, c_k, update, dt_dev, dt_host, qdot);
+    launch_volume<N>(eng, a_k, c_k, update, dta, qdot);
     if (ev) hipEventRecord(ev[3], eng->stream);
 }
 
 // ev: NULL, or 4 events recorded around the three launches of the stage
-void stage(hx3d_engine *eng, double a_k, double c_k, int update, const double *dt_dev, double dt_host,
-           double *qdot, hipEvent_t *ev = nullptr)
+void stage(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
+           hipEvent_t *ev = nullptr)
 {
     switch (eng->N) {
-    case 1: stage_n<1>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
-    case 2: stage_n<2>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
-    case 3: stage_n<3>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
-    case 4: stage_n<4>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
-    case 5: stage_n<5>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
-    case 6: stage_n<6>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
-    default: stage_n<7>(eng, a_k, c_k, update, dt_dev, dt_host, qdot, ev); break;
+    case 1: stage_n<1>(eng, a_k, c_k, update, dta, qdot, ev); break;
+    case 2: stage_n<2>(eng, a_k, c_k, update, dta, qdot, ev); break;
+    case 3: stage_n<3>(eng, a_k, c_k, update, dta, qdot, ev); break;
+    case 4: stage_n<4>(eng, a_k, c_k, update, dta, qdot, ev); break;
+    case 5: stage_n<5>(eng, a_k, c_k, update, dta, qdot, ev); break;
+    case 6: stage_n<6>(eng, a_k, c_k, update, dta, qdot, ev); break;
+    default: stage_n<7>(eng, a_k, c_k, update, dta, qdot, ev); break;
     }
 }
 
-void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int advance)
+DtArgs host_dt(double dt)
 {
-    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, eng->dBits);
+    DtArgs a{};
+    a.host_dt = dt;
+    return a;
+}
+
+// Standalone MaxTimeStep of the resident state into dt slot `slot` (bits reset first);
+// k_dt_finish then writes dt to dScal[0].
+void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
+{
+    unsigned long long *b = eng->dBits + 2 * slot;
+    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, b, 2);
     const long nb = (eng->nodes + 255) / 256;
     hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
-                       eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dBits);
-    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, eng->dBits, eng->dScal, advance);
+                       eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, b);
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, b, eng->dScal, 0);
 }
 
 // Williamson RK3 coefficients (ExplicitMethods.f90:697-699)
@@ -843,6 +509,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     eng->S = ((eng->nodes + 63) / 64) * 64;
     eng->FS = (long)eng->F * eng->nf;
     eng->inviscid = d->inviscid;
+    if (const char *vt = std::getenv("HX3D_VOLUME_TILE")) eng->vol_tile = std::atoi(vt) == 1 ? 1 : 0;
     eng->ops = make_ops(N);
     {
         double x[8], w[8];
@@ -969,13 +636,13 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     // ---- device -------------------------------------------------------------------
     int rc;
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
-        (rc = dalloc(eng, &eng->dGrad, (size_t)15 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
+        (rc = dalloc(eng, &eng->dVgrad, (size_t)12 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
         (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
         (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * FS)) ||
         (rc = dalloc(eng, &eng->dFstar, (size_t)NV * FS)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) || (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) ||
         (rc = dalloc(eng, &eng->dScal, 4)) || (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
-        (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2)) ||
+        (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 8)) ||
         (rc = dalloc(eng, &eng->dNbr, nbr.size())) || (rc = dalloc(eng, &eng->dEfn, efn.size())) ||
         (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
         return bail(rc);
@@ -1005,7 +672,7 @@ int hx3d_engine_destroy(hx3d_engine *eng)
 {
     if (!eng) return HX3D_OK;
     if (eng->stream) hipStreamSynchronize(eng->stream);
-    void *ptrs[] = {eng->dQ, eng->dG, eng->dGrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dFstar, eng->dQDot,
+    void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dFstar, eng->dQDot,
                     eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dEfn, eng->dFlag, eng->dFnode};
     for (void *p : ptrs)
@@ -1062,7 +729,7 @@ int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host)
 {
     (void)t;   // no source terms on this path (UserDefinedSourceTermNS is zero)
     if (!eng) return HX3D_EINVAL;
-    stage(eng, 0.0, 0.0, 0, nullptr, 0.0, eng->dQDot);
+    stage(eng, 0.0, 0.0, 0, host_dt(0.0), eng->dQDot);
     HIPCHK(hipGetLastError());
     eng->qdot_valid = true;
     if (QDot_host) {
@@ -1078,7 +745,7 @@ int hx3d_rk3_step(hx3d_engine *eng, double t, double dt)
 {
     (void)t;
     if (!eng) return HX3D_EINVAL;
-    for (int k = 0; k < 3; ++k) stage(eng, RK_A[k], RK_C[k], 1, nullptr, dt, k == 2 ? eng->dQDot : nullptr);
+    for (int k = 0; k < 3; ++k) stage(eng, RK_A[k], RK_C[k], 1, host_dt(dt), k == 2 ? eng->dQDot : nullptr);
     eng->qdot_valid = false;
     HIPCHK(hipGetLastError());
     return HX3D_OK;
@@ -1087,7 +754,7 @@ int hx3d_rk3_step(hx3d_engine *eng, double t, double dt)
 int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt)
 {
     if (!eng || !dt) return HX3D_EINVAL;
-    launch_maxdt(eng, cfl, dcfl, 0);
+    launch_maxdt(eng, cfl, dcfl, 3);
     HIPCHK(hipMemcpyAsync(dt, eng->dScal, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     return HX3D_OK;
@@ -1103,11 +770,25 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
         HIPCHK(hipEventCreate(&e));
         eng->evpool.push_back(e);
     }
+    // dt of step s: the minimum of the limits in slot s%3 (slot 0 from a standalone MaxTimeStep
+    // of the current state; later slots accumulated by the last stage of the previous step
+    // from the state it produces).  Stage 1 of step s resets slot (s+2)%3 for step s+1.
+    if (nsteps > 0) {
+        hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, eng->dBits, 6);
+        const long nbn = (eng->nodes + 255) / 256;
+        hipLaunchKernelGGL(k_maxdt, dim3(nbn), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
+                           eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dBits);
+    }
     for (int s = 0; s < nsteps; ++s) {
-        launch_maxdt(eng, cfl, dcfl, 1);
-        for (int k = 0; k < 3; ++k)
-            stage(eng, RK_A[k], RK_C[k], 1, eng->dScal, 0.0, nullptr,
+        for (int k = 0; k < 3; ++k) {
+            DtArgs a{};
+            a.in = eng->dBits + 2 * (s % 3);
+            a.cfl = cfl; a.dcfl = dcfl; a.dcsi = eng->dcsi;
+            if (k == 0) a.reset = eng->dBits + 2 * ((s + 2) % 3);
+            if (k == 2) { a.out = eng->dBits + 2 * ((s + 1) % 3); a.time = eng->dScal + 1; }
+            stage(eng, RK_A[k], RK_C[k], 1, a, nullptr,
                   eng->timing ? &eng->evpool[(size_t)12 * s + 4 * k] : nullptr);
+        }
     }
     HIPCHK(hipGetLastError());
     if (eng->timing && nsteps > 0) {

@@ -56,57 +56,60 @@ HX_HD void euler_contravariant(const Phys &ph, const double q[NV], const double 
             out[d][c] = F[c][0] * ja[3 * d] + F[c][1] * ja[3 * d + 1] + F[c][2] * ja[3 * d + 2];
 }
 
-// navierstokes/Physics_NS.f90:246-304 ViscousFlux_STATE (beta = 0).  F[v][c].
-HX_HD void viscous_flux(const Phys &ph, const double q[NV], const double gx[NV], const double gy[NV],
-                        const double gz[NV], double mu, double kappa, double F[NV][3])
+// navierstokes/Physics_NS.f90:246-304 ViscousFlux_STATE (beta = 0), split in two halves:
+//   viscous_state   the velocity gradients u_x, u_y, u_z and grad T it forms from Q and the
+//                   lifted state gradients: vg[0..2] = u_x, vg[3..5] = u_y, vg[6..8] = u_z,
+//                   vg[9..11] = grad T;
+//   viscous_flux    the stress / heat-flux tail F[v][c] from mu, kappa, the velocity and vg.
+// Together they perform the reference's operations in its order.
+HX_HD void viscous_state(const Phys &ph, const double q[NV], const double gx[NV], const double gy[NV],
+                         const double gz[NV], double vg[12])
 {
     const double ir = 1.0 / q[0];
     const double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
     const double ud[3] = {u * ir, v * ir, w * ir};
-    double ux[3], uy[3], uz[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        ux[d] = ir * gx[1 + d] - ud[d] * gx[0];
-        uy[d] = ir * gy[1 + d] - ud[d] * gy[0];
-        uz[d] = ir * gz[1 + d] - ud[d] * gz[0];
+        vg[d] = ir * gx[1 + d] - ud[d] * gx[0];
+        vg[3 + d] = ir * gy[1 + d] - ud[d] * gy[0];
+        vg[6 + d] = ir * gz[1 + d] - ud[d] * gz[0];
     }
     const double g = ph.gm1 * ph.gammaM2;
-    const double nTx = g * (ir * gx[4] - q[4] * ir * ir * gx[0] - u * ux[0] - v * ux[1] - w * ux[2]);
-    const double nTy = g * (ir * gy[4] - q[4] * ir * ir * gy[0] - u * uy[0] - v * uy[1] - w * uy[2]);
-    const double nTz = g * (ir * gz[4] - q[4] * ir * ir * gz[0] - u * uz[0] - v * uz[1] - w * uz[2]);
+    vg[9] = g * (ir * gx[4] - q[4] * ir * ir * gx[0] - u * vg[0] - v * vg[1] - w * vg[2]);
+    vg[10] = g * (ir * gy[4] - q[4] * ir * ir * gy[0] - u * vg[3] - v * vg[4] - w * vg[5]);
+    vg[11] = g * (ir * gz[4] - q[4] * ir * ir * gz[0] - u * vg[6] - v * vg[7] - w * vg[8]);
+}
+
+HX_HD void viscous_flux(double mu, double kappa, double u, double v, double w, const double vg[12],
+                        double F[NV][3])
+{
+    const double *ux = vg, *uy = vg + 3, *uz = vg + 6;
     const double divV = ux[0] + uy[1] + uz[2];
     F[0][0] = 0.0;
     F[1][0] = mu * (2.0 * ux[0] - 2.0 / 3.0 * divV);
     F[2][0] = mu * (ux[1] + uy[0]);
     F[3][0] = mu * (ux[2] + uz[0]);
-    F[4][0] = F[1][0] * u + F[2][0] * v + F[3][0] * w + kappa * nTx;
+    F[4][0] = F[1][0] * u + F[2][0] * v + F[3][0] * w + kappa * vg[9];
     F[0][1] = 0.0;
     F[1][1] = F[2][0];
     F[2][1] = mu * (2.0 * uy[1] - 2.0 / 3.0 * divV);
     F[3][1] = mu * (uy[2] + uz[1]);
-    F[4][1] = F[1][1] * u + F[2][1] * v + F[3][1] * w + kappa * nTy;
+    F[4][1] = F[1][1] * u + F[2][1] * v + F[3][1] * w + kappa * vg[10];
     F[0][2] = 0.0;
     F[1][2] = F[3][0];
     F[2][2] = F[3][1];
     F[3][2] = mu * (2.0 * uz[2] - 2.0 / 3.0 * divV);
-    F[4][2] = F[1][2] * u + F[2][2] * v + F[3][2] * w + kappa * nTz;
+    F[4][2] = F[1][2] * u + F[2][2] * v + F[3][2] * w + kappa * vg[11];
 }
 
 // common/LESModels.f90:256-307 Smagorinsky and :358-~440 WALE eddy viscosity (no wall
 // model), velocity gradients of getVelocityGradients_State
 // (navierstokes/VariableConversion_NS.f90:373-392).
-HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], const double gx[NV],
-                           const double gy[NV], const double gz[NV])
+HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], const double vg[12])
 {
-    const double ir = 1.0 / q[0], ir2 = ir * ir;
-    const double ud[3] = {q[1] * ir2, q[2] * ir2, q[3] * ir2};
-    double ux[3], uy[3], uz[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-        ux[b] = ir * gx[1 + b] - ud[b] * gx[0];
-        uy[b] = ir * gy[1 + b] - ud[b] * gy[0];
-        uz[b] = ir * gz[1 + b] - ud[b] * gz[0];
-    }
+    // velocity gradients: the viscous-flux form of vg (they agree with
+    // getVelocityGradients_State up to the rounding of u/rho vs Q/rho^2)
+    const double *ux = vg, *uy = vg + 3, *uz = vg + 6;
     if (ph.les == 1) {
         // S(:,1) = U_x, S(:,2) = U_y, S(:,3) = U_z; S(1,:) += U_x ...; S = S/2; |S| = sqrt(2 sum S*S)
         double S[3][3];
@@ -294,13 +297,14 @@ HX_HD void lxf_flux(const Phys &ph, const double QL[NV], const double QR[NV], co
 // Two-point volume fluxes, RiemannSolvers_NS.f90:2085-2438 (Standard, Ducros, Kennedy-Gruber,
 // Pirozzoli): fSharp = f Ja_x + g Ja_y + h Ja_z of the averaged states and metrics.
 // q: conservative state; p: (u, v, w, pressure, 1/rho) as the reference computes them there.
-HX_HD void two_point_flux(const Phys &ph, const double *qa, const double *pa, const double *ja,
-                          const double *qb, const double *pb, const double *jb, double fs[NV])
+template <int AVG>
+HX_HD void two_point_flux_t(const double *qa, const double *pa, const double *ja, const double *qb,
+                            const double *pb, const double *jb, double fs[NV])
 {
     const double uL = pa[0], vL = pa[1], wL = pa[2], pL = pa[3];
     const double uR = pb[0], vR = pb[1], wR = pb[2], pR = pb[3];
     double f[NV], g[NV], h[NV], J[3];
-    switch (ph.averaging) {
+    switch (AVG) {
     case 1:
 #pragma unroll
         for (int d = 0; d < 3; ++d) J[d] = 0.5 * (ja[d] + jb[d]);
@@ -328,14 +332,14 @@ HX_HD void two_point_flux(const Phys &ph, const double *qa, const double *pa, co
         const double u = 0.5 * (uL + uR), v = 0.5 * (vL + vR), w = 0.5 * (wL + wR);
         const double p = 0.5 * (pL + pR);
         double e;
-        if (ph.averaging == 2) e = 0.5 * (qa[4] * pa[4] + qb[4] * pb[4]);
+        if (AVG == 2) e = 0.5 * (qa[4] * pa[4] + qb[4] * pb[4]);
         else e = 0.5 * ((qa[4] + pL) * pa[4] + (qb[4] + pR) * pb[4]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) J[d] = 0.5 * (ja[d] + jb[d]);
         f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * v; f[3] = rho * u * w;
         g[0] = rho * v; g[1] = rho * v * u; g[2] = rho * v * v + p; g[3] = rho * v * w;
         h[0] = rho * w; h[1] = rho * w * u; h[2] = rho * w * v; h[3] = rho * w * w + p;
-        if (ph.averaging == 2) {
+        if (AVG == 2) {
             f[4] = rho * u * e + p * u; g[4] = rho * v * e + p * v; h[4] = rho * w * e + p * w;
         } else {
             f[4] = rho * u * e; g[4] = rho * v * e; h[4] = rho * w * e;
