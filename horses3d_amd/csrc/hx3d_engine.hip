@@ -122,6 +122,12 @@ __global__ void k_dt_reset(unsigned long long *bits, int n)
     if ((int)threadIdx.x < n) bits[threadIdx.x] = 0x7FF0000000000000ull;  // +inf
 }
 
+// a host-given dt as both limits of a dt slot
+__global__ void k_dt_set(unsigned long long *bits, double dt)
+{
+    if (threadIdx.x < 2) bits[threadIdx.x] = (unsigned long long)__double_as_longlong(dt);
+}
+
 // dt = min(convective, viscous) (DGSEMClass.f90:1024-1030); with advance != 0 also
 // time += dt (device scalars: [0] dt, [1] t)
 __global__ void k_dt_finish(const unsigned long long *bits, double *sc, int advance)
@@ -331,7 +337,7 @@ void left_to_right(int i, int j, int Nx, int Ny, int rot, int &ii, int &jj)
 
 struct hx3d_engine {
     int K = 0, N = 0, n1 = 0, nf = 0, ne = 0, F = 0;
-    long nodes = 0, S = 0, FS = 0;
+    long nodes = 0, S = 0, FS = 0, ES = 0;
     Phys ph{};
     Ops ops{};
     int inviscid = 0;
@@ -339,12 +345,12 @@ struct hx3d_engine {
     double dcsi = 0;
     hipStream_t stream = nullptr;
     double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
-    double *dFstar = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
-    double *dJ = nullptr, *dFgeo = nullptr;
+    double *dFe = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
+    double *dJ = nullptr, *dFgeo = nullptr, *dEgeo = nullptr;
     double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr;
     unsigned long long *dBits = nullptr;
-    int *dNbr = nullptr, *dEfn = nullptr, *dFlag = nullptr;
-    int2 *dFnode = nullptr;
+    int *dNbr = nullptr, *dFlag = nullptr;
+    int4 *dFnode = nullptr;
     bool qdot_valid = false, timing = false;
     double ktime[4] = {0, 0, 0, 0};
     std::vector<hipEvent_t> evpool;   // timing: 4 events per stage of the last hx3d_advance
@@ -374,16 +380,17 @@ void launch_grad(hx3d_engine *eng)
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
     hipLaunchKernelGGL((hxk::k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
-                       eng->dInvJ, eng->dNbr, eng->dEfn, eng->dFgeo, eng->dVgrad, eng->K, eng->S,
-                       eng->FS, eng->ops, eng->ph);
+                       eng->dInvJ, eng->dNbr, eng->dEgeo, eng->dVgrad, eng->K, eng->S, eng->ES,
+                       eng->ops, eng->ph);
 }
 
 template <int N>
 void launch_face(hx3d_engine *eng)
 {
-    const long nb = (eng->FS + 255) / 256;
+    const int nb = grid_for((int)((eng->FS + 255) / 256));
     hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dVgrad,
-                       eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFstar, eng->FS, eng->S, eng->ph);
+                       eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
+                       1.0 / eng->ops.w[0], eng->ph);
 }
 
 template <int N, int FORM, int EPB>
@@ -393,8 +400,8 @@ void launch_volume_e(hx3d_engine *eng, double a_k, double c_k, int update, const
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
     hipLaunchKernelGGL((hxk::k_volume<N, EPB, FORM>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
-                       eng->dVgrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dEfn, eng->dFstar, eng->dG,
-                       qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->FS, eng->ops, eng->ph);
+                       eng->dVgrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dG,
+                       qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->ES, eng->ops, eng->ph);
 }
 
 // Two compiled tilings of the volume kernel per order: the default elements per block
@@ -447,10 +454,16 @@ void stage(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &d
     }
 }
 
-DtArgs host_dt(double dt)
+// dt slots (pairs of bit patterns) in dBits: 0-2 rotate through hx3d_advance, 3 is
+// hx3d_max_dt's, 4 holds a host-given dt
+constexpr int DT_SLOTS = 5;
+
+DtArgs host_dt(hx3d_engine *eng, double dt)
 {
+    unsigned long long *b = eng->dBits + 2 * 4;
+    hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, eng->stream, b, dt);
     DtArgs a{};
-    a.host_dt = dt;
+    a.in = b;
     return a;
 }
 
@@ -508,6 +521,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     eng->nodes = (long)eng->K * eng->ne;
     eng->S = ((eng->nodes + 63) / 64) * 64;
     eng->FS = (long)eng->F * eng->nf;
+    eng->ES = (((long)eng->K * 6 * eng->nf + 63) / 64) * 64;
     eng->inviscid = d->inviscid;
     if (const char *vt = std::getenv("HX3D_VOLUME_TILE")) eng->vol_tile = std::atoi(vt) == 1 ? 1 : 0;
     eng->ops = make_ops(N);
@@ -534,7 +548,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     ph.les = d->euler ? 0 : d->les;
 
     const int K = eng->K, n1 = eng->n1, nf = eng->nf, ne = eng->ne, F = eng->F;
-    const long S = eng->S, nodes = eng->nodes, FS = eng->FS;
+    const long S = eng->S, nodes = eng->nodes, FS = eng->FS, ES = eng->ES;
     // ---- host tables --------------------------------------------------------------
     auto node_of = [&](int e, int slot, int a, int b) -> long {
         int idx[3];
@@ -543,8 +557,10 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         idx[SLOT_V[slot]] = b;
         return (long)e * ne + idx[0] + n1 * (idx[1] + n1 * idx[2]);
     };
-    std::vector<int> nbr((size_t)K * 6 * nf, -1), efn((size_t)K * 6 * nf, -1);
-    std::vector<int2> fnode((size_t)FS);
+    // per element face node x = (6e + slot) nf + a + (N+1) b: nbr[x] = 2 * (neighbour node) +
+    // (1 on the face's right side); per face node: (left node, right node, left x, right x)
+    std::vector<int> nbr((size_t)K * 6 * nf, -1);
+    std::vector<int4> fnode((size_t)FS);
     std::vector<unsigned char> fslot(F);
     for (int f = 0; f < F; ++f) {
         const int eL = d->fElem[2 * f], eR = d->fElem[2 * f + 1];
@@ -561,20 +577,20 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
                 left_to_right(i, j, N, N, rot, ii, jj);
                 const long gL = node_of(eL, sL, i, j), gR = node_of(eR, sR, ii, jj);
                 const long q = (long)f * nf + i + n1 * j;
-                fnode[q] = make_int2((int)gL, (int)gR);
-                nbr[((size_t)eL * 6 + sL) * nf + i + n1 * j] = (int)gR;
-                nbr[((size_t)eR * 6 + sR) * nf + ii + n1 * jj] = (int)gL;
-                efn[((size_t)eL * 6 + sL) * nf + i + n1 * j] = (int)(2 * q);
-                efn[((size_t)eR * 6 + sR) * nf + ii + n1 * jj] = (int)(2 * q + 1);
+                const long xL = ((long)eL * 6 + sL) * nf + i + n1 * j;
+                const long xR = ((long)eR * 6 + sR) * nf + ii + n1 * jj;
+                fnode[q] = make_int4((int)gL, (int)gR, (int)xL, (int)xR);
+                nbr[xL] = (int)(2 * gR);
+                nbr[xR] = (int)(2 * gL + 1);
             }
     }
     for (size_t x = 0; x < nbr.size(); ++x)
-        if (nbr[x] < 0 || efn[x] < 0) return bail(eng->fail(HX3D_EINVAL, "an element face node has no neighbour"));
-    if (nodes > 0x7fffffffL || 2 * FS > 0x7fffffffL)
+        if (nbr[x] < 0) return bail(eng->fail(HX3D_EINVAL, "an element face node has no neighbour"));
+    if (2 * nodes > 0x7fffffffL || ES > 0x7fffffffL)
         return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit node indices"));
     // geometry SoA, J, 1/J, quadrature weights, LES filter widths, face frames
     std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), jj((size_t)S, 1.0), wq((size_t)S, 0.0),
-        edelta(K), fdelta(F), fgeo((size_t)10 * FS, 0.0);
+        edelta(K), fdelta(F), fgeo((size_t)10 * FS, 0.0), egeo((size_t)4 * ES, 0.0);
     double xg[8], wg[8];
     gl_nodes(N, xg, wg);
     for (long g = 0; g < nodes; ++g) {
@@ -629,6 +645,12 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
                     fgeo[(size_t)(6 + c) * FS + q] = t2[c];
                 }
                 fgeo[(size_t)9 * FS + q] = jac;
+                for (int c = 0; c < 3; ++c) {
+                    egeo[(size_t)c * ES + fnode[q].z] = nr[c];
+                    egeo[(size_t)c * ES + fnode[q].w] = nr[c];
+                }
+                egeo[(size_t)3 * ES + fnode[q].z] = jac;
+                egeo[(size_t)3 * ES + fnode[q].w] = jac;
                 surf += wg[i] * wg[j] * jac;
             }
         fdelta[f] = std::sqrt(surf / nf);   // SD.f90:1378
@@ -639,11 +661,11 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         (rc = dalloc(eng, &eng->dVgrad, (size_t)12 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
         (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
         (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * FS)) ||
-        (rc = dalloc(eng, &eng->dFstar, (size_t)NV * FS)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
+        (rc = dalloc(eng, &eng->dFe, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dEgeo, (size_t)4 * ES)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) || (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) ||
         (rc = dalloc(eng, &eng->dScal, 4)) || (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
-        (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 8)) ||
-        (rc = dalloc(eng, &eng->dNbr, nbr.size())) || (rc = dalloc(eng, &eng->dEfn, efn.size())) ||
+        (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
+        (rc = dalloc(eng, &eng->dNbr, nbr.size())) ||
         (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
         return bail(rc);
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
@@ -655,7 +677,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         !up(eng->dJ, jj.data(), jj.size() * 8) || !up(eng->dFgeo, fgeo.data(), fgeo.size() * 8) ||
         !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
         !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dNbr, nbr.data(), nbr.size() * 4) ||
-        !up(eng->dEfn, efn.data(), efn.size() * 4) || !up(eng->dFnode, fnode.data(), fnode.size() * 8))
+        !up(eng->dEgeo, egeo.data(), egeo.size() * 8) || !up(eng->dFnode, fnode.data(), fnode.size() * 16))
         return bail(eng->fail(HX3D_EDEVICE, "upload failed"));
     // zero state, RK register and time
     const long nb = (NV * S + 255) / 256;
@@ -672,9 +694,9 @@ int hx3d_engine_destroy(hx3d_engine *eng)
 {
     if (!eng) return HX3D_OK;
     if (eng->stream) hipStreamSynchronize(eng->stream);
-    void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dFstar, eng->dQDot,
-                    eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
-                    eng->dBits, eng->dNbr, eng->dEfn, eng->dFlag, eng->dFnode};
+    void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
+                    eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
+                    eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &e : eng->evpool) hipEventDestroy(e);
@@ -729,7 +751,7 @@ int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host)
 {
     (void)t;   // no source terms on this path (UserDefinedSourceTermNS is zero)
     if (!eng) return HX3D_EINVAL;
-    stage(eng, 0.0, 0.0, 0, host_dt(0.0), eng->dQDot);
+    stage(eng, 0.0, 0.0, 0, host_dt(eng, 0.0), eng->dQDot);
     HIPCHK(hipGetLastError());
     eng->qdot_valid = true;
     if (QDot_host) {
@@ -745,7 +767,8 @@ int hx3d_rk3_step(hx3d_engine *eng, double t, double dt)
 {
     (void)t;
     if (!eng) return HX3D_EINVAL;
-    for (int k = 0; k < 3; ++k) stage(eng, RK_A[k], RK_C[k], 1, host_dt(dt), k == 2 ? eng->dQDot : nullptr);
+    const DtArgs a = host_dt(eng, dt);
+    for (int k = 0; k < 3; ++k) stage(eng, RK_A[k], RK_C[k], 1, a, k == 2 ? eng->dQDot : nullptr);
     eng->qdot_valid = false;
     HIPCHK(hipGetLastError());
     return HX3D_OK;

@@ -295,45 +295,88 @@ HX_HD void lxf_flux(const Phys &ph, const double QL[NV], const double QR[NV], co
 }
 
 // Two-point volume fluxes, RiemannSolvers_NS.f90:2085-2438 (Standard, Ducros, Kennedy-Gruber,
-// Pirozzoli): fSharp = f Ja_x + g Ja_y + h Ja_z of the averaged states and metrics.
-// q: conservative state; p: (u, v, w, pressure, 1/rho) as the reference computes them there.
+// Pirozzoli): fSharp = f Ja_x + g Ja_y + h Ja_z of the averaged states and metrics, evaluated
+// from per-node data with the reference's operation order.  node_data() forms, once per node,
+// what the flux of that averaging reads of a node (u, v, w, p as RiemannSolvers_NS.f90:2323-2329;
+// the energy factors e = rhoE/rho and h = (rhoE + p)/rho of Kennedy-Gruber and Pirozzoli are the
+// products the reference forms inside the flux, taken out of the pair loop).
+constexpr int node_data_size(int avg) { return avg == 0 ? 8 : avg == 1 ? 9 : 6; }
+
 template <int AVG>
-HX_HD void two_point_flux_t(const double *qa, const double *pa, const double *ja, const double *qb,
-                            const double *pb, const double *jb, double fs[NV])
+HX_HD void node_data(const Phys &ph, const double q[NV], double *a)
 {
-    const double uL = pa[0], vL = pa[1], wL = pa[2], pL = pa[3];
-    const double uR = pb[0], vR = pb[1], wR = pb[2], pR = pb[3];
+    const double ir = 1.0 / q[0];
+    const double u = ir * q[1], v = ir * q[2], w = ir * q[3];
+    const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
+    if (AVG == 0) {          // rho u, rho v, rho w, rho e, u, v, w, p
+        a[0] = q[1]; a[1] = q[2]; a[2] = q[3]; a[3] = q[4];
+        a[4] = u; a[5] = v; a[6] = w; a[7] = p;
+    } else if (AVG == 1) {   // q, u, v, w, p
+        a[0] = q[0]; a[1] = q[1]; a[2] = q[2]; a[3] = q[3]; a[4] = q[4];
+        a[5] = u; a[6] = v; a[7] = w; a[8] = p;
+    } else {                 // rho, u, v, w, p, e (KG) or h (Pirozzoli)
+        a[0] = q[0]; a[1] = u; a[2] = v; a[3] = w; a[4] = p;
+        a[5] = AVG == 2 ? q[4] * ir : (q[4] + p) * ir;
+    }
+}
+
+// Kennedy-Gruber and Pirozzoli are symmetric in (a, b) in IEEE arithmetic; Standard and Ducros
+// are not, and the reference evaluates every pair with the lower node index as a.
+constexpr bool two_point_symmetric(int avg) { return avg >= 2; }
+
+template <int AVG>
+HX_HD void two_point(const double *a, const double *ja, const double *b, const double *jb, double fs[NV])
+{
     double f[NV], g[NV], h[NV], J[3];
-    switch (AVG) {
-    case 1:
+    if (AVG == 0) {
+        const double uL = a[4], vL = a[5], wL = a[6], pL = a[7];
+        const double uR = b[4], vR = b[5], wR = b[6], pR = b[7];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[d] = ja[d] + jb[d];
+        f[0] = (a[0] + b[0]);
+        f[1] = (a[0] * uL + b[0] * uR + pL + pR);
+        f[2] = (a[0] * vL + b[0] * vR);
+        f[3] = (a[0] * wL + b[0] * wR);
+        f[4] = (uL * (a[3] + pL) + uR * (b[3] + pR));
+        g[0] = (a[1] + b[1]);
+        g[1] = (a[1] * uL + b[1] * uR);
+        g[2] = (a[1] * vL + b[1] * vR + pL + pR);
+        g[3] = (a[1] * wL + b[1] * wR);
+        g[4] = (vL * (a[3] + pL) + vR * (b[3] + pR));
+        h[0] = (a[2] + b[2]);
+        h[1] = (a[2] * uL + b[2] * uR);
+        h[2] = (a[2] * vL + b[2] * vR);
+        h[3] = (a[2] * wL + b[2] * wR + pL + pR);
+        h[4] = (wL * (a[3] + pL) + wR * (b[3] + pR));
+#pragma unroll
+        for (int q = 0; q < NV; ++q) fs[q] = 0.25 * (f[q] * J[0] + g[q] * J[1] + h[q] * J[2]);
+    } else if (AVG == 1) {
+        const double uL = a[5], vL = a[6], wL = a[7], pL = a[8];
+        const double uR = b[5], vR = b[6], wR = b[7], pR = b[8];
 #pragma unroll
         for (int d = 0; d < 3; ++d) J[d] = 0.5 * (ja[d] + jb[d]);
-        f[0] = 0.25 * (qa[0] + qb[0]) * (uL + uR);
-        f[1] = 0.25 * (qa[1] + qb[1]) * (uL + uR) + 0.5 * (pL + pR);
-        f[2] = 0.25 * (qa[2] + qb[2]) * (uL + uR);
-        f[3] = 0.25 * (qa[3] + qb[3]) * (uL + uR);
-        f[4] = 0.25 * (qa[4] + pL + qb[4] + pR) * (uL + uR);
-        g[0] = 0.25 * (qa[0] + qb[0]) * (vL + vR);
-        g[1] = 0.25 * (qa[1] + qb[1]) * (vL + vR);
-        g[2] = 0.25 * (qa[2] + qb[2]) * (vL + vR) + 0.5 * (pL + pR);
-        g[3] = 0.25 * (qa[3] + qb[3]) * (vL + vR);
-        g[4] = 0.25 * (qa[4] + pL + qb[4] + pR) * (vL + vR);
-        h[0] = 0.25 * (qa[0] + qb[0]) * (wL + wR);
-        h[1] = 0.25 * (qa[1] + qb[1]) * (wL + wR);
-        h[2] = 0.25 * (qa[2] + qb[2]) * (wL + wR);
-        h[3] = 0.25 * (qa[3] + qb[3]) * (wL + wR) + 0.5 * (pL + pR);
-        h[4] = 0.25 * (qa[4] + pL + qb[4] + pR) * (wL + wR);
+        f[0] = 0.25 * (a[0] + b[0]) * (uL + uR);
+        f[1] = 0.25 * (a[1] + b[1]) * (uL + uR) + 0.5 * (pL + pR);
+        f[2] = 0.25 * (a[2] + b[2]) * (uL + uR);
+        f[3] = 0.25 * (a[3] + b[3]) * (uL + uR);
+        f[4] = 0.25 * (a[4] + pL + b[4] + pR) * (uL + uR);
+        g[0] = 0.25 * (a[0] + b[0]) * (vL + vR);
+        g[1] = 0.25 * (a[1] + b[1]) * (vL + vR);
+        g[2] = 0.25 * (a[2] + b[2]) * (vL + vR) + 0.5 * (pL + pR);
+        g[3] = 0.25 * (a[3] + b[3]) * (vL + vR);
+        g[4] = 0.25 * (a[4] + pL + b[4] + pR) * (vL + vR);
+        h[0] = 0.25 * (a[0] + b[0]) * (wL + wR);
+        h[1] = 0.25 * (a[1] + b[1]) * (wL + wR);
+        h[2] = 0.25 * (a[2] + b[2]) * (wL + wR);
+        h[3] = 0.25 * (a[3] + b[3]) * (wL + wR) + 0.5 * (pL + pR);
+        h[4] = 0.25 * (a[4] + pL + b[4] + pR) * (wL + wR);
 #pragma unroll
         for (int v = 0; v < NV; ++v) fs[v] = f[v] * J[0] + g[v] * J[1] + h[v] * J[2];
-        return;
-    case 2:
-    case 3: {
-        const double rho = 0.5 * (qa[0] + qb[0]);
-        const double u = 0.5 * (uL + uR), v = 0.5 * (vL + vR), w = 0.5 * (wL + wR);
-        const double p = 0.5 * (pL + pR);
-        double e;
-        if (AVG == 2) e = 0.5 * (qa[4] * pa[4] + qb[4] * pb[4]);
-        else e = 0.5 * ((qa[4] + pL) * pa[4] + (qb[4] + pR) * pb[4]);
+    } else {
+        const double rho = 0.5 * (a[0] + b[0]);
+        const double u = 0.5 * (a[1] + b[1]), v = 0.5 * (a[2] + b[2]), w = 0.5 * (a[3] + b[3]);
+        const double p = 0.5 * (a[4] + b[4]);
+        const double e = 0.5 * (a[5] + b[5]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) J[d] = 0.5 * (ja[d] + jb[d]);
         f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * v; f[3] = rho * u * w;
@@ -346,29 +389,21 @@ HX_HD void two_point_flux_t(const double *qa, const double *pa, const double *ja
         }
 #pragma unroll
         for (int q = 0; q < NV; ++q) fs[q] = f[q] * J[0] + g[q] * J[1] + h[q] * J[2];
-        return; }
-    default:
-#pragma unroll
-        for (int d = 0; d < 3; ++d) J[d] = ja[d] + jb[d];
-        f[0] = (qa[1] + qb[1]);
-        f[1] = (qa[1] * uL + qb[1] * uR + pL + pR);
-        f[2] = (qa[1] * vL + qb[1] * vR);
-        f[3] = (qa[1] * wL + qb[1] * wR);
-        f[4] = (uL * (qa[4] + pL) + uR * (qb[4] + pR));
-        g[0] = (qa[2] + qb[2]);
-        g[1] = (qa[2] * uL + qb[2] * uR);
-        g[2] = (qa[2] * vL + qb[2] * vR + pL + pR);
-        g[3] = (qa[2] * wL + qb[2] * wR);
-        g[4] = (vL * (qa[4] + pL) + vR * (qb[4] + pR));
-        h[0] = (qa[3] + qb[3]);
-        h[1] = (qa[3] * uL + qb[3] * uR);
-        h[2] = (qa[3] * vL + qb[3] * vR);
-        h[3] = (qa[3] * wL + qb[3] * wR + pL + pR);
-        h[4] = (wL * (qa[4] + pL) + wR * (qb[4] + pR));
-#pragma unroll
-        for (int q = 0; q < NV; ++q) fs[q] = 0.25 * (f[q] * J[0] + g[q] * J[1] + h[q] * J[2]);
-        return;
     }
+}
+
+// One direction of euler_contravariant (the same operations for row d).
+HX_HD void euler_contravariant_row(const Phys &ph, const double q[NV], const double jd[3], double out[NV])
+{
+    const double u = q[1] / q[0], v = q[2] / q[0], w = q[3] / q[0];
+    const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
+    const double F[NV][3] = {{q[1], q[2], q[3]},
+                             {q[1] * u + p, q[1] * v, q[1] * w},
+                             {q[1] * v, q[2] * v + p, q[2] * w},
+                             {q[1] * w, q[2] * w, q[3] * w + p},
+                             {(q[4] + p) * u, (q[4] + p) * v, (q[4] + p) * w}};
+#pragma unroll
+    for (int c = 0; c < NV; ++c) out[c] = F[c][0] * jd[0] + F[c][1] * jd[1] + F[c][2] * jd[2];
 }
 
 }  // namespace hx
