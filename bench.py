@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--amplitude", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 5 times (the "
                          "known-size reference for calibrating FETCH_SIZE / WRITE_SIZE counters)")
@@ -160,8 +160,8 @@ def main():
         k["GBs"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
     dom = max(kern, key=lambda k: kern[k]["ms"])
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_config2.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(REPO, "profiles", "traffic_tgv16_n4.json")
+    if os.path.exists(tpath) and (args.n, args.order) == (16, 4):
         tj = json.load(open(tpath))
         traffic = tj.get("k_volume_bytes_per_launch") if dom == "volume" else tj.get("k_%s_bytes_per_launch" % dom)
     roof = dict(bound="hbm", kernel="k_" + dom, achieved=round(kern[dom]["GBs"], 1), peak=HBM_PEAK_GBS,

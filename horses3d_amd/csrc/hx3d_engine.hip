@@ -15,8 +15,14 @@
 //             integral of the face fluxes, 1/J (SD.f90:483-494), and the Williamson RK3
 //             register update (TakeRK3Step, ExplicitMethods.f90:748-775), fused.
 // Device layout: structure of arrays, one contiguous array per variable:
-//   Q[v*S + g], G[v*S + g], grad[(5d + v)*S + g], Ja[(3d + c)*S + g], invJ[g]
-// with g = e*(N+1)^3 + i + (N+1)(j + (N+1)k) and S the padded node count.
+//   per node g = e*(N+1)^3 + i + (N+1)(j + (N+1)k), S the padded node count:
+//     Q[v*S + g], G[v*S + g], vgrad[c*S + g] (velocity gradients, grad T), Ja[(3d + c)*S + g],
+//     J[g], invJ[g]
+//   per element face node x = (6e + slot)(N+1)^2 + a + (N+1)b, ES the padded count:
+//     fe[v*ES + x] (face flux times 1/w_0, negated on the right side), egeo[c*ES + x]
+//     (normal, |Ja|), nbr[x] (2 * neighbour node + side)
+//   per face node q = f(N+1)^2 + i + (N+1)j (left element's face coordinates), FS the count:
+//     fgeo[c*FS + q] (n, t1, t2, |Ja|), fnode[q] (left node, right node, left x, right x)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -7,9 +7,10 @@
  *       Solver/src/NavierStokesSolver/SpatialDiscretization.f90:227-320, interface
  *       ComputeTimeDerivative_f in Solver/src/libs/discretization/DGSEMClass.f90:77-97
  *       -> hx3d_compute_qdot
- *   - TakeRK3Step(mesh, particles, t, deltaT, ComputeTimeDerivative)
- *       Solver/src/libs/timeintegrator/ExplicitMethods.f90:667-788 (RKStepFcn,
- *       TimeIntegrator.f90:77) -> hx3d_rk3_step
+ *   - TakeRK3Step(mesh, particles, t, deltaT, ComputeTimeDerivative, ...)
+ *       Solver/src/libs/timeintegrator/ExplicitMethods.f90:667-788, interface TimeStep_FCN
+ *       (TimeIntegratorDefinitions.f90:11-34) of the RKStep pointer (TimeIntegrator.f90:63)
+ *       -> hx3d_rk3_step
  * together with the step-loop helpers the time integrator calls around them:
  *   - MaxTimeStep (DGSEMClass.f90:870-1035)               -> hx3d_max_dt
  *   - ComputeMaxResiduals (DGSEMClass.f90:770-860)        -> hx3d_residuals
