@@ -36,6 +36,7 @@
 #include "../../include/hx3d.h"
 #include "physics.hpp"
 #include "stage_kernels.hpp"
+#include "mixed_kernels.hpp"
 
 using hx::NV;
 using hx::Phys;
@@ -357,6 +358,15 @@ struct hx3d_engine {
     unsigned long long *dBits = nullptr;
     int *dNbr = nullptr, *dFlag = nullptr;
     int4 *dFnode = nullptr;
+    // mixed orders (p-anisotropic elements, mortar faces)
+    bool mixed = false;
+    long FSm = 0;
+    int *dN3 = nullptr, *dEfoff = nullptr, *dElist = nullptr, *dEofn = nullptr;
+    long *dOff = nullptr;
+    hxm::FaceRec *dFaces = nullptr;
+    hxm::OpsAll *dOpsAll = nullptr;
+    double *dU = nullptr, *dLift = nullptr;
+    int bucket_first[9] = {0}, bucket_count[9] = {0};   // elements needing w waves: w = 1..8
     bool qdot_valid = false, timing = false;
     double ktime[4] = {0, 0, 0, 0};
     std::vector<hipEvent_t> evpool;   // timing: 4 events per stage of the last hx3d_advance
@@ -446,9 +456,13 @@ void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs 
 }
 
 // ev: NULL, or 4 events recorded around the three launches of the stage
+void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
+                 hipEvent_t *ev);
+
 void stage(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
            hipEvent_t *ev = nullptr)
 {
+    if (eng->mixed) return stage_mixed(eng, a_k, c_k, update, dta, qdot, ev);
     switch (eng->N) {
     case 1: stage_n<1>(eng, a_k, c_k, update, dta, qdot, ev); break;
     case 2: stage_n<2>(eng, a_k, c_k, update, dta, qdot, ev); break;
@@ -475,14 +489,320 @@ DtArgs host_dt(hx3d_engine *eng, double dt)
 
 // Standalone MaxTimeStep of the resident state into dt slot `slot` (bits reset first);
 // k_dt_finish then writes dt to dScal[0].
+void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long long *bits);
+
 void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 {
     unsigned long long *b = eng->dBits + 2 * slot;
     hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, b, 2);
-    const long nb = (eng->nodes + 255) / 256;
-    hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
-                       eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, b);
+    launch_maxdt_bits(eng, cfl, dcfl, b);
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, b, eng->dScal, 0);
+}
+
+// ------------------------------------------------------------------------------------
+// mixed orders
+// ------------------------------------------------------------------------------------
+bool almost_equal(double a, double b)
+{
+    // AlmostEqual (Utilities.f90:110-139), tolerance 2 epsilon
+    const double tol = 2.0 * 2.220446049250313e-16;
+    if (a == 0.0 || b == 0.0) return std::fabs(a - b) <= tol;
+    return std::fabs(b - a) <= tol * std::max(std::fabs(a), std::fabs(b));
+}
+
+void bary(int N, const double *x, double *wb)
+{
+    // BarycentricWeights (InterpolationAndDerivatives.f90:230-262)
+    for (int j = 0; j <= N; ++j) wb[j] = 1.0;
+    for (int j = 1; j <= N; ++j)
+        for (int k = 0; k < j; ++k) { wb[k] *= (x[k] - x[j]); wb[j] *= (x[j] - x[k]); }
+    for (int j = 0; j <= N; ++j) wb[j] = 1.0 / wb[j];
+}
+
+// PolynomialInterpolationMatrix (InterpolationAndDerivatives.f90:413-461): T (M+1)x(N+1)
+void interp_matrix(int N, int M, const double *xo, const double *wb, const double *xn, double *T)
+{
+    for (int k = 0; k <= M; ++k) {
+        bool match = false;
+        for (int j = 0; j <= N; ++j) {
+            T[k * (N + 1) + j] = 0.0;
+            if (almost_equal(xn[k], xo[j])) { match = true; T[k * (N + 1) + j] = 1.0; }
+        }
+        if (!match) {
+            double s = 0.0;
+            for (int j = 0; j <= N; ++j) {
+                const double t = wb[j] / (xn[k] - xo[j]);
+                T[k * (N + 1) + j] = t;
+                s = s + t;
+            }
+            for (int j = 0; j <= N; ++j) T[k * (N + 1) + j] = T[k * (N + 1) + j] / s;
+        }
+    }
+}
+
+// Tset(N -> M) (InterpolationMatrix_Construct, InterpolationMatrices.f90:42-108): Lagrange
+// interpolation when N < M, otherwise the transpose of the M -> N interpolation weighted by
+// w_N(j) / w_M(i) (the L2 projection)
+void tset_host(int N, int M, double *T)
+{
+    double xo[8], wo[8], xd[8], wd[8], wb[8];
+    gl_nodes(N, xo, wo);
+    gl_nodes(M, xd, wd);
+    if (N < M) {
+        bary(N, xo, wb);
+        interp_matrix(N, M, xo, wb, xd, T);
+    } else {
+        double Tin[64];
+        bary(M, xd, wb);
+        interp_matrix(M, N, xd, wb, xo, Tin);
+        for (int j = 0; j <= N; ++j)
+            for (int i = 0; i <= M; ++i) T[i * (N + 1) + j] = Tin[j * (M + 1) + i] * wo[j] / wd[i];
+    }
+}
+
+template <int FORM>
+void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
+{
+    constexpr bool SPLIT = FORM >= 0;
+    constexpr int ND = hx::node_data_size(SPLIT ? FORM : 0);
+    const int per_node = 3 * (SPLIT ? 4 : 5) + (SPLIT ? ND + 3 : 0);
+    for (int w = 1; w <= 8; ++w) {
+        if (!eng->bucket_count[w]) continue;
+        const size_t lds = sizeof(double) * per_node * 64 * w;
+        hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->bucket_count[w]), dim3(64 * w), lds, eng->stream,
+                           eng->dElist + eng->bucket_first[w], eng->dQ, eng->dU, eng->dN3, eng->dOff,
+                           eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG,
+                           qdot, a_k, c_k, update, dta, eng->S, eng->ES, eng->ph);
+    }
+}
+
+void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
+                 hipEvent_t *ev)
+{
+    if (ev) hipEventRecord(ev[0], eng->stream);
+    if (!eng->ph.euler) {
+        hipLaunchKernelGGL(hxm::km_face_grad, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
+                           eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
+                           eng->S, eng->ES, eng->FSm);
+        for (int w = 1; w <= 8; ++w) {
+            if (!eng->bucket_count[w]) continue;
+            hipLaunchKernelGGL(hxm::km_grad, dim3(eng->bucket_count[w]), dim3(64 * w), sizeof(double) * NV * 64 * w,
+                               eng->stream, eng->dElist + eng->bucket_first[w], eng->dQ, eng->dN3, eng->dOff,
+                               eng->dEfoff, eng->dJa, eng->dInvJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES);
+        }
+    }
+    if (ev) hipEventRecord(ev[1], eng->stream);
+    hipLaunchKernelGGL(hxm::km_face, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
+                       eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll, eng->dFe, eng->F, eng->S,
+                       eng->ES, eng->FSm, eng->ph);
+    if (ev) hipEventRecord(ev[2], eng->stream);
+    if (eng->inviscid != HX3D_INVISCID_SPLIT) launch_mvolume<hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
+    else switch (eng->ph.averaging) {
+        case HX3D_AVG_DUCROS: launch_mvolume<hxk::AVG_DUCROS>(eng, a_k, c_k, update, dta, qdot); break;
+        case HX3D_AVG_KENNEDYGRUBER: launch_mvolume<hxk::AVG_KG>(eng, a_k, c_k, update, dta, qdot); break;
+        case HX3D_AVG_PIROZZOLI: launch_mvolume<hxk::AVG_PIROZZOLI>(eng, a_k, c_k, update, dta, qdot); break;
+        default: launch_mvolume<hxk::AVG_STANDARD>(eng, a_k, c_k, update, dta, qdot); break;
+        }
+    if (ev) hipEventRecord(ev[3], eng->stream);
+}
+
+// MaxTimeStep limits of the resident state accumulated into bits[0..1] (reset beforehand)
+void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long long *bits)
+{
+    const long nb = (eng->nodes + 255) / 256;
+    if (eng->mixed)
+        hipLaunchKernelGGL(hxm::km_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ, eng->dEofn,
+                           eng->dN3, eng->dOpsAll, eng->nodes, eng->S, cfl, dcfl, eng->ph, bits);
+    else
+        hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ, eng->nodes,
+                           eng->S, eng->dcsi, cfl, dcfl, eng->ph, bits);
+}
+
+// physics constants (PhysicsStorage_NS.f90:145-439, ThermodynamicsAir :115-132)
+void set_physics(hx3d_engine *eng, const hx3d_case_desc *d)
+{
+    Phys &ph = eng->ph;
+    ph.gamma = 1.4; ph.gm1 = 1.4 - 1.0;
+    ph.gammaM2 = ph.gamma * d->mach * d->mach;
+    ph.euler = d->euler ? 1 : 0;
+    ph.mu = d->euler ? 0.0 : 1.0 / d->reynolds;
+    ph.mu_to_kappa = d->euler ? 0.0 : 1.0 / (ph.gm1 * d->mach * d->mach * d->prandtl);
+    const double Tref = 520.0 * 5.0 / 9.0, Ssu = 198.6 * 5.0 / 9.0;
+    ph.S_div_T = Ssu / Tref;
+    ph.Trenorm = Tref / Tref;
+    ph.lambda_stab = d->lambda_stab;
+    ph.les_c = d->les_c;
+    ph.riemann = d->riemann;
+    ph.cp = ph.gamma * (1.0 / (1.4 - 1.0));
+    ph.averaging = d->averaging;
+    ph.les = d->euler ? 0 : d->les;
+    eng->inviscid = d->inviscid;
+}
+
+// Mixed orders: element and face tables (Face_LinkWithElements, FaceClass.f90:187-278), the
+// operators of every order, element buckets, device arrays.  The face frames at the face
+// orders must come with the descriptor (MappedGeometryFace of the mortar faces).
+int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
+{
+    eng->mixed = true;
+    const int K = d->K, F = d->F;
+    eng->K = K; eng->F = F;
+    if (!(d->fNormal && d->fT1 && d->fT2 && d->fJac))
+        return eng->fail(HX3D_EINVAL, "mixed orders: the descriptor must give the face geometry (fNormal, fT1, fT2, fJac) at the face orders");
+    std::vector<int> N3(d->N, d->N + 3 * K);
+    std::vector<long> off(K + 1, 0);
+    for (int e = 0; e < K; ++e) off[e + 1] = off[e] + (long)(N3[3 * e] + 1) * (N3[3 * e + 1] + 1) * (N3[3 * e + 2] + 1);
+    const long nodes = off[K];
+    eng->nodes = nodes;
+    eng->S = ((nodes + 63) / 64) * 64;
+    const long S = eng->S;
+    std::vector<int> efoff(6 * K + 1, 0);
+    for (int e = 0; e < K; ++e)
+        for (int s = 0; s < 6; ++s)
+            efoff[6 * e + s + 1] = efoff[6 * e + s] + (N3[3 * e + SLOT_U[s]] + 1) * (N3[3 * e + SLOT_V[s]] + 1);
+    eng->ES = (((long)efoff[6 * K] + 63) / 64) * 64;
+    const long ES = eng->ES;
+    if (2 * nodes > 0x7fffffffL || ES > 0x7fffffffL) return eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit indices");
+    // faces
+    std::vector<hxm::FaceRec> faces(F);
+    long fsm = 0;
+    for (int f = 0; f < F; ++f) {
+        hxm::FaceRec &r = faces[f];
+        const int rot = d->fRot[f];
+        for (int sd = 0; sd < 2; ++sd) {
+            const int e = d->fElem[2 * f + sd], s = d->fESide[2 * f + sd] - 1;
+            if (e < 0 || e >= K || s < 0 || s > 5 || rot < 0 || rot > 7)
+                return eng->fail(HX3D_EINVAL, "face " + std::to_string(f) + ": bad element/side/rotation");
+            if (d->faceID[6 * e + s] != f || d->faceSide[6 * e + s] != sd + 1)
+                return eng->fail(HX3D_EINVAL, "face " + std::to_string(f) + ": inconsistent with element faceID/faceSide");
+            r.e[sd] = e; r.slot[sd] = s;
+            r.Nel[sd][0] = N3[3 * e + SLOT_U[s]];
+            r.Nel[sd][1] = N3[3 * e + SLOT_V[s]];
+        }
+        r.rot = rot;
+        r.Nfs[0][0] = r.Nel[0][0]; r.Nfs[0][1] = r.Nel[0][1];
+        const bool keep = rot == 0 || rot == 2 || rot == 5 || rot == 7;
+        r.Nfs[1][0] = keep ? r.Nel[1][0] : r.Nel[1][1];
+        r.Nfs[1][1] = keep ? r.Nel[1][1] : r.Nel[1][0];
+        r.Nf[0] = std::max(r.Nfs[0][0], r.Nfs[1][0]);
+        r.Nf[1] = std::max(r.Nfs[0][1], r.Nfs[1][1]);
+        for (int sd = 0; sd < 2; ++sd) {
+            const bool a = r.Nfs[sd][0] != r.Nf[0], b = r.Nfs[sd][1] != r.Nf[1];
+            r.proj[sd] = (!a && !b) ? 0 : (a && !b) ? 1 : (!a && b) ? 2 : 3;
+        }
+        r.foff = (int)fsm;
+        fsm += (long)(r.Nf[0] + 1) * (r.Nf[1] + 1);
+    }
+    eng->FSm = fsm;
+    // operators of every order
+    hxm::OpsAll oa;
+    std::memset(&oa, 0, sizeof(oa));
+    std::vector<double> wq((size_t)S, 0.0);
+    double xn[8][8], wn[8][8];
+    for (int n = 1; n <= MAXN; ++n) {
+        const Ops o = make_ops(n);
+        std::memcpy(oa.D[n], o.D, sizeof(o.D));
+        std::memcpy(oa.hatD[n], o.hatD, sizeof(o.hatD));
+        std::memcpy(oa.sharpD[n], o.sharpD, sizeof(o.sharpD));
+        gl_nodes(n, xn[n], wn[n]);
+        oa.b0[n] = 1.0 / wn[n][0];
+        oa.dcsi[n] = 1.0 / std::fabs(xn[n][1] - xn[n][0]);
+        for (int m = 1; m <= MAXN; ++m)
+            if (m != n) tset_host(n, m, oa.T[n][m]);
+    }
+    // geometry, weights, LES filter widths (SD.f90:420, :1378)
+    std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), jj((size_t)S, 1.0), edelta(K), fdelta(F),
+        fgeo((size_t)10 * fsm, 0.0);
+    std::vector<int> eofn((size_t)nodes);
+    for (int e = 0; e < K; ++e) {
+        const int nx = N3[3 * e] + 1, ny = N3[3 * e + 1] + 1, nz = N3[3 * e + 2] + 1;
+        double vol = 0.0;
+        for (int kk = 0; kk < nz; ++kk)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const long g = off[e] + i + nx * (j + ny * kk);
+                    for (int c = 0; c < 9; ++c) ja[(size_t)c * S + g] = d->Ja[9 * g + c];
+                    ij[g] = 1.0 / d->J[g];
+                    jj[g] = d->J[g];
+                    wq[g] = wn[nx - 1][i] * wn[ny - 1][j] * wn[nz - 1][kk];
+                    vol += wq[g] * d->J[g];
+                    eofn[g] = e;
+                }
+        edelta[e] = std::pow(vol / (nx * ny * nz), 1.0 / 3.0);
+    }
+    for (int f = 0; f < F; ++f) {
+        const hxm::FaceRec &r = faces[f];
+        double surf = 0.0;
+        for (int j = 0; j <= r.Nf[1]; ++j)
+            for (int i = 0; i <= r.Nf[0]; ++i) {
+                const long q = r.foff + i + (r.Nf[0] + 1) * j;
+                for (int c = 0; c < 3; ++c) {
+                    fgeo[(size_t)c * fsm + q] = d->fNormal[3 * q + c];
+                    fgeo[(size_t)(3 + c) * fsm + q] = d->fT1[3 * q + c];
+                    fgeo[(size_t)(6 + c) * fsm + q] = d->fT2[3 * q + c];
+                }
+                fgeo[(size_t)9 * fsm + q] = d->fJac[q];
+                surf += wn[r.Nf[0]][i] * wn[r.Nf[1]][j] * d->fJac[q];
+            }
+        fdelta[f] = std::sqrt(surf / ((r.Nf[0] + 1) * (r.Nf[1] + 1)));
+    }
+    // element buckets by the waves one element needs
+    std::vector<int> elist;
+    for (int w = 1; w <= 8; ++w) {
+        eng->bucket_first[w] = (int)elist.size();
+        for (int e = 0; e < K; ++e)
+            if ((int)((off[e + 1] - off[e] + 63) / 64) == w) elist.push_back(e);
+        eng->bucket_count[w] = (int)elist.size() - eng->bucket_first[w];
+    }
+    {
+        // the volume kernel keeps three directions of fluxes (+ two-point node data and a metric
+        // row for the split form) of one element in LDS
+        const bool split = d->inviscid == HX3D_INVISCID_SPLIT;
+        const int per_node = 3 * (split ? 4 : 5) + (split ? hx::node_data_size(d->averaging) + 3 : 0);
+        int dev = 0, maxlds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return eng->fail(HX3D_EDEVICE, "no HIP device");
+        for (int w = 1; w <= 8; ++w)
+            if (eng->bucket_count[w] && (long)sizeof(double) * per_node * 64 * w > maxlds)
+                return eng->fail(HX3D_EUNSUPPORTED, "an element of this size needs more LDS than a block may use");
+    }
+    int rc;
+    if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
+        (rc = dalloc(eng, &eng->dU, (size_t)15 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
+        (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
+        (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * fsm)) ||
+        (rc = dalloc(eng, &eng->dFe, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dLift, (size_t)15 * ES)) ||
+        (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) ||
+        (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) || (rc = dalloc(eng, &eng->dScal, 4)) ||
+        (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
+        (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
+        (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dN3, N3.size())) ||
+        (rc = dalloc(eng, &eng->dOff, off.size())) || (rc = dalloc(eng, &eng->dEfoff, efoff.size())) ||
+        (rc = dalloc(eng, &eng->dElist, elist.size())) || (rc = dalloc(eng, &eng->dEofn, eofn.size())) ||
+        (rc = dalloc(eng, &eng->dFaces, faces.size())) || (rc = dalloc(eng, &eng->dOpsAll, 1)))
+        return rc;
+    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
+        return eng->fail(HX3D_EDEVICE, "hipStreamCreate failed");
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(eng->dJa, ja.data(), ja.size() * 8) || !up(eng->dInvJ, ij.data(), ij.size() * 8) ||
+        !up(eng->dJ, jj.data(), jj.size() * 8) || !up(eng->dFgeo, fgeo.data(), fgeo.size() * 8) ||
+        !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
+        !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dN3, N3.data(), N3.size() * 4) ||
+        !up(eng->dOff, off.data(), off.size() * 8) || !up(eng->dEfoff, efoff.data(), efoff.size() * 4) ||
+        !up(eng->dElist, elist.data(), elist.size() * 4) || !up(eng->dEofn, eofn.data(), eofn.size() * 4) ||
+        !up(eng->dFaces, faces.data(), faces.size() * sizeof(hxm::FaceRec)) ||
+        !up(eng->dOpsAll, &oa, sizeof(oa)))
+        return eng->fail(HX3D_EDEVICE, "upload failed");
+    const long nb = (NV * S + 255) / 256;
+    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, (long)NV * S, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dG, (long)NV * S, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3((15 * S + 255) / 256), dim3(256), 0, eng->stream, eng->dU, 15L * S, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, eng->stream, eng->dScal, 4L, 0.0);
+    if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "initialisation failed");
+    return HX3D_OK;
 }
 
 // Williamson RK3 coefficients (ExplicitMethods.f90:697-699)
@@ -509,11 +829,13 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         !d->Ja || !d->J)
         return bail(eng->fail(HX3D_EINVAL, "descriptor: missing arrays or K <= 0"));
     const int N = d->N[0];
+    bool uniform = true;
     for (int e = 0; e < d->K; ++e)
-        for (int c = 0; c < 3; ++c)
-            if (d->N[3 * e + c] != N)
-                return bail(eng->fail(HX3D_EUNSUPPORTED, "plain engine needs one uniform order; mixed orders go to the batched engine"));
-    if (N < 1 || N > MAXN) return bail(eng->fail(HX3D_EUNSUPPORTED, "order must be 1..7"));
+        for (int c = 0; c < 3; ++c) {
+            if (d->N[3 * e + c] < 1 || d->N[3 * e + c] > MAXN)
+                return bail(eng->fail(HX3D_EUNSUPPORTED, "orders must be 1..7"));
+            if (d->N[3 * e + c] != N) uniform = false;
+        }
     if (2L * d->F != 6L * d->K) return bail(eng->fail(HX3D_EUNSUPPORTED, "every element face must be an interior (periodic) face: need F = 3K"));
     if (d->riemann < 0 || d->riemann > 1 || d->inviscid < 0 || d->inviscid > 1 || d->averaging < 0 ||
         d->averaging > 4 || d->les < 0 || d->les > 2)
@@ -522,6 +844,8 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         return bail(eng->fail(HX3D_EUNSUPPORTED, "split form with Morinishi averaging is not implemented"));
     if (d->mach <= 0 || (!d->euler && d->reynolds <= 0))
         return bail(eng->fail(HX3D_EINVAL, "descriptor: mach and reynolds must be positive"));
+    set_physics(eng, d);
+    if (!uniform) return bail(create_mixed(eng, d));
     eng->K = d->K; eng->N = N; eng->n1 = N + 1; eng->nf = eng->n1 * eng->n1; eng->ne = eng->nf * eng->n1;
     eng->F = d->F;
     eng->nodes = (long)eng->K * eng->ne;
@@ -536,22 +860,6 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         gl_nodes(N, x, w);
         eng->dcsi = 1.0 / std::fabs(x[1] - x[0]);
     }
-    // physics constants (PhysicsStorage_NS.f90:145-439, ThermodynamicsAir :115-132)
-    Phys &ph = eng->ph;
-    ph.gamma = 1.4; ph.gm1 = 1.4 - 1.0;
-    ph.gammaM2 = ph.gamma * d->mach * d->mach;
-    ph.euler = d->euler ? 1 : 0;
-    ph.mu = d->euler ? 0.0 : 1.0 / d->reynolds;
-    ph.mu_to_kappa = d->euler ? 0.0 : 1.0 / (ph.gm1 * d->mach * d->mach * d->prandtl);
-    const double Tref = 520.0 * 5.0 / 9.0, Ssu = 198.6 * 5.0 / 9.0;
-    ph.S_div_T = Ssu / Tref;
-    ph.Trenorm = Tref / Tref;
-    ph.lambda_stab = d->lambda_stab;
-    ph.les_c = d->les_c;
-    ph.riemann = d->riemann;
-    ph.cp = ph.gamma * (1.0 / (1.4 - 1.0));
-    ph.averaging = d->averaging;
-    ph.les = d->euler ? 0 : d->les;
 
     const int K = eng->K, n1 = eng->n1, nf = eng->nf, ne = eng->ne, F = eng->F;
     const long S = eng->S, nodes = eng->nodes, FS = eng->FS, ES = eng->ES;
@@ -702,7 +1010,8 @@ int hx3d_engine_destroy(hx3d_engine *eng)
     if (eng->stream) hipStreamSynchronize(eng->stream);
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
-                    eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode};
+                    eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff, eng->dElist,
+                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &e : eng->evpool) hipEventDestroy(e);
@@ -804,9 +1113,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
     // from the state it produces).  Stage 1 of step s resets slot (s+2)%3 for step s+1.
     if (nsteps > 0) {
         hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, eng->dBits, 6);
-        const long nbn = (eng->nodes + 255) / 256;
-        hipLaunchKernelGGL(k_maxdt, dim3(nbn), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
-                           eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dBits);
+        launch_maxdt_bits(eng, cfl, dcfl, eng->dBits);
     }
     for (int s = 0; s < nsteps; ++s) {
         for (int k = 0; k < 3; ++k) {

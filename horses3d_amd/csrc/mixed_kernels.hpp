@@ -1,0 +1,644 @@
+// Mixed-order stage kernels of the hx3d engine: elements with their own orders (Nx, Ny, Nz)
+// and conforming faces between mismatched orders through the reference's mortar operators
+// (Face_AdaptSolutionToFace / Face_AdaptGradientsToFace, FaceClass.f90:281-512, and
+// Face_ProjectFluxToElements / Face_ProjectGradientFluxToElements, FaceClass.f90:597-700,
+// 865-964).  Included by hx3d_engine.hip after stage_kernels.hpp.
+//
+// One stage = km_face_grad -> km_grad -> km_face -> km_volume:
+//   km_face_grad  one wave per face: both Q traces to the face order, BR1 unStar there,
+//                 projected back onto each side's element face (lift[15][ES]);
+//   km_grad       one block per element: local gradient + lifting -> grad Q (15 values);
+//   km_face       one wave per face: Q and grad Q traces to the face order, Riemann + BR1
+//                 fluxes there, projected back (right side negated) and scaled by the
+//                 receiving element's surface weight -> fe[5][ES];
+//   km_volume     one block per element: volume integral, surface integral, 1/J, RK update and
+//                 the next step's dt limits.
+// Element kernels run once per bucket of elements needing the same number of waves.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "physics.hpp"
+#include "stage_kernels.hpp"
+
+namespace hxm {
+
+using hx::NV;
+using hx::Phys;
+using hxk::DtArgs;
+
+constexpr int MAXN1 = 8;   // orders 1..7
+
+// Operators of every order N (row i, column j at [N][i*8 + j]) and the mortar matrices
+// Tset(N -> M) ((M+1) x (N+1), row-major with row length N+1), InterpolationMatrices.f90:42-108.
+struct OpsAll {
+    double D[MAXN1][64], hatD[MAXN1][64], sharpD[MAXN1][64];
+    double b0[MAXN1];     // 1/w_0 = 1/w_N, the GL surface weight
+    double dcsi[MAXN1];   // 1/|x_1 - x_0|, MaxTimeStep (DGSEMClass.f90:948-959)
+    double T[MAXN1][MAXN1][64];
+};
+
+// Face record: elements, slots, orders, rotation and projection types (Face_LinkWithElements,
+// FaceClass.f90:187-278).  NfL/NfR are each side's orders in the face orientation, Nf their max.
+struct FaceRec {
+    int e[2], slot[2];          // left, right element and element face slot (0..5)
+    int Nel[2][2];              // element face orders (element face orientation)
+    int Nfs[2][2];              // the same in the face orientation
+    int Nf[2], rot, proj[2];    // proj: 0 none, 1 first index, 2 second, 3 both
+    int foff;                   // first face node (face order) of this face
+};
+
+// element face slot 0..5 = EFRONT, EBACK, EBOTTOM, ERIGHT, ETOP, ELEFT: the fixed axis, its end,
+// and the axes of the slot's face coordinates (a, b)
+__device__ __constant__ int M_SLOT_AXIS[6] = {1, 1, 2, 0, 2, 0};
+__device__ __constant__ int M_SLOT_PLUS[6] = {0, 1, 0, 1, 1, 0};
+__device__ __constant__ int M_SLOT_U[6] = {0, 0, 0, 1, 0, 1};
+__device__ __constant__ int M_SLOT_V[6] = {2, 2, 1, 2, 1, 2};
+
+__device__ __forceinline__ void left_to_right_d(int i, int j, int Nx, int Ny, int rot, int &ii, int &jj)
+{
+    // leftIndexes2Right (MeshTypes.f90:70-108)
+    switch (rot) {
+    case 0: ii = i; jj = j; break;
+    case 1: ii = Ny - j; jj = i; break;
+    case 2: ii = Nx - i; jj = Ny - j; break;
+    case 3: ii = j; jj = Nx - i; break;
+    case 4: ii = j; jj = i; break;
+    case 5: ii = Nx - i; jj = j; break;
+    case 6: ii = Ny - j; jj = Nx - i; break;
+    default: ii = i; jj = Ny - j; break;
+    }
+}
+
+// node of element e at element face coordinates (a, b) of slot s
+__device__ __forceinline__ long slot_node(const int *N3, const long *off, int e, int s, int a, int b)
+{
+    const int *Ne = &N3[3 * e];
+    int idx[3];
+    idx[M_SLOT_AXIS[s]] = M_SLOT_PLUS[s] ? Ne[M_SLOT_AXIS[s]] : 0;
+    idx[M_SLOT_U[s]] = a;
+    idx[M_SLOT_V[s]] = b;
+    return off[e] + idx[0] + (Ne[0] + 1) * (idx[1] + (Ne[1] + 1) * idx[2]);
+}
+
+// Face_AdaptSolutionToFace for one value at face node (i, j) of order Nf from the side's trace
+// in the face orientation, tr[c][m + (Ns0+1) l] (stride 64 between values).
+__device__ __forceinline__ double adapt(const OpsAll &ops, const FaceRec &r, int side, const double *tr,
+                                        int i, int j)
+{
+    const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1], n0 = r.Nf[0], n1 = r.Nf[1];
+    const int pt = r.proj[side];
+    if (pt == 0) return tr[i + (n0 + 1) * j];
+    const double *T0 = ops.T[N0][n0], *T1 = ops.T[N1][n1];
+    double acc = 0.0;
+    if (pt == 1) {
+        for (int l = 0; l <= N0; ++l) acc += T0[i * (N0 + 1) + l] * tr[l + (N0 + 1) * j];
+    } else if (pt == 2) {
+        for (int l = 0; l <= N1; ++l) acc += T1[j * (N1 + 1) + l] * tr[i + (N0 + 1) * l];
+    } else {
+        for (int l = 0; l <= N1; ++l)
+            for (int m = 0; m <= N0; ++m) acc += T0[i * (N0 + 1) + m] * T1[j * (N1 + 1) + l] * tr[m + (N0 + 1) * l];
+    }
+    return acc;
+}
+
+// Face_ProjectFluxToElements for one value at node (i, j) of the side's face-orientation grid
+// from the face-order values fl[q] (q = m + (Nf0+1) l).
+__device__ __forceinline__ double project(const OpsAll &ops, const FaceRec &r, int side, const double *fl,
+                                          int i, int j)
+{
+    const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1], n0 = r.Nf[0], n1 = r.Nf[1];
+    const int pt = r.proj[side];
+    if (pt == 0) return fl[i + (n0 + 1) * j];
+    const double *T0 = ops.T[n0][N0], *T1 = ops.T[n1][N1];
+    double acc = 0.0;
+    if (pt == 1) {
+        for (int l = 0; l <= n0; ++l) acc += T0[i * (n0 + 1) + l] * fl[l + (n0 + 1) * j];
+    } else if (pt == 2) {
+        for (int l = 0; l <= n1; ++l) acc += T1[j * (n1 + 1) + l] * fl[i + (n0 + 1) * l];
+    } else {
+        for (int l = 0; l <= n1; ++l)
+            for (int m = 0; m <= n0; ++m) acc += T0[i * (n0 + 1) + m] * T1[j * (n1 + 1) + l] * fl[m + (n0 + 1) * l];
+    }
+    return acc;
+}
+
+// Stage the side's element trace of nv node arrays src[c*S + g] into tr[c*64 + q] in the face
+// orientation (the right side is rotated, Face_AdaptSolutionToFace).  One wave, lane = q.
+__device__ __forceinline__ void gather_trace(const FaceRec &r, int side, const int *N3, const long *off,
+                                             const double *src, int nv, long S, double *tr, int lane)
+{
+    const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
+    const int nq = (N0 + 1) * (N1 + 1);
+    if (lane < nq) {
+        const int i = lane % (N0 + 1), j = lane / (N0 + 1);
+        int a = i, b = j;
+        if (side == 1) left_to_right_d(i, j, N0, N1, r.rot, a, b);
+        const long g = slot_node(N3, off, r.e[side], r.slot[side], a, b);
+        for (int c = 0; c < nv; ++c) tr[c * 64 + lane] = src[c * S + g];
+    }
+}
+
+// element face node of the side's face-orientation node (i, j)
+__device__ __forceinline__ long elem_face_node(const FaceRec &r, int side, const int *efoff, int i, int j)
+{
+    int a = i, b = j;
+    if (side == 1) left_to_right_d(i, j, r.Nfs[1][0], r.Nfs[1][1], r.rot, a, b);
+    return (long)efoff[6 * r.e[side] + r.slot[side]] + a + (r.Nel[side][0] + 1) * b;
+}
+
+// ------------------------------------------------------------------------------------
+// km_face_grad: BR1 unStar = 1/2 (U_R - U_L) |Ja| n at the face nodes
+// (BR1_ComputeElementInterfaceAverage, EllipticBR1.f90:571-627), projected onto both sides:
+// lift[(3v + d)... laid out as lift[(d*5 + v)*ES + x].
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const long *__restrict__ off,
+             const int *__restrict__ efoff, const FaceRec *__restrict__ faces, const double *__restrict__ fgeo,
+             const OpsAll *__restrict__ opsp, double *__restrict__ lift, int F, long S, long ES, long FSm)
+{
+    __shared__ double sT[2][NV * 64];
+    __shared__ double sU[3 * NV][64];
+    const int f = blockIdx.x, lane = threadIdx.x;
+    if (f >= F) return;   // uniform per block
+    const FaceRec r = faces[f];
+    const OpsAll &ops = *opsp;
+    gather_trace(r, 0, N3, off, Q, NV, S, sT[0], lane);
+    gather_trace(r, 1, N3, off, Q, NV, S, sT[1], lane);
+    __syncthreads();
+    const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
+    if (lane < nq) {
+        const int i = lane % (n0 + 1), j = lane / (n0 + 1);
+        const long fq = r.foff + lane;
+        const double nx = fgeo[fq], ny = fgeo[FSm + fq], nz = fgeo[2 * FSm + fq], jac = fgeo[9 * FSm + fq];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double qL = adapt(ops, r, 0, &sT[0][v * 64], i, j);
+            const double qR = adapt(ops, r, 1, &sT[1][v * 64], i, j);
+            const double uS = 0.5 * (qR - qL) * jac;
+            sU[0 * NV + v][lane] = uS * nx;
+            sU[1 * NV + v][lane] = uS * ny;
+            sU[2 * NV + v][lane] = uS * nz;
+        }
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; ++side) {
+        const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
+        if (lane < (N0 + 1) * (N1 + 1)) {
+            const int i = lane % (N0 + 1), j = lane / (N0 + 1);
+            const long x = elem_face_node(r, side, efoff, i, j);
+            for (int c = 0; c < 3 * NV; ++c) lift[c * ES + x] = project(ops, r, side, sU[c], i, j);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// km_grad: local gradient (HexElement_ComputeLocalGradient, HexElementClass.f90:427-531) and
+// BR1 lifting (BR1_GradientFaceLoop, EllipticBR1.f90:531-569, with the weights b of
+// VectorWeakIntegrals_StdFace): grad Q at every node, U[(5d + v)*S + g].
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(512)
+km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *__restrict__ N3,
+        const long *__restrict__ off, const int *__restrict__ efoff, const double *__restrict__ Ja,
+        const double *__restrict__ invJ, const double *__restrict__ lift, const OpsAll *__restrict__ opsp,
+        double *__restrict__ U, long S, long ES)
+{
+    HIP_DYNAMIC_SHARED(double, smem)
+    const int e = elist[blockIdx.x];
+    const OpsAll &ops = *opsp;
+    const int Nx = N3[3 * e], Ny = N3[3 * e + 1], Nz = N3[3 * e + 2];
+    const int nx = Nx + 1, ny = Ny + 1, nz = Nz + 1, ne = nx * ny * nz;
+    const int n = threadIdx.x;
+    const bool active = n < ne;
+    const int nc = active ? n : 0;   // loads are unconditional (see k_volume)
+    const int i = nc % nx, j = (nc / nx) % ny, k = nc / (nx * ny);
+    const long g = off[e] + nc;
+    double q[NV], ja[9];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
+    const double iJ = invJ[g];
+    // lifting values of the node's faces (slot order 5/3, 0/1, 2/4); absent faces give 0
+    const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
+    const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
+    const long xy = efoff[6 * e + (j == 0 ? 0 : 1)] + i + nx * k;
+    const long xz = efoff[6 * e + (k == 0 ? 2 : 4)] + i + nx * j;
+    double hx_[3 * NV], hy_[3 * NV], hz_[3 * NV];
+#pragma unroll
+    for (int c = 0; c < 3 * NV; ++c) {
+        const double a = lift[c * ES + xx], b = lift[c * ES + xy], d = lift[c * ES + xz];
+        hx_[c] = onx ? a : 0.0;
+        hy_[c] = ony ? b : 0.0;
+        hz_[c] = onz ? d : 0.0;
+    }
+    double *sQ = smem;   // [NV][ne]
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sQ[v * ne + n] = q[v];
+    }
+    __syncthreads();
+    if (!active) return;
+    const double *Dx = ops.D[Nx], *Dy = ops.D[Ny], *Dz = ops.D[Nz];
+    double uxi[NV], ueta[NV], uzeta[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { uxi[v] = 0.0; ueta[v] = 0.0; uzeta[v] = 0.0; }
+    for (int l = 0; l < nx; ++l) {
+        const double di = Dx[i * 8 + l];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uxi[v] = uxi[v] + sQ[v * ne + l + nx * (j + ny * k)] * di;
+    }
+    for (int l = 0; l < ny; ++l) {
+        const double dj = Dy[j * 8 + l];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ueta[v] = ueta[v] + sQ[v * ne + i + nx * (l + ny * k)] * dj;
+    }
+    for (int l = 0; l < nz; ++l) {
+        const double dk = Dz[k * 8 + l];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uzeta[v] = uzeta[v] + sQ[v * ne + i + nx * (j + ny * l)] * dk;
+    }
+    const double bx = ops.b0[Nx], by = ops.b0[Ny], bz = ops.b0[Nz];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double gr[3];
+        gr[0] = (uxi[v] * ja[0] + ueta[v] * ja[3] + uzeta[v] * ja[6]) * iJ;
+        gr[1] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
+        gr[2] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double ax = onx ? hx_[d * NV + v] * bx : 0.0;
+            const double ay = ony ? hy_[d * NV + v] * by : 0.0;
+            const double az = onz ? hz_[d * NV + v] * bz : 0.0;
+            const double fi = ((0.0 + ax) + ay) + az;
+            U[(d * NV + v) * S + g] = (onx || ony || onz) ? gr[d] + fi * iJ : gr[d];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// km_face: computeElementInterfaceFlux (SD.f90:1710-1799) at the face order with the face
+// viscosities of compute_viscosity_at_faces (SD.f90:1345-1398), projected back
+// (FaceClass.f90:597-700) and scaled by the receiving element's surface weight:
+// fe[v*ES + x] = (+/-) projected flux * b.
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *__restrict__ N3,
+        const long *__restrict__ off, const int *__restrict__ efoff, const FaceRec *__restrict__ faces,
+        const double *__restrict__ fgeo, const double *__restrict__ fdelta, const OpsAll *__restrict__ opsp,
+        double *__restrict__ fe, int F, long S, long ES, long FSm, Phys ph)
+{
+    constexpr int NT = NV + 3 * NV;   // Q and grad Q
+    __shared__ double sT[2][NT * 64];
+    __shared__ double sF[NV][64];
+    const int f = blockIdx.x, lane = threadIdx.x;
+    if (f >= F) return;   // uniform per block
+    const FaceRec r = faces[f];
+    const OpsAll &ops = *opsp;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        gather_trace(r, side, N3, off, Q, NV, S, sT[side], lane);
+        if (!ph.euler) gather_trace(r, side, N3, off, U, 3 * NV, S, sT[side] + NV * 64, lane);
+    }
+    __syncthreads();
+    const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
+    if (lane < nq) {
+        const int i = lane % (n0 + 1), j = lane / (n0 + 1);
+        const long fq = r.foff + lane;
+        double nrm[3], t1[3], t2[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nrm[c] = fgeo[c * FSm + fq];
+            t1[c] = fgeo[(3 + c) * FSm + fq];
+            t2[c] = fgeo[(6 + c) * FSm + fq];
+        }
+        const double jac = fgeo[9 * FSm + fq];
+        double qs[2][NV];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qs[s][v] = adapt(ops, r, s, &sT[s][v * 64], i, j);
+        double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (!ph.euler) {
+            // BR1_RiemannSolver (EllipticBR1.f90:816-868)
+            double Fs[2][NV][3];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                double gx[NV], gy[NV], gz[NV], vg[12];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    gx[v] = adapt(ops, r, s, &sT[s][(NV + v) * 64], i, j);
+                    gy[v] = adapt(ops, r, s, &sT[s][(2 * NV + v) * 64], i, j);
+                    gz[v] = adapt(ops, r, s, &sT[s][(3 * NV + v) * 64], i, j);
+                }
+                hx::viscous_state(ph, qs[s], gx, gy, gz, vg);
+                double mu, kappa;
+                hx::laminar_mu_kappa(ph, qs[s], mu, kappa);
+                if (ph.les) {
+                    const double mt = hx::les_viscosity(ph, fdelta[f], qs[s], vg);
+                    mu = mu + mt;
+                    kappa = kappa + mt * ph.mu_to_kappa;
+                }
+                const double ir = 1.0 / qs[s][0];
+                hx::viscous_flux(mu, kappa, qs[s][1] * ir, qs[s][2] * ir, qs[s][3] * ir, vg, Fs[s]);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                visc[v] = 0.5 * (Fs[0][v][0] + Fs[1][v][0]) * nrm[0] + 0.5 * (Fs[0][v][1] + Fs[1][v][1]) * nrm[1] +
+                          0.5 * (Fs[0][v][2] + Fs[1][v][2]) * nrm[2];
+        }
+        double inv[NV];
+        if (ph.riemann == 1) hx::lxf_flux(ph, qs[0], qs[1], nrm, t1, t2, inv);
+        else hx::roe_flux(ph, qs[0], qs[1], nrm, inv);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sF[v][lane] = (inv[v] - visc[v]) * jac;
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; ++side) {
+        const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
+        if (lane < (N0 + 1) * (N1 + 1)) {
+            const int i = lane % (N0 + 1), j = lane / (N0 + 1);
+            const long x = elem_face_node(r, side, efoff, i, j);
+            const int e = r.e[side], s = r.slot[side];
+            const double b = ops.b0[N3[3 * e + M_SLOT_AXIS[s]]];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double a = project(ops, r, side, sF[v], i, j);
+                fe[v * ES + x] = (side == 0 ? a : -1.0 * a) * b;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// km_volume: as hxk::k_volume with the element's own orders per direction, the viscous
+// state formed from grad Q (ViscousFlux_STATE), and MaxTimeStep's 1/|x_1 - x_0| per direction.
+// LDS (dynamic): sF[3][NF][ne], sN[ND][ne], sJ[3][ne].
+// ------------------------------------------------------------------------------------
+template <int FORM>
+__global__ void __launch_bounds__(512)
+km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *__restrict__ U,
+          const int *__restrict__ N3, const long *__restrict__ off, const int *__restrict__ efoff,
+          const double *__restrict__ Ja, const double *__restrict__ Jac, const double *__restrict__ edelta,
+          const double *__restrict__ fe, const OpsAll *__restrict__ opsp, double *__restrict__ G,
+          double *__restrict__ qdot_out, double a_k, double c_k, int update, DtArgs dta, long S, long ES,
+          Phys ph)
+{
+    constexpr bool SPLIT = FORM >= 0;
+    constexpr int AVG = SPLIT ? FORM : 0;
+    constexpr int ND = hx::node_data_size(AVG);
+    constexpr bool SYM = hx::two_point_symmetric(AVG);
+    constexpr int NF = SPLIT ? 4 : 5, V0 = SPLIT ? 1 : 0;
+    HIP_DYNAMIC_SHARED(double, smem)
+    const int e = elist[blockIdx.x];
+    const OpsAll &ops = *opsp;
+    const int Nx = N3[3 * e], Ny = N3[3 * e + 1], Nz = N3[3 * e + 2];
+    const int nx = Nx + 1, ny = Ny + 1, nz = Nz + 1, ne = nx * ny * nz;
+    const int t = threadIdx.x;
+    const bool active = t < ne;
+    const int n = active ? t : 0;
+    const int i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
+    const long g = off[e] + n;
+    const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
+    double *sF = smem;                        // [3][NF][ne]
+    double *sN = sF + 3 * NF * ne;            // [ND][ne]
+    double *sJ = sN + (SPLIT ? ND * ne : 0);  // [3][ne]
+    // loads, unconditional (see hxk::k_volume)
+    double sx[NV], sy[NV], sz[NV];
+    {
+        const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
+        const long xy = efoff[6 * e + (j == 0 ? 0 : 1)] + i + nx * k;
+        const long xz = efoff[6 * e + (k == 0 ? 2 : 4)] + i + nx * j;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double a = fe[v * ES + xx], b = fe[v * ES + xy], c = fe[v * ES + xz];
+            sx[v] = onx ? a : 0.0;
+            sy[v] = ony ? b : 0.0;
+            sz[v] = onz ? c : 0.0;
+        }
+    }
+    double q[NV], ja[9], gq[3][NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) gq[d][v] = U[(d * NV + v) * S + g];
+    const double dl = edelta[e];
+    const double jg = Jac[g];
+    if (active) {
+        double Fvc[3][NV];
+        if (!ph.euler) {
+            double vg[12];
+            hx::viscous_state(ph, q, gq[0], gq[1], gq[2], vg);
+            double mu, kappa;
+            hx::laminar_mu_kappa(ph, q, mu, kappa);
+            if (ph.les) {
+                const double mt = hx::les_viscosity(ph, dl, q, vg);
+                mu = mu + mt;
+                kappa = kappa + mt * ph.mu_to_kappa;
+            }
+            const double ir = 1.0 / q[0];
+            double Fv[NV][3];
+            hx::viscous_flux(mu, kappa, q[1] * ir, q[2] * ir, q[3] * ir, vg, Fv);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    Fvc[d][v] = Fv[v][0] * ja[3 * d] + Fv[v][1] * ja[3 * d + 1] + Fv[v][2] * ja[3 * d + 2];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fvc[d][v] = 0.0;
+        }
+        if (SPLIT) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 1; v < NV; ++v) sF[(d * NF + v - V0) * ne + n] = Fvc[d][v];
+            double a[ND];
+            hx::node_data<AVG>(ph, q, a);
+#pragma unroll
+            for (int c = 0; c < ND; ++c) sN[c * ne + n] = a[c];
+        } else {
+            double Fi[3][NV];
+            hx::euler_contravariant(ph, q, ja, Fi);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sF[(d * NF + v - V0) * ne + n] = Fi[d][v] - Fvc[d][v];
+        }
+    }
+    double xs[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) xs[v] = ((0.0 + sx[v]) + sy[v]) + sz[v];
+    double qd[NV], gk[NV], dt0 = 0.0, dt1 = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) qd[v] = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        if (SPLIT) {
+            if (d) __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sJ[c * ne + n] = ja[3 * d + c];
+            }
+        }
+        if (SPLIT || d == 0) __syncthreads();
+        if (d == 2) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gk[v] = G[v * S + g];
+            dt0 = hxk::bits_to_double(dta.in[0]);
+            dt1 = hxk::bits_to_double(dta.in[1]);
+        }
+        if (!active) continue;
+        const int own = d == 0 ? i : (d == 1 ? j : k);
+        const int Nd = d == 0 ? Nx : (d == 1 ? Ny : Nz);
+        const int stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
+        const int base = n - own * stride;
+        const double *hatD = ops.hatD[Nd], *sharpD = ops.sharpD[Nd];
+        if (!SPLIT) {
+            for (int l = 0; l <= Nd; ++l) {
+                const double hh = hatD[own * 8 + l];
+                const int m = base + l * stride;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[(d * NF + v) * ne + m];
+            }
+        } else {
+            double fi[NV];
+            hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+            for (int l = 0; l <= Nd; ++l) {
+                const double sh = sharpD[own * 8 + l], hh = hatD[own * 8 + l];
+                const int m = base + l * stride;
+                double a[ND], b[ND], jb[3], fs[NV];
+#pragma unroll
+                for (int c = 0; c < ND; ++c) { a[c] = sN[c * ne + n]; b[c] = sN[c * ne + m]; }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) jb[c] = sJ[c * ne + m];
+                if (SYM) {
+                    hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
+                } else {
+                    const bool lo = own < l;
+                    double x1[ND], x2[ND], j1[3], j2[3];
+#pragma unroll
+                    for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
+                    hx::two_point<AVG>(x1, j1, x2, j2, fs);
+                }
+                qd[0] = qd[0] + sh * (l == own ? fi[0] : fs[0]);
+#pragma unroll
+                for (int v = 1; v < NV; ++v)
+                    qd[v] = qd[v] + sh * (l == own ? fi[v] : fs[v]) + hh * sF[(d * NF + v - V0) * ne + m];
+            }
+        }
+    }
+    double tc = INFINITY, tv = INFINITY;
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (SPLIT) qd[v] = -qd[v];
+            hxk::hold(xs[v]);
+            qd[v] = qd[v] - xs[v];
+        }
+        const double ijg = 1.0 / jg;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qd[v] = qd[v] * ijg;
+        if (qdot_out) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qdot_out[v * S + g] = qd[v];
+        }
+        if (update) {
+            hxk::hold(dt0);
+            hxk::hold(dt1);
+            const double dt = fmin(dt0, dt1);
+            if (dta.time && blockIdx.x == 0 && t == 0) dta.time[0] += dt;
+            const double cdt = c_k * dt;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                hxk::hold(gk[v]);
+                const double gv = a_k * gk[v] + qd[v];
+                G[v * S + g] = gv;
+                q[v] = q[v] + cdt * gv;
+                Q[v * S + g] = q[v];
+            }
+            if (dta.out) {
+                // MaxTimeStep of the new state (DGSEMClass.f90:870-1035)
+                const double dc = ops.dcsi[Nx], de = ops.dcsi[Ny], dz = ops.dcsi[Nz];
+                const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
+                const double p = hx::pressure(ph, q);
+                const double c = sqrt(ph.gamma * p / q[0]);
+                const double e0 = u + c, e1 = v + c, e2 = w + c;
+                const double lx = fabs(ja[0] * e0 + ja[1] * e1 + ja[2] * e2) * dc;
+                const double ly = fabs(ja[3] * e0 + ja[4] * e1 + ja[5] * e2) * de;
+                const double lz = fabs(ja[6] * e0 + ja[7] * e1 + ja[8] * e2) * dz;
+                tc = dta.cfl * fabs(jg) / (lx + ly + lz);
+                if (!ph.euler) {
+                    const double mu = hx::sutherland(ph, ph.gammaM2 * p / q[0]);
+                    const double vx = mu * (dc * dc) * fabs(ja[0] + ja[1] + ja[2]);
+                    const double vy = mu * (de * de) * fabs(ja[3] + ja[4] + ja[5]);
+                    const double vz = mu * (dz * dz) * fabs(ja[6] + ja[7] + ja[8]);
+                    tv = dta.dcfl * fabs(jg) / (vx + vy + vz);
+                }
+            }
+        }
+    }
+    if (update && dta.out) {
+        for (int o = 32; o > 0; o >>= 1) {
+            tc = fmin(tc, __shfl_xor(tc, o, 64));
+            tv = fmin(tv, __shfl_xor(tv, o, 64));
+        }
+        if ((t & 63) == 0) {
+            atomicMin(&dta.out[0], (unsigned long long)__double_as_longlong(tc));
+            atomicMin(&dta.out[1], (unsigned long long)__double_as_longlong(tv));
+        }
+    }
+    if (dta.reset && blockIdx.x == 0 && t < 2) dta.reset[t] = 0x7FF0000000000000ull;
+}
+
+// MaxTimeStep of the resident state for mixed orders: one thread per node, the element of each
+// node from eofn (DGSEMClass.f90:870-1035).
+__global__ void __launch_bounds__(256)
+km_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ Jac,
+         const int *__restrict__ eofn, const int *__restrict__ N3, const OpsAll *__restrict__ opsp,
+         long nodes, long S, double cfl, double dcfl, Phys ph, unsigned long long *__restrict__ bits)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double tc = INFINITY, tv = INFINITY;
+    if (g < nodes) {
+        const int e = eofn[g];
+        const double dc = opsp->dcsi[N3[3 * e]], de = opsp->dcsi[N3[3 * e + 1]], dz = opsp->dcsi[N3[3 * e + 2]];
+        double q[NV], ja[9];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
+        const double jg = Jac[g];
+        const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
+        const double p = hx::pressure(ph, q);
+        const double c = sqrt(ph.gamma * p / q[0]);
+        const double e0 = u + c, e1 = v + c, e2 = w + c;
+        const double lx = fabs(ja[0] * e0 + ja[1] * e1 + ja[2] * e2) * dc;
+        const double ly = fabs(ja[3] * e0 + ja[4] * e1 + ja[5] * e2) * de;
+        const double lz = fabs(ja[6] * e0 + ja[7] * e1 + ja[8] * e2) * dz;
+        tc = cfl * fabs(jg) / (lx + ly + lz);
+        if (!ph.euler) {
+            const double mu = hx::sutherland(ph, ph.gammaM2 * p / q[0]);
+            const double vx = mu * (dc * dc) * fabs(ja[0] + ja[1] + ja[2]);
+            const double vy = mu * (de * de) * fabs(ja[3] + ja[4] + ja[5]);
+            const double vz = mu * (dz * dz) * fabs(ja[6] + ja[7] + ja[8]);
+            tv = dcfl * fabs(jg) / (vx + vy + vz);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        tc = fmin(tc, __shfl_xor(tc, o, 64));
+        tv = fmin(tv, __shfl_xor(tv, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&bits[0], (unsigned long long)__double_as_longlong(tc));
+        atomicMin(&bits[1], (unsigned long long)__double_as_longlong(tv));
+    }
+}
+
+}  // namespace hxm

@@ -274,18 +274,21 @@ struct DtArgs {
 
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-#ifndef HX3D_VOLUME_PAIRS
-#define HX3D_VOLUME_PAIRS 0
-#endif
-#ifndef HX3D_VOLUME_WAVES
-#define HX3D_VOLUME_WAVES 1
-#endif
 #ifndef HX3D_VOLUME_UNROLL
 #define HX3D_VOLUME_UNROLL 1
 #endif
 #define HX_STR(x) #x
 #define HX_UNROLL(n) _Pragma(HX_STR(unroll n))
 
+// One thread per node, EPB elements per block.  The contravariant fluxes of all three
+// directions (split form: the viscous ones, whose mass entry is zero; standard form: inviscid
+// minus viscous) and the two-point node data go to LDS once; the split form also stages the
+// metric row of the current direction, the only per-direction rewrite.  Per direction each node
+// accumulates, in l order, qd = (qd + sharpD F#) + hatD Fv (DGIntegrals.f90:92-129) or
+// qd = qd + hatD F (:56-87).
+#ifndef HX3D_VOLUME_WAVES
+#define HX3D_VOLUME_WAVES 1
+#endif
 template <int N, int EPB, int FORM>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
 __attribute__((amdgpu_waves_per_eu(HX3D_VOLUME_WAVES)))
@@ -299,15 +302,10 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     constexpr int AVG = SPLIT ? FORM : 0;
     constexpr int ND = hx::node_data_size(AVG);
     constexpr bool SYM = hx::two_point_symmetric(AVG);
-    constexpr bool PAIRS = SPLIT && HX3D_VOLUME_PAIRS;
-    constexpr int NPAIR = N * (N + 1) / 2;
-    // sF: contravariant flux of the current direction (standard: inviscid - viscous; split:
-    // viscous); sN: two-point node data; sJ: metric row of the current direction;
-    // sPair (pair variant): the two-point fluxes of the current direction, slot-major.
-    __shared__ double sF[EPB][NV][ne];
+    constexpr int NF = SPLIT ? 4 : 5, V0 = SPLIT ? 1 : 0;
+    __shared__ double sF[EPB][3][NF][ne];
     __shared__ double sN[SPLIT ? EPB : 1][SPLIT ? ND : 1][SPLIT ? ne : 1];
     __shared__ double sJ[SPLIT ? EPB : 1][SPLIT ? 3 : 1][SPLIT ? ne : 1];
-    __shared__ double sPair[PAIRS ? EPB : 1][PAIRS ? NPAIR : 1][PAIRS ? NV : 1][PAIRS ? nf : 1];
     __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
     const int nb = (K + EPB - 1) / EPB;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -318,38 +316,35 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     const long g = (long)e * ne + n;
     const int i = n % n1, j = (n / n1) % n1, k = n / nf;
     const bool onx = i == 0 || i == N, ony = j == 0 || j == N, onz = k == 0 || k == N;
-    // ---- Every global load of the node is issued here, unconditionally (padding threads read
-    // a clamped element and discard it): a load inside a branch makes the compiler wait for the
-    // whole memory queue at the first use of any earlier load.  The face values, the RK register
-    // and dt are consumed after the volume integral, so their latency overlaps it.
+    // ---- Loads are unconditional (padding threads read a clamped element and discard it): a
+    // load inside a branch makes the compiler wait for the whole memory queue at the first use
+    // of any earlier load.  The face values are issued first and summed once the node's own
+    // data has arrived (loads complete in order); G and dt are issued at the last direction
+    // and consumed after it.
     const long gc = (long)(e < K ? e : K - 1) * ne + n;
+    const int ec = (int)(gc / ne);
     const double hop = operator_entry<n1>(ops.hatD, t), sop = operator_entry<n1>(ops.sharpD, t);
-    double dt0 = bits_to_double(dta.in[0]), dt1 = bits_to_double(dta.in[1]);
-    double q[NV], ja[9], vg[12], gk[NV], sx[NV], sy[NV], sz[NV];
+    double sx[NV], sy[NV], sz[NV];
+    {
+        const long xx = efx<N>(ec, i == 0 ? 5 : 3, j, k), xy = efx<N>(ec, j == 0 ? 0 : 1, i, k),
+                   xz = efx<N>(ec, k == 0 ? 2 : 4, i, j);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double fx = fe[v * ES + xx], fy = fe[v * ES + xy], fz = fe[v * ES + xz];
+            sx[v] = onx ? fx : 0.0;   // faces the node is not on add an exact zero
+            sy[v] = ony ? fy : 0.0;
+            sz[v] = onz ? fz : 0.0;
+        }
+    }
+    double q[NV], ja[9], vg[12];
 #pragma unroll
     for (int v = 0; v < NV; ++v) q[v] = Q[v * S + gc];
 #pragma unroll
     for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + gc];
 #pragma unroll
     for (int c = 0; c < 12; ++c) vg[c] = vgrad[c * S + gc];
-    const double dl = edelta[gc / ne];
+    const double dl = edelta[ec];
     const double jg = Jac[gc];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) gk[v] = G[v * S + gc];
-    {
-        // surface values; faces the node is not on contribute an exact zero
-        const int ee = (int)(gc / ne);
-        const long xx = efx<N>(ee, i == 0 ? 5 : 3, j, k), xy = efx<N>(ee, j == 0 ? 0 : 1, i, k),
-                   xz = efx<N>(ee, k == 0 ? 2 : 4, i, j);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double fx = fe[v * ES + xx], fy = fe[v * ES + xy], fz = fe[v * ES + xz];
-            sx[v] = onx ? fx : 0.0;
-            sy[v] = ony ? fy : 0.0;
-            sz[v] = onz ? fz : 0.0;
-        }
-    }
-    double Fc[3][NV], a[ND];
     if (active) {
         double Fvc[3][NV];
         if (!ph.euler) {
@@ -379,7 +374,8 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) Fc[d][v] = Fvc[d][v];
+                for (int v = 1; v < NV; ++v) sF[el][d][v - V0][n] = Fvc[d][v];
+            double a[ND];
             hx::node_data<AVG>(ph, q, a);
 #pragma unroll
             for (int c = 0; c < ND; ++c) sN[el][c][n] = a[c];
@@ -389,134 +385,88 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) Fc[d][v] = Fi[d][v] - Fvc[d][v];
+                for (int v = 0; v < NV; ++v) sF[el][d][v - V0][n] = Fi[d][v] - Fvc[d][v];
         }
     }
+    // Surface integral (TimeDerivative_FacesContribution, SD.f90:1686-1701): the stored face
+    // values already carry b = 1/w_0 and the side's sign; faces in slot order 5, 3, 0, 1, 2, 4
+    double xs[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) xs[v] = ((0.0 + sx[v]) + sy[v]) + sz[v];
     if (t < n1 * n1) { sHat[t] = hop; sSharp[t] = sop; }
-    double qd[NV];
+    double qd[NV], gk[NV], dt0 = 0.0, dt1 = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = 0.0;
-    // ScalarWeakIntegrals_StdVolumeGreen (DGIntegrals.f90:56-87) or
-    // ScalarWeakIntegrals_SplitVolumeDivergence (:92-129): xi, then eta, then zeta, each in
-    // l order.  Every thread reaches every barrier; inactive threads (block padding) only skip
-    // the work between them.
+    // xi, then eta, then zeta.  Every thread reaches every barrier; inactive threads (block
+    // padding) only skip the work between them.
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        if (d) __syncthreads();     // previous direction's reads are done
-        if (active) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) sF[el][v][n] = Fc[d][v];
-            if (SPLIT) {
+        if (SPLIT) {
+            if (d) __syncthreads();     // previous direction's metric rows are read
+            if (active) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) sJ[el][c][n] = ja[3 * d + c];
             }
         }
-        __syncthreads();
+        if (SPLIT || d == 0) __syncthreads();
+        if (d == 2) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) gk[v] = G[v * S + gc];
+            dt0 = bits_to_double(dta.in[0]);
+            dt1 = bits_to_double(dta.in[1]);
+        }
+        if (!active) continue;
         const int own = d == 0 ? i : (d == 1 ? j : k);
-        const int line = d == 0 ? j + n1 * k : (d == 1 ? i + n1 * k : i + n1 * j);
-        auto node_at = [&](int l) {
-            return d == 0 ? l + n1 * (j + n1 * k) : (d == 1 ? i + n1 * (l + n1 * k) : i + n1 * (j + n1 * l));
-        };
-        // the two-point flux of nodes own and l of the line, lower index first
-        auto pair_flux = [&](int l, double fs[NV]) {
-            const int m = node_at(l);
-            double b[ND], jb[3];
-#pragma unroll
-            for (int c = 0; c < ND; ++c) b[c] = sN[el][c][m];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) jb[c] = sJ[el][c][m];
-            if (SYM) {
-                hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
-            } else {
-                const bool lo = own < l;
-                double x1[ND], x2[ND], j1[3], j2[3];
-#pragma unroll
-                for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
-                hx::two_point<AVG>(x1, j1, x2, j2, fs);
-            }
-        };
+        const int stride = d == 0 ? 1 : (d == 1 ? n1 : nf);
+        const int base = n - own * stride;
         if (!SPLIT) {
-            if (active) {
 #pragma unroll
-                for (int l = 0; l < n1; ++l) {
-                    const double hh = sHat[own * n1 + l];
-                    const int m = node_at(l);
+            for (int l = 0; l < n1; ++l) {
+                const double hh = sHat[own * n1 + l];
+                const int m = base + l * stride;
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[el][v][m];
-                }
-            }
-        } else if (!PAIRS) {
-            // SplitDG_ComputeSplitFormFluxes (HyperbolicSplitForm.f90:64-116): the diagonal is
-            // the inviscid contravariant flux, the other entries the two-point fluxes
-            if (active) {
-                double fi[NV];
-                hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
-                HX_UNROLL(HX3D_VOLUME_UNROLL)
-                for (int l = 0; l < n1; ++l) {
-                    const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
-                    const int m = node_at(l);
-                    double fs[NV];
-                    pair_flux(l, fs);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const double f = l == own ? fi[v] : fs[v];
-                        qd[v] = v == 0 ? qd[v] + sh * f : qd[v] + sh * f + hh * sF[el][v][m];
-                    }
-                }
+                for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[el][d][v][m];
             }
         } else {
-            // each unordered pair {lo, hi} of the line is evaluated once, by node own for
-            // l = own + s (mod N+1), s = 1..N/2 (and s = (N+1)/2 for own < (N+1)/2 when N is odd)
-            if (active) {
+            // SplitDG_ComputeSplitFormFluxes (HyperbolicSplitForm.f90:64-116): the diagonal is
+            // the inviscid contravariant flux, the other entries the two-point fluxes of
+            // (lower, higher) node index
+            double fi[NV];
+            hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+            HX_UNROLL(HX3D_VOLUME_UNROLL)
+            for (int l = 0; l < n1; ++l) {
+                const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
+                const int m = base + l * stride;
+                double a[ND], b[ND], jb[3], fs[NV];
 #pragma unroll
-                for (int s = 1; s <= (N + 1) / 2; ++s) {
-                    if (2 * s == N + 1 && own >= s) continue;
-                    const int l = (own + s) % n1;
-                    double fs[NV];
-                    pair_flux(l, fs);
-                    const int lo = own < l ? own : l, hi = own < l ? l : own;
-                    const int slot = lo * n1 - lo * (lo + 1) / 2 + (hi - lo - 1);
+                for (int c = 0; c < ND; ++c) { a[c] = sN[el][c][n]; b[c] = sN[el][c][m]; }
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) sPair[el][slot][v][line] = fs[v];
+                for (int c = 0; c < 3; ++c) jb[c] = sJ[el][c][m];
+                if (SYM) {
+                    hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
+                } else {
+                    const bool lo = own < l;
+                    double x1[ND], x2[ND], j1[3], j2[3];
+#pragma unroll
+                    for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
+                    hx::two_point<AVG>(x1, j1, x2, j2, fs);
                 }
-            }
-            __syncthreads();
-            if (active) {
-                double fi[NV];
-                hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+                qd[0] = qd[0] + sh * (l == own ? fi[0] : fs[0]);
 #pragma unroll
-                for (int l = 0; l < n1; ++l) {
-                    const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
-                    const int m = node_at(l);
-                    const int lo = own < l ? own : l, hi = own < l ? l : own;
-                    const int slot = lo * n1 - lo * (lo + 1) / 2 + (hi - lo - 1);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const double f = l == own ? fi[v] : sPair[el][slot < 0 ? 0 : slot][v][line];
-                        qd[v] = v == 0 ? qd[v] + sh * f : qd[v] + sh * f + hh * sF[el][v][m];
-                    }
-                }
+                for (int v = 1; v < NV; ++v)
+                    qd[v] = qd[v] + sh * (l == own ? fi[v] : fs[v]) + hh * sF[el][d][v - V0][m];
             }
         }
     }
     double tc = INFINITY, tv = INFINITY;   // next step's time-step limits of this node
     if (active) {
-        if (SPLIT) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qd[v] = -qd[v];
-        }
-        // Surface integral (TimeDerivative_FacesContribution, SD.f90:1686-1701): the stored
-        // face values already carry b = 1/w_0 and the side's sign; faces in slot order
-        // 5, 3, 0, 1, 2, 4 (absent faces add an exact zero)
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            hold(sx[v]);
-            hold(sy[v]);
-            hold(sz[v]);
-            const double x = ((0.0 + sx[v]) + sy[v]) + sz[v];
-            qd[v] = qd[v] - x;
+            if (SPLIT) qd[v] = -qd[v];
+            hold(xs[v]);
+            qd[v] = qd[v] - xs[v];
         }
         // QDot / J (SD.f90:490) as a product with 1/J: the rounding difference is one ulp of the
         // final derivative, taken after the cancellation of volume and surface terms

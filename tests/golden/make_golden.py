@@ -55,6 +55,15 @@ CASES = {
         n=2, N=7, amp=0.1, seed=9, steps=2, dumps=[0], noise=0.05, noise_seed=91,
         ctl=dict(mach=0.4, re=2000.0, riemann="lax-friedrichs", inviscid="standard",
                  averaging="Kennedy-Gruber")),
+    # config 3 in small: p-anisotropic orders (Nx, Ny, Nz) uniform on {2..5} per element, mortar
+    # faces between mismatched orders, velocity noise, Mach 0.2, Roe + BR1
+    "c3s_tgv_n3_mixed_p2to5_roe": dict(
+        n=3, N=5, orders=(2, 5, 33), amp=0.0, seed=0, steps=3, dumps=[0, 2], noise=1e-3, noise_seed=333,
+        ctl=dict(mach=0.2, re=1600.0)),
+    "x_noise_n2_mixed_p2to6_curved_split_pirozzoli_lxf": dict(
+        n=2, N=6, orders=(2, 6, 44), amp=0.1, seed=4, steps=2, dumps=[0], noise=0.05, noise_seed=441,
+        ctl=dict(mach=0.3, re=800.0, riemann="lax-friedrichs", inviscid="split-form",
+                 averaging="Pirozzoli")),
     "x_noise_n3_p2_standard_roe": dict(
         n=3, N=2, amp=0.1, seed=10, steps=2, dumps=[0], noise=0.05, noise_seed=101,
         ctl=dict(mach=0.8, re=400.0)),
@@ -74,6 +83,11 @@ def run_case(name, spec, workdir):
                       bface_order=max(4, spec["N"]))
     ctl = dict(spec["ctl"])
     mach, re = ctl.pop("mach"), ctl.pop("re")
+    if spec.get("orders"):
+        lo, hi, oseed = spec["orders"]
+        orders = np.random.default_rng(oseed).integers(lo, hi + 1, (spec["n"] ** 3, 3))
+        rc.write_orders(os.path.join(d, "orders.txt"), orders)
+        ctl["order_file"] = "orders.txt"
     env = {}
     if spec.get("noise"):
         # pass 1 (no steps) to obtain the node coordinates the reference builds
@@ -115,7 +129,7 @@ def run_case(name, spec, workdir):
     out["ke_monitor"] = ke
     meta = dict(spec["ctl"])
     meta.update(n=spec["n"], order=spec["N"], amp=spec["amp"], phase_seed=spec["seed"],
-                noise=spec.get("noise", 0.0))
+                noise=spec.get("noise", 0.0), orders=spec.get("orders"))
     out["meta"] = np.array(repr(meta))
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **out)

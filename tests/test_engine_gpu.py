@@ -112,3 +112,19 @@ def test_conservation_and_nan_flag():
     eng.set_state(Q)
     assert eng.has_nan()
     eng.close()
+
+
+def test_mixed_orders_three_steps_match_reference():
+    """p-anisotropic 3^3 box, orders (Nx, Ny, Nz) on {2..5}, mortar faces, Roe + BR1: three
+    device-dt RK3 steps against the reference's state and kinetic-energy monitor."""
+    case, meta = load_golden([p for p in GOLDEN if "c3s_tgv" in p][0])
+    assert len({tuple(n) for n in case["N"]}) > 1
+    eng = engine_for(case, physics_kwargs(meta))
+    eng.set_state(case["Q_step0"])
+    t = eng.advance(2, 0.4, 0.4, t=0.0)
+    assert po.normwise_error(eng.get_state(), case["Q_step2"]).max() < 1e-12
+    t = eng.advance(int(case["steps"]) - 2, 0.4, 0.4)
+    assert t == pytest.approx(float(case["t_final"]), rel=1e-13)
+    assert po.normwise_error(eng.get_state(), case["Q_final"]).max() < 1e-12
+    assert abs(eng.kinetic_energy() / case["ke_monitor"][-1, 2] - 1.0) < 1e-10
+    eng.close()
