@@ -217,6 +217,73 @@ def periodic_box(n, N, amplitude=0.0, phase_seed=0, phases=None):
     return case
 
 
+def random_orders(K, lo, hi, seed):
+    """Per-element orders (Nx, Ny, Nz) drawn uniformly on {lo..hi} from a seed, shape (K, 3)."""
+    return np.random.default_rng(seed).integers(lo, hi + 1, (K, 3)).astype(np.int32)
+
+
+def periodic_box_mixed(n, orders):
+    """Straight periodic box [-pi,pi]^3 of n^3 hexahedra with per-element orders
+    orders[e] = (Nx, Ny, Nz) (e = i + n*(j + n*k)).  Faces join element e's +axis face to its
+    neighbour's -axis face with rotation 0; a face between mismatched orders has the larger
+    order in each direction (Face_LinkWithElements, FaceClass.f90:187-278), its frame
+    n = +e_axis, t1 = e_u, t2 = n x t1 and |Ja| = (h/2)^2 at every face node.  Metrics of the
+    affine map: Ja^d_c = (h/2)^2 delta_dc, J = (h/2)^3."""
+    orders = np.asarray(orders, dtype=np.int32).reshape(-1, 3)
+    K = n ** 3
+    assert orders.shape[0] == K
+    h = 2.0 * L / n
+    ne = np.prod(orders + 1, axis=1)
+    off = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+    nodes = int(off[-1])
+    X = np.zeros((nodes, 3))
+    Ja = np.zeros((nodes, 9))
+    for c in range(3):
+        Ja[:, 4 * c] = (0.5 * h) ** 2
+    J = np.full(nodes, (0.5 * h) ** 3)
+    E = np.arange(K)
+    ei, ej, ek = E % n, (E // n) % n, E // (n * n)
+    lo = np.stack([-L + h * ei, -L + h * ej, -L + h * ek], axis=1)
+    for e in range(K):
+        Nx, Ny, Nz = orders[e]
+        xs = [gl_nodes(int(m))[0] for m in (Nx, Ny, Nz)]
+        P = np.stack(np.meshgrid(*xs, indexing="ij"), axis=-1)          # [i, j, k, 3]
+        pts = lo[e] + (P + 1.0) * 0.5 * h
+        X[off[e]:off[e + 1]] = pts.transpose(2, 1, 0, 3).reshape(-1, 3)  # i fastest
+    eid = lambda a, b, c: (a % n) + n * ((b % n) + n * (c % n))
+    faceID = np.zeros((K, 6), np.int32)
+    faceSide = np.zeros((K, 6), np.int32)
+    F = 3 * K
+    fElem = np.zeros((F, 2), np.int32)
+    fESide = np.zeros((F, 2), np.int32)
+    fNf = np.zeros((F, 2), np.int32)
+    uv = {0: (1, 2), 1: (0, 2), 2: (0, 1)}      # face coordinates (a, b) of an axis's faces
+    for a, (plus, minus) in enumerate([(ERIGHT, ELEFT), (EBACK, EFRONT), (ETOP, EBOTTOM)]):
+        nb = eid(ei + (a == 0), ej + (a == 1), ek + (a == 2))
+        f = 3 * E + a
+        fElem[f, 0], fElem[f, 1] = E, nb
+        fESide[f, 0], fESide[f, 1] = plus, minus
+        faceID[E, plus - 1], faceSide[E, plus - 1] = f, 1
+        faceID[nb, minus - 1], faceSide[nb, minus - 1] = f, 2
+        u, v = uv[a]
+        fNf[f, 0] = np.maximum(orders[E, u], orders[nb, u])
+        fNf[f, 1] = np.maximum(orders[E, v], orders[nb, v])
+    fn = np.prod(fNf + 1, axis=1)
+    foff = np.concatenate([[0], np.cumsum(fn)]).astype(np.int64)
+    FS = int(foff[-1])
+    normal = np.zeros((FS, 3))
+    t1 = np.zeros((FS, 3))
+    for f in range(F):
+        a = f % 3
+        normal[foff[f]:foff[f + 1], a] = 1.0
+        t1[foff[f]:foff[f + 1], uv[a][0]] = 1.0
+    t2 = np.cross(normal, t1)
+    fjac = np.full(FS, (0.5 * h) ** 2)
+    return dict(K=K, N=orders, off=off, faceID=faceID, faceSide=faceSide, F=F, fElem=fElem,
+                fESide=fESide, fRot=np.zeros(F, np.int32), fNf=fNf, foff=foff, Ja=Ja, J=J, x=X,
+                normal=normal, t1=t1, t2=t2, fjac=fjac, n=n, amplitude=0.0)
+
+
 def element_face_nodes(N, slot):
     """Node indices n = i + (N+1)(j + (N+1)k) of an element face slot, ordered (a + (N+1) b)."""
     n1 = N + 1
