@@ -75,14 +75,11 @@ __device__ __forceinline__ double wave_sum(double x)
 }
 
 // MaxTimeStep (DGSEMClass.f90:870-1035): per-node convective and viscous limits, block
-// minimum, then an unsigned 64-bit atomic minimum on the bit patterns (order-preserving for
-// non-negative doubles, so the result is exact and independent of scheduling).
+// minima to part, reduced by k_dt_reduce (the minimum is exact and independent of order).
 __global__ void __launch_bounds__(256)
 k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ Jac,
-        long nodes, long S, double dcsi, double cfl, double dcfl, Phys ph,
-        unsigned long long *__restrict__ bits)
+        long nodes, long S, double dcsi, double cfl, double dcfl, Phys ph, double *__restrict__ part)
 {
-    __shared__ double sc[4], sv[4];
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double tc = INFINITY, tv = INFINITY;
     if (g < nodes) {
@@ -112,21 +109,29 @@ k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doubl
             tv = dcfl * fabs(jac) / (vx + vy + vz);
         }
     }
-    tc = wave_min(tc);
-    tv = wave_min(tv);
-    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
-    if (ln == 0) { sc[w] = tc; sv[w] = tv; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) { tc = fmin(tc, sc[x]); tv = fmin(tv, sv[x]); }
-        atomicMin(&bits[0], (unsigned long long)__double_as_longlong(tc));
-        atomicMin(&bits[1], (unsigned long long)__double_as_longlong(tv));
-    }
+    hxk::block_min_pair(tc, tv, part, threadIdx.x);
 }
 
-__global__ void k_dt_reset(unsigned long long *bits, int n)
+// The minima of n (convective, viscous) pairs as the bit patterns of a dt slot.
+__global__ void __launch_bounds__(1024)
+k_dt_reduce(const double *__restrict__ part, int n, unsigned long long *__restrict__ bits)
 {
-    if ((int)threadIdx.x < n) bits[threadIdx.x] = 0x7FF0000000000000ull;  // +inf
+    __shared__ double sa[16], sb[16];
+    double a = INFINITY, b = INFINITY;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) {
+        a = fmin(a, part[2 * x]);
+        b = fmin(b, part[2 * x + 1]);
+    }
+    a = wave_min(a);
+    b = wave_min(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) { a = fmin(a, sa[x]); b = fmin(b, sb[x]); }
+        bits[0] = (unsigned long long)__double_as_longlong(a);
+        bits[1] = (unsigned long long)__double_as_longlong(b);
+    }
 }
 
 // a host-given dt as both limits of a dt slot
@@ -354,7 +359,8 @@ struct hx3d_engine {
     double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
     double *dFe = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
     double *dJ = nullptr, *dFgeo = nullptr, *dEgeo = nullptr;
-    double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr;
+    double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr, *dDtPart = nullptr;
+    int vol_blocks = 0;   // blocks of the last uniform-order volume launch
     unsigned long long *dBits = nullptr;
     int *dNbr = nullptr, *dFlag = nullptr;
     int4 *dFnode = nullptr;
@@ -415,6 +421,7 @@ void launch_volume_e(hx3d_engine *eng, double a_k, double c_k, int update, const
     constexpr int ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
+    eng->vol_blocks = nb;
     hipLaunchKernelGGL((hxk::k_volume<N, EPB, FORM>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
                        eng->dVgrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dG,
                        qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->ES, eng->ops, eng->ph);
@@ -474,7 +481,7 @@ void stage(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &d
     }
 }
 
-// dt slots (pairs of bit patterns) in dBits: 0-2 rotate through hx3d_advance, 3 is
+// dt slots (pairs of bit patterns) in dBits: 0-1 alternate through hx3d_advance, 3 is
 // hx3d_max_dt's, 4 holds a host-given dt
 constexpr int DT_SLOTS = 5;
 
@@ -494,7 +501,6 @@ void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long 
 void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 {
     unsigned long long *b = eng->dBits + 2 * slot;
-    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, b, 2);
     launch_maxdt_bits(eng, cfl, dcfl, b);
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, b, eng->dScal, 0);
 }
@@ -566,13 +572,16 @@ void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const 
     constexpr bool SPLIT = FORM >= 0;
     constexpr int ND = hx::node_data_size(SPLIT ? FORM : 0);
     const int per_node = 3 * (SPLIT ? 4 : 5) + (SPLIT ? ND + 3 : 0);
+    DtArgs a = dta;   // time and the slot reset are advanced by the first launch only
     for (int w = 1; w <= 8; ++w) {
         if (!eng->bucket_count[w]) continue;
         const size_t lds = sizeof(double) * per_node * 64 * w;
         hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->bucket_count[w]), dim3(64 * w), lds, eng->stream,
                            eng->dElist + eng->bucket_first[w], eng->dQ, eng->dU, eng->dN3, eng->dOff,
                            eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG,
-                           qdot, a_k, c_k, update, dta, eng->S, eng->ES, eng->ph);
+                           qdot, a_k, c_k, update, a, eng->S, eng->ES, eng->ph);
+        a.time = nullptr;
+        if (a.part) a.part += 2 * (long)eng->bucket_count[w];
     }
 }
 
@@ -606,16 +615,17 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
     if (ev) hipEventRecord(ev[3], eng->stream);
 }
 
-// MaxTimeStep limits of the resident state accumulated into bits[0..1] (reset beforehand)
+// MaxTimeStep limits of the resident state into the dt slot bits[0..1]
 void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long long *bits)
 {
     const long nb = (eng->nodes + 255) / 256;
     if (eng->mixed)
         hipLaunchKernelGGL(hxm::km_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ, eng->dEofn,
-                           eng->dN3, eng->dOpsAll, eng->nodes, eng->S, cfl, dcfl, eng->ph, bits);
+                           eng->dN3, eng->dOpsAll, eng->nodes, eng->S, cfl, dcfl, eng->ph, eng->dDtPart);
     else
         hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ, eng->nodes,
-                           eng->S, eng->dcsi, cfl, dcfl, eng->ph, bits);
+                           eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dDtPart);
+    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, eng->stream, eng->dDtPart, (int)nb, bits);
 }
 
 // physics constants (PhysicsStorage_NS.f90:145-439, ThermodynamicsAir :115-132)
@@ -776,6 +786,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) ||
         (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) || (rc = dalloc(eng, &eng->dScal, 4)) ||
         (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
+        (rc = dalloc(eng, &eng->dDtPart, (size_t)2 * (std::max((long)K, (S + 255) / 256) + 64))) ||
         (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
         (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dN3, N3.size())) ||
         (rc = dalloc(eng, &eng->dOff, off.size())) || (rc = dalloc(eng, &eng->dEfoff, efoff.size())) ||
@@ -978,6 +989,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         (rc = dalloc(eng, &eng->dFe, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dEgeo, (size_t)4 * ES)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) || (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) ||
         (rc = dalloc(eng, &eng->dScal, 4)) || (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
+        (rc = dalloc(eng, &eng->dDtPart, (size_t)2 * (std::max((long)K, (S + 255) / 256) + 64))) ||
         (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
         (rc = dalloc(eng, &eng->dNbr, nbr.size())) ||
         (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
@@ -1011,7 +1023,7 @@ int hx3d_engine_destroy(hx3d_engine *eng)
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff, eng->dElist,
-                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift};
+                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto &e : eng->evpool) hipEventDestroy(e);
@@ -1108,23 +1120,21 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
         HIPCHK(hipEventCreate(&e));
         eng->evpool.push_back(e);
     }
-    // dt of step s: the minimum of the limits in slot s%3 (slot 0 from a standalone MaxTimeStep
-    // of the current state; later slots accumulated by the last stage of the previous step
-    // from the state it produces).  Stage 1 of step s resets slot (s+2)%3 for step s+1.
-    if (nsteps > 0) {
-        hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, eng->stream, eng->dBits, 6);
-        launch_maxdt_bits(eng, cfl, dcfl, eng->dBits);
-    }
+    // dt of step s: the minimum of the limits in slot s%2 (slot 0 from a standalone MaxTimeStep
+    // of the current state; later slots from the block minima the last stage of the previous
+    // step forms of the state it produces, reduced by k_dt_reduce).
+    if (nsteps > 0) launch_maxdt_bits(eng, cfl, dcfl, eng->dBits);
     for (int s = 0; s < nsteps; ++s) {
         for (int k = 0; k < 3; ++k) {
             DtArgs a{};
-            a.in = eng->dBits + 2 * (s % 3);
+            a.in = eng->dBits + 2 * (s % 2);
             a.cfl = cfl; a.dcfl = dcfl; a.dcsi = eng->dcsi;
-            if (k == 0) a.reset = eng->dBits + 2 * ((s + 2) % 3);
-            if (k == 2) { a.out = eng->dBits + 2 * ((s + 1) % 3); a.time = eng->dScal + 1; }
+            if (k == 2) { a.part = eng->dDtPart; a.time = eng->dScal + 1; }
             stage(eng, RK_A[k], RK_C[k], 1, a, nullptr,
                   eng->timing ? &eng->evpool[(size_t)12 * s + 4 * k] : nullptr);
         }
+        hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, eng->stream, eng->dDtPart,
+                           eng->mixed ? eng->K : eng->vol_blocks, eng->dBits + 2 * ((s + 1) % 2));
     }
     HIPCHK(hipGetLastError());
     if (eng->timing && nsteps > 0) {

@@ -563,7 +563,7 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
                 q[v] = q[v] + cdt * gv;
                 Q[v * S + g] = q[v];
             }
-            if (dta.out) {
+            if (dta.part) {
                 // MaxTimeStep of the new state (DGSEMClass.f90:870-1035)
                 const double dc = ops.dcsi[Nx], de = ops.dcsi[Ny], dz = ops.dcsi[Nz];
                 const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
@@ -584,25 +584,15 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
             }
         }
     }
-    if (update && dta.out) {
-        for (int o = 32; o > 0; o >>= 1) {
-            tc = fmin(tc, __shfl_xor(tc, o, 64));
-            tv = fmin(tv, __shfl_xor(tv, o, 64));
-        }
-        if ((t & 63) == 0) {
-            atomicMin(&dta.out[0], (unsigned long long)__double_as_longlong(tc));
-            atomicMin(&dta.out[1], (unsigned long long)__double_as_longlong(tv));
-        }
-    }
-    if (dta.reset && blockIdx.x == 0 && t < 2) dta.reset[t] = 0x7FF0000000000000ull;
+    if (update && dta.part) hxk::block_min_pair(tc, tv, dta.part, t);
 }
 
 // MaxTimeStep of the resident state for mixed orders: one thread per node, the element of each
-// node from eofn (DGSEMClass.f90:870-1035).
+// node from eofn (DGSEMClass.f90:870-1035); block minima to part (see k_dt_reduce).
 __global__ void __launch_bounds__(256)
 km_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ Jac,
          const int *__restrict__ eofn, const int *__restrict__ N3, const OpsAll *__restrict__ opsp,
-         long nodes, long S, double cfl, double dcfl, Phys ph, unsigned long long *__restrict__ bits)
+         long nodes, long S, double cfl, double dcfl, Phys ph, double *__restrict__ part)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double tc = INFINITY, tv = INFINITY;
@@ -631,14 +621,7 @@ km_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doub
             tv = dcfl * fabs(jg) / (vx + vy + vz);
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        tc = fmin(tc, __shfl_xor(tc, o, 64));
-        tv = fmin(tv, __shfl_xor(tv, o, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&bits[0], (unsigned long long)__double_as_longlong(tc));
-        atomicMin(&bits[1], (unsigned long long)__double_as_longlong(tv));
-    }
+    hxk::block_min_pair(tc, tv, part, threadIdx.x);
 }
 
 }  // namespace hxm

@@ -266,13 +266,34 @@ k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const dou
 // ------------------------------------------------------------------------------------
 struct DtArgs {
     const unsigned long long *in;   // [2]: convective, viscous limit bit patterns of this step's dt
-    unsigned long long *out;        // [2]: accumulate the next step's limits (or null)
-    unsigned long long *reset;      // [2]: set to +inf by one thread (or null)
+    double *part;                   // [2 * blocks]: this block's minima of the next step's limits
+                                    // (or null); reduced by k_dt_reduce
     double *time;                   // t += dt by one thread (or null)
     double cfl, dcfl, dcsi;         // MaxTimeStep parameters (DGSEMClass.f90:870-1035)
 };
 
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Block minimum of (a, b) written to part[2*blockIdx.x + 0/1]; every thread of the block calls
+// it (inactive threads with +inf).  The minimum of doubles is exact and independent of order,
+// so the two-level reduction (here, then k_dt_reduce) gives the same bits as a serial one,
+// without contended atomics.
+__device__ __forceinline__ void block_min_pair(double a, double b, double *part, int t)
+{
+    __shared__ double sm[2][16];
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmin(a, __shfl_xor(a, o, 64));
+        b = fmin(b, __shfl_xor(b, o, 64));
+    }
+    if ((t & 63) == 0) { sm[0][t >> 6] = a; sm[1][t >> 6] = b; }
+    __syncthreads();
+    if (t == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; ++w) { a = fmin(a, sm[0][w]); b = fmin(b, sm[1][w]); }
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
 
 #ifndef HX3D_VOLUME_UNROLL
 #define HX3D_VOLUME_UNROLL 1
@@ -491,7 +512,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
                 q[v] = q[v] + cdt * gv;
                 Q[v * S + g] = q[v];
             }
-            if (dta.out) {
+            if (dta.part) {
                 // MaxTimeStep of the new state (DGSEMClass.f90:870-1035 with
                 // ComputeEigenvaluesForState, Physics_NS.f90:927-962)
                 const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
@@ -513,19 +534,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             }
         }
     }
-    if (update && dta.out) {
-        // every lane of the (full, 64-lane) wave takes part; inactive lanes hold +inf.  The
-        // minimum over bit patterns is exact and order independent for non-negative doubles.
-        for (int o = 32; o > 0; o >>= 1) {
-            tc = fmin(tc, __shfl_xor(tc, o, 64));
-            tv = fmin(tv, __shfl_xor(tv, o, 64));
-        }
-        if ((t & 63) == 0) {
-            atomicMin(&dta.out[0], (unsigned long long)__double_as_longlong(tc));
-            atomicMin(&dta.out[1], (unsigned long long)__double_as_longlong(tv));
-        }
-    }
-    if (dta.reset && blockIdx.x == 0 && t < 2) dta.reset[t] = 0x7FF0000000000000ull;
+    if (update && dta.part) block_min_pair(tc, tv, dta.part, t);
 }
 
 }  // namespace hxk
