@@ -407,12 +407,12 @@ void launch_grad(hx3d_engine *eng)
 }
 
 template <int N>
-void launch_face(hx3d_engine *eng)
+void launch_face(hx3d_engine *eng, const hxk::DtReduce &dtr)
 {
     const int nb = grid_for((int)((eng->FS + 255) / 256));
     hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dVgrad,
                        eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
-                       1.0 / eng->ops.w[0], eng->ph);
+                       1.0 / eng->ops.w[0], eng->ph, dtr);
 }
 
 template <int N, int FORM, int EPB>
@@ -456,7 +456,7 @@ void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs 
     if (ev) hipEventRecord(ev[0], eng->stream);
     if (!eng->ph.euler) launch_grad<N>(eng);
     if (ev) hipEventRecord(ev[1], eng->stream);
-    launch_face<N>(eng);
+    launch_face<N>(eng, dta.red);
     if (ev) hipEventRecord(ev[2], eng->stream);
     launch_volume<N>(eng, a_k, c_k, update, dta, qdot);
     if (ev) hipEventRecord(ev[3], eng->stream);
@@ -603,7 +603,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
     if (ev) hipEventRecord(ev[1], eng->stream);
     hipLaunchKernelGGL(hxm::km_face, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
                        eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll, eng->dFe, eng->F, eng->S,
-                       eng->ES, eng->FSm, eng->ph);
+                       eng->ES, eng->FSm, eng->ph, dta.red);
     if (ev) hipEventRecord(ev[2], eng->stream);
     if (eng->inviscid != HX3D_INVISCID_SPLIT) launch_mvolume<hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
     else switch (eng->ph.averaging) {
@@ -1120,21 +1120,23 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
         HIPCHK(hipEventCreate(&e));
         eng->evpool.push_back(e);
     }
-    // dt of step s: the minimum of the limits in slot s%2 (slot 0 from a standalone MaxTimeStep
-    // of the current state; later slots from the block minima the last stage of the previous
-    // step forms of the state it produces, reduced by k_dt_reduce).
+    // dt of step s: the minimum of the limits in slot s%2.  Slot 0 comes from a standalone
+    // MaxTimeStep of the current state; the last stage of step s-1 leaves block minima of the
+    // limits of the state it produces, which block 0 of step s's first face kernel reduces
+    // into the slot before any kernel of step s reads dt.
     if (nsteps > 0) launch_maxdt_bits(eng, cfl, dcfl, eng->dBits);
+    int nparts = 0;
     for (int s = 0; s < nsteps; ++s) {
         for (int k = 0; k < 3; ++k) {
             DtArgs a{};
             a.in = eng->dBits + 2 * (s % 2);
             a.cfl = cfl; a.dcfl = dcfl; a.dcsi = eng->dcsi;
+            if (k == 0 && s > 0) a.red = hxk::DtReduce{eng->dDtPart, nparts, eng->dBits + 2 * (s % 2)};
             if (k == 2) { a.part = eng->dDtPart; a.time = eng->dScal + 1; }
             stage(eng, RK_A[k], RK_C[k], 1, a, nullptr,
                   eng->timing ? &eng->evpool[(size_t)12 * s + 4 * k] : nullptr);
+            if (k == 2) nparts = eng->mixed ? eng->K : eng->vol_blocks;
         }
-        hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, eng->stream, eng->dDtPart,
-                           eng->mixed ? eng->K : eng->vol_blocks, eng->dBits + 2 * ((s + 1) % 2));
     }
     HIPCHK(hipGetLastError());
     if (eng->timing && nsteps > 0) {

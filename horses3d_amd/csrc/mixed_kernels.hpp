@@ -285,12 +285,13 @@ __global__ void __launch_bounds__(64)
 km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *__restrict__ N3,
         const long *__restrict__ off, const int *__restrict__ efoff, const FaceRec *__restrict__ faces,
         const double *__restrict__ fgeo, const double *__restrict__ fdelta, const OpsAll *__restrict__ opsp,
-        double *__restrict__ fe, int F, long S, long ES, long FSm, Phys ph)
+        double *__restrict__ fe, int F, long S, long ES, long FSm, Phys ph, hxk::DtReduce dtr)
 {
     constexpr int NT = NV + 3 * NV;   // Q and grad Q
     __shared__ double sT[2][NT * 64];
     __shared__ double sF[NV][64];
     const int f = blockIdx.x, lane = threadIdx.x;
+    if (dtr.bits && f == 0) hxk::reduce_dt_partials(dtr.part, dtr.n, dtr.bits, lane);
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
     const OpsAll &ops = *opsp;

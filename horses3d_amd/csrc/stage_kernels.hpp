@@ -65,8 +65,12 @@ __device__ __forceinline__ long efx(int e, int slot, int a, int b)
 //   nbr[x]        2 * (neighbour's node) + (1 if this element is the face's right side)
 //   egeo[c*ES+x]  face normal (c = 0..2) and surface Jacobian |Ja| (c = 3) of the face
 // ------------------------------------------------------------------------------------
+#ifndef HX3D_GRAD_WAVES
+#define HX3D_GRAD_WAVES 1
+#endif
 template <int N, int EPB>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
+__attribute__((amdgpu_waves_per_eu(HX3D_GRAD_WAVES)))
 k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
        const int *__restrict__ nbr, const double *__restrict__ egeo, double *__restrict__ vgrad,
        int K, long S, long ES, Ops ops, Phys ph)
@@ -194,13 +198,24 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
 // The flux times the surface weight b = 1/w_0 is stored for both sides in element face order,
 // negated for the right side (TimeDerivative_FacesContribution, SD.f90:1686-1701).
 // ------------------------------------------------------------------------------------
+__device__ __forceinline__ void reduce_dt_partials(const double *part, int n, unsigned long long *bits, int t);
+
+// dtred: (partials, count, slot) of the previous step's dt limits, reduced by block 0 first
+// (or null)
+struct DtReduce {
+    const double *part;
+    int n;
+    unsigned long long *bits;
+};
+
 template <int N>
 __global__ void __launch_bounds__(256)
 k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ fgeo,
        const int4 *__restrict__ fnode, const double *__restrict__ fdelta, double *__restrict__ fe,
-       long FS, long S, long ES, double b0, Phys ph)
+       long FS, long S, long ES, double b0, Phys ph, DtReduce dtr)
 {
     constexpr int n1 = N + 1, nf = n1 * n1;
+    if (dtr.bits && blockIdx.x == 0) reduce_dt_partials(dtr.part, dtr.n, dtr.bits, threadIdx.x);
     const long fq = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (fq >= FS) return;
     const int4 gg = fnode[fq];
@@ -269,6 +284,7 @@ struct DtArgs {
     double *part;                   // [2 * blocks]: this block's minima of the next step's limits
                                     // (or null); reduced by k_dt_reduce
     double *time;                   // t += dt by one thread (or null)
+    DtReduce red;                   // consumed by the stage's face kernel (not the volume kernel)
     double cfl, dcfl, dcsi;         // MaxTimeStep parameters (DGSEMClass.f90:870-1035)
 };
 
@@ -295,6 +311,30 @@ __device__ __forceinline__ void block_min_pair(double a, double b, double *part,
     }
 }
 
+// The minima of n (convective, viscous) pairs of part as the bit patterns of a dt slot; every
+// thread of the calling block takes part.
+__device__ __forceinline__ void reduce_dt_partials(const double *part, int n, unsigned long long *bits, int t)
+{
+    __shared__ double sm[2][16];
+    double a = INFINITY, b = INFINITY;
+    for (int x = t; x < n; x += blockDim.x) {
+        a = fmin(a, part[2 * x]);
+        b = fmin(b, part[2 * x + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmin(a, __shfl_xor(a, o, 64));
+        b = fmin(b, __shfl_xor(b, o, 64));
+    }
+    if ((t & 63) == 0) { sm[0][t >> 6] = a; sm[1][t >> 6] = b; }
+    __syncthreads();
+    if (t == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; ++w) { a = fmin(a, sm[0][w]); b = fmin(b, sm[1][w]); }
+        bits[0] = (unsigned long long)__double_as_longlong(a);
+        bits[1] = (unsigned long long)__double_as_longlong(b);
+    }
+}
+
 #ifndef HX3D_VOLUME_UNROLL
 #define HX3D_VOLUME_UNROLL 1
 #endif
@@ -307,12 +347,13 @@ __device__ __forceinline__ void block_min_pair(double a, double b, double *part,
 // metric row of the current direction, the only per-direction rewrite.  Per direction each node
 // accumulates, in l order, qd = (qd + sharpD F#) + hatD Fv (DGIntegrals.f90:92-129) or
 // qd = qd + hatD F (:56-87).
-#ifndef HX3D_VOLUME_WAVES
-#define HX3D_VOLUME_WAVES 1
-#endif
+// Minimum waves per SIMD asked of the register allocator: 3 where the kernel fits in 168 VGPRs
+// without spilling (standard form, Kennedy-Gruber, Pirozzoli), else no constraint.
+constexpr int volume_waves(int form) { return (form == -1 || form == 2 || form == 3) ? 3 : 1; }
+
 template <int N, int EPB, int FORM>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
-__attribute__((amdgpu_waves_per_eu(HX3D_VOLUME_WAVES)))
+__attribute__((amdgpu_waves_per_eu(volume_waves(FORM))))
 k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ Ja,
          const double *__restrict__ Jac, const double *__restrict__ edelta, const double *__restrict__ fe,
          double *__restrict__ G, double *__restrict__ qdot_out, double a_k, double c_k, int update,
