@@ -86,28 +86,10 @@ k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doubl
         double q[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
-        // ComputeEigenvaluesForState (Physics_NS.f90:927-962)
-        const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
-        const double p = hx::pressure(ph, q);
-        const double a = sqrt(ph.gamma * p / q[0]);
-        const double ev0 = u + a, ev1 = v + a, ev2 = w + a;
         double ja[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
-        const double jac = Jac[g];
-        const double lx = fabs(ja[0] * ev0 + ja[1] * ev1 + ja[2] * ev2) * dcsi;
-        const double ly = fabs(ja[3] * ev0 + ja[4] * ev1 + ja[5] * ev2) * dcsi;
-        const double lz = fabs(ja[6] * ev0 + ja[7] * ev1 + ja[8] * ev2) * dcsi;
-        tc = cfl * fabs(jac) / (lx + ly + lz);
-        if (!ph.euler) {
-            const double T = ph.gammaM2 * p / q[0];
-            const double mu = hx::sutherland(ph, T);
-            const double d2 = dcsi * dcsi;
-            const double vx = mu * d2 * fabs(ja[0] + ja[1] + ja[2]);
-            const double vy = mu * d2 * fabs(ja[3] + ja[4] + ja[5]);
-            const double vz = mu * d2 * fabs(ja[6] + ja[7] + ja[8]);
-            tv = dcfl * fabs(jac) / (vx + vy + vz);
-        }
+        hx::dt_limits(ph, q, ja, Jac[g], dcsi, dcsi, dcsi, cfl, dcfl, tc, tv);
     }
     hxk::block_min_pair(tc, tv, part, threadIdx.x);
 }
@@ -356,11 +338,10 @@ struct hx3d_engine {
     int vol_tile = 0;   // 0: default elements per block of the volume kernel, 1: half
     double dcsi = 0;
     hipStream_t stream = nullptr;
-    double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr, *dInvJ = nullptr;
+    double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr;
     double *dFe = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
     double *dJ = nullptr, *dFgeo = nullptr, *dEgeo = nullptr;
     double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr, *dDtPart = nullptr;
-    int vol_blocks = 0;   // blocks of the last uniform-order volume launch
     unsigned long long *dBits = nullptr;
     int *dNbr = nullptr, *dFlag = nullptr;
     int4 *dFnode = nullptr;
@@ -395,15 +376,22 @@ int dalloc(hx3d_engine *eng, T **p, size_t n)
 // elements per block for the element kernels
 constexpr int epb_for(int N) { return N == 1 ? 16 : N == 2 ? 8 : N == 3 ? 4 : N == 4 ? 2 : 1; }
 
+// blocks of the uniform-order gradient kernel
+int grad_grid(const hx3d_engine *eng)
+{
+    const int epb = epb_for(eng->N);
+    return ((((eng->K + epb - 1) / epb) + 7) / 8) * 8;
+}
+
 template <int N>
-void launch_grad(hx3d_engine *eng)
+void launch_grad(hx3d_engine *eng, const hxk::DtLimits &dtl)
 {
     constexpr int EPB = epb_for(N), ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
     hipLaunchKernelGGL((hxk::k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
-                       eng->dInvJ, eng->dNbr, eng->dEgeo, eng->dVgrad, eng->K, eng->S, eng->ES,
-                       eng->ops, eng->ph);
+                       eng->dJ, eng->dNbr, eng->dEgeo, eng->dVgrad, eng->K, eng->S, eng->ES,
+                       eng->ops, eng->ph, dtl);
 }
 
 template <int N>
@@ -421,7 +409,6 @@ void launch_volume_e(hx3d_engine *eng, double a_k, double c_k, int update, const
     constexpr int ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
     const int nb = grid_for((eng->K + EPB - 1) / EPB);
-    eng->vol_blocks = nb;
     hipLaunchKernelGGL((hxk::k_volume<N, EPB, FORM>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
                        eng->dVgrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dG,
                        qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->ES, eng->ops, eng->ph);
@@ -454,7 +441,7 @@ void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs 
              hipEvent_t *ev)
 {
     if (ev) hipEventRecord(ev[0], eng->stream);
-    if (!eng->ph.euler) launch_grad<N>(eng);
+    if (!eng->ph.euler) launch_grad<N>(eng, dta.lim);
     if (ev) hipEventRecord(ev[1], eng->stream);
     launch_face<N>(eng, dta.red);
     if (ev) hipEventRecord(ev[2], eng->stream);
@@ -581,7 +568,6 @@ void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const 
                            eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG,
                            qdot, a_k, c_k, update, a, eng->S, eng->ES, eng->ph);
         a.time = nullptr;
-        if (a.part) a.part += 2 * (long)eng->bucket_count[w];
     }
 }
 
@@ -589,6 +575,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
                  hipEvent_t *ev)
 {
     if (ev) hipEventRecord(ev[0], eng->stream);
+    hxk::DtLimits lim = dta.lim;
     if (!eng->ph.euler) {
         hipLaunchKernelGGL(hxm::km_face_grad, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
                            eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
@@ -597,7 +584,9 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
             if (!eng->bucket_count[w]) continue;
             hipLaunchKernelGGL(hxm::km_grad, dim3(eng->bucket_count[w]), dim3(64 * w), sizeof(double) * NV * 64 * w,
                                eng->stream, eng->dElist + eng->bucket_first[w], eng->dQ, eng->dN3, eng->dOff,
-                               eng->dEfoff, eng->dJa, eng->dInvJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES);
+                               eng->dEfoff, eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES,
+                               eng->ph, lim);
+            if (lim.part) lim.part += 2 * (long)eng->bucket_count[w];
         }
     }
     if (ev) hipEventRecord(ev[1], eng->stream);
@@ -721,7 +710,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
             if (m != n) tset_host(n, m, oa.T[n][m]);
     }
     // geometry, weights, LES filter widths (SD.f90:420, :1378)
-    std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), jj((size_t)S, 1.0), edelta(K), fdelta(F),
+    std::vector<double> ja((size_t)9 * S, 0.0), jj((size_t)S, 1.0), edelta(K), fdelta(F),
         fgeo((size_t)10 * fsm, 0.0);
     std::vector<int> eofn((size_t)nodes);
     for (int e = 0; e < K; ++e) {
@@ -732,8 +721,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
                 for (int i = 0; i < nx; ++i) {
                     const long g = off[e] + i + nx * (j + ny * kk);
                     for (int c = 0; c < 9; ++c) ja[(size_t)c * S + g] = d->Ja[9 * g + c];
-                    ij[g] = 1.0 / d->J[g];
-                    jj[g] = d->J[g];
+                                jj[g] = d->J[g];
                     wq[g] = wn[nx - 1][i] * wn[ny - 1][j] * wn[nz - 1][kk];
                     vol += wq[g] * d->J[g];
                     eofn[g] = e;
@@ -780,7 +768,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
     int rc;
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dU, (size_t)15 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
-        (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
+        (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
         (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * fsm)) ||
         (rc = dalloc(eng, &eng->dFe, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dLift, (size_t)15 * ES)) ||
         (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) ||
@@ -798,7 +786,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!up(eng->dJa, ja.data(), ja.size() * 8) || !up(eng->dInvJ, ij.data(), ij.size() * 8) ||
+    if (!up(eng->dJa, ja.data(), ja.size() * 8) ||
         !up(eng->dJ, jj.data(), jj.size() * 8) || !up(eng->dFgeo, fgeo.data(), fgeo.size() * 8) ||
         !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
         !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dN3, N3.data(), N3.size() * 4) ||
@@ -914,13 +902,12 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     if (2 * nodes > 0x7fffffffL || ES > 0x7fffffffL)
         return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit node indices"));
     // geometry SoA, J, 1/J, quadrature weights, LES filter widths, face frames
-    std::vector<double> ja((size_t)9 * S, 0.0), ij((size_t)S, 1.0), jj((size_t)S, 1.0), wq((size_t)S, 0.0),
+    std::vector<double> ja((size_t)9 * S, 0.0), jj((size_t)S, 1.0), wq((size_t)S, 0.0),
         edelta(K), fdelta(F), fgeo((size_t)10 * FS, 0.0), egeo((size_t)4 * ES, 0.0);
     double xg[8], wg[8];
     gl_nodes(N, xg, wg);
     for (long g = 0; g < nodes; ++g) {
         for (int c = 0; c < 9; ++c) ja[(size_t)c * S + g] = d->Ja[9 * g + c];
-        ij[g] = 1.0 / d->J[g];
         jj[g] = d->J[g];
         const int n = (int)(g % ne);
         wq[g] = wg[n % n1] * wg[(n / n1) % n1] * wg[n / nf];
@@ -984,7 +971,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     int rc;
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dVgrad, (size_t)12 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
-        (rc = dalloc(eng, &eng->dInvJ, (size_t)S)) || (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
+        (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
         (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * FS)) ||
         (rc = dalloc(eng, &eng->dFe, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dEgeo, (size_t)4 * ES)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dEdelta, (size_t)K)) || (rc = dalloc(eng, &eng->dFdelta, (size_t)F)) ||
@@ -999,7 +986,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!up(eng->dJa, ja.data(), ja.size() * 8) || !up(eng->dInvJ, ij.data(), ij.size() * 8) ||
+    if (!up(eng->dJa, ja.data(), ja.size() * 8) ||
         !up(eng->dJ, jj.data(), jj.size() * 8) || !up(eng->dFgeo, fgeo.data(), fgeo.size() * 8) ||
         !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
         !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dNbr, nbr.data(), nbr.size() * 4) ||
@@ -1020,7 +1007,7 @@ int hx3d_engine_destroy(hx3d_engine *eng)
 {
     if (!eng) return HX3D_OK;
     if (eng->stream) hipStreamSynchronize(eng->stream);
-    void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dInvJ, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
+    void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff, eng->dElist,
                     eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart};
@@ -1120,22 +1107,36 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
         HIPCHK(hipEventCreate(&e));
         eng->evpool.push_back(e);
     }
-    // dt of step s: the minimum of the limits in slot s%2.  Slot 0 comes from a standalone
-    // MaxTimeStep of the current state; the last stage of step s-1 leaves block minima of the
-    // limits of the state it produces, which block 0 of step s's first face kernel reduces
-    // into the slot before any kernel of step s reads dt.
-    if (nsteps > 0) launch_maxdt_bits(eng, cfl, dcfl, eng->dBits);
-    int nparts = 0;
+    // dt of step s (slot 0): MaxTimeStep of the state the step starts from.  The first stage's
+    // gradient kernel leaves block minima of the limits (an Euler case has no gradient kernel:
+    // a standalone pass writes them), which block 0 of the first stage's face kernel reduces into
+    // the slot before any volume kernel of the step reads dt.
+    unsigned long long *slot = eng->dBits;
     for (int s = 0; s < nsteps; ++s) {
+        int nparts;
+        if (eng->ph.euler) {
+            const long nb = (eng->nodes + 255) / 256;
+            if (eng->mixed)
+                hipLaunchKernelGGL(hxm::km_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
+                                   eng->dEofn, eng->dN3, eng->dOpsAll, eng->nodes, eng->S, cfl, dcfl, eng->ph,
+                                   eng->dDtPart);
+            else
+                hipLaunchKernelGGL(k_maxdt, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dJa, eng->dJ,
+                                   eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dDtPart);
+            nparts = (int)nb;
+        } else {
+            nparts = eng->mixed ? eng->K : grad_grid(eng);
+        }
         for (int k = 0; k < 3; ++k) {
             DtArgs a{};
-            a.in = eng->dBits + 2 * (s % 2);
-            a.cfl = cfl; a.dcfl = dcfl; a.dcsi = eng->dcsi;
-            if (k == 0 && s > 0) a.red = hxk::DtReduce{eng->dDtPart, nparts, eng->dBits + 2 * (s % 2)};
-            if (k == 2) { a.part = eng->dDtPart; a.time = eng->dScal + 1; }
+            a.in = slot;
+            if (k == 0) {
+                if (!eng->ph.euler) a.lim = hxk::DtLimits{eng->dDtPart, cfl, dcfl, eng->dcsi};
+                a.red = hxk::DtReduce{eng->dDtPart, nparts, slot};
+            }
+            if (k == 2) a.time = eng->dScal + 1;
             stage(eng, RK_A[k], RK_C[k], 1, a, nullptr,
                   eng->timing ? &eng->evpool[(size_t)12 * s + 4 * k] : nullptr);
-            if (k == 2) nparts = eng->mixed ? eng->K : eng->vol_blocks;
         }
     }
     HIPCHK(hipGetLastError());

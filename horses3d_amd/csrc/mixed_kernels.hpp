@@ -199,8 +199,8 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
 __global__ void __launch_bounds__(512)
 km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *__restrict__ N3,
         const long *__restrict__ off, const int *__restrict__ efoff, const double *__restrict__ Ja,
-        const double *__restrict__ invJ, const double *__restrict__ lift, const OpsAll *__restrict__ opsp,
-        double *__restrict__ U, long S, long ES)
+        const double *__restrict__ Jac, const double *__restrict__ lift, const OpsAll *__restrict__ opsp,
+        double *__restrict__ U, long S, long ES, Phys ph, hxk::DtLimits dtl)
 {
     HIP_DYNAMIC_SHARED(double, smem)
     const int e = elist[blockIdx.x];
@@ -217,7 +217,8 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
     for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
 #pragma unroll
     for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
-    const double iJ = invJ[g];
+    const double jg = Jac[g];
+    const double iJ = 1.0 / jg;   // the same bits as a stored 1/J
     // lifting values of the node's faces (slot order 5/3, 0/1, 2/4); absent faces give 0
     const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
     const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
@@ -237,7 +238,8 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
         for (int v = 0; v < NV; ++v) sQ[v * ne + n] = q[v];
     }
     __syncthreads();
-    if (!active) return;
+    double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this node (stage 1)
+    if (active) {
     const double *Dx = ops.D[Nx], *Dy = ops.D[Ny], *Dz = ops.D[Nz];
     double uxi[NV], ueta[NV], uzeta[NV];
 #pragma unroll
@@ -273,6 +275,10 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
             U[(d * NV + v) * S + g] = (onx || ony || onz) ? gr[d] + fi * iJ : gr[d];
         }
     }
+    if (dtl.part)
+        hx::dt_limits(ph, q, ja, jg, ops.dcsi[Nx], ops.dcsi[Ny], ops.dcsi[Nz], dtl.cfl, dtl.dcfl, tc, tv);
+    }
+    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, n);
 }
 
 // ------------------------------------------------------------------------------------
@@ -535,7 +541,6 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
             }
         }
     }
-    double tc = INFINITY, tv = INFINITY;
     if (active) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -564,28 +569,8 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
                 q[v] = q[v] + cdt * gv;
                 Q[v * S + g] = q[v];
             }
-            if (dta.part) {
-                // MaxTimeStep of the new state (DGSEMClass.f90:870-1035)
-                const double dc = ops.dcsi[Nx], de = ops.dcsi[Ny], dz = ops.dcsi[Nz];
-                const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
-                const double p = hx::pressure(ph, q);
-                const double c = sqrt(ph.gamma * p / q[0]);
-                const double e0 = u + c, e1 = v + c, e2 = w + c;
-                const double lx = fabs(ja[0] * e0 + ja[1] * e1 + ja[2] * e2) * dc;
-                const double ly = fabs(ja[3] * e0 + ja[4] * e1 + ja[5] * e2) * de;
-                const double lz = fabs(ja[6] * e0 + ja[7] * e1 + ja[8] * e2) * dz;
-                tc = dta.cfl * fabs(jg) / (lx + ly + lz);
-                if (!ph.euler) {
-                    const double mu = hx::sutherland(ph, ph.gammaM2 * p / q[0]);
-                    const double vx = mu * (dc * dc) * fabs(ja[0] + ja[1] + ja[2]);
-                    const double vy = mu * (de * de) * fabs(ja[3] + ja[4] + ja[5]);
-                    const double vz = mu * (dz * dz) * fabs(ja[6] + ja[7] + ja[8]);
-                    tv = dta.dcfl * fabs(jg) / (vx + vy + vz);
-                }
-            }
         }
     }
-    if (update && dta.part) hxk::block_min_pair(tc, tv, dta.part, t);
 }
 
 // MaxTimeStep of the resident state for mixed orders: one thread per node, the element of each
@@ -605,22 +590,7 @@ km_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doub
         for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
 #pragma unroll
         for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
-        const double jg = Jac[g];
-        const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
-        const double p = hx::pressure(ph, q);
-        const double c = sqrt(ph.gamma * p / q[0]);
-        const double e0 = u + c, e1 = v + c, e2 = w + c;
-        const double lx = fabs(ja[0] * e0 + ja[1] * e1 + ja[2] * e2) * dc;
-        const double ly = fabs(ja[3] * e0 + ja[4] * e1 + ja[5] * e2) * de;
-        const double lz = fabs(ja[6] * e0 + ja[7] * e1 + ja[8] * e2) * dz;
-        tc = cfl * fabs(jg) / (lx + ly + lz);
-        if (!ph.euler) {
-            const double mu = hx::sutherland(ph, ph.gammaM2 * p / q[0]);
-            const double vx = mu * (dc * dc) * fabs(ja[0] + ja[1] + ja[2]);
-            const double vy = mu * (de * de) * fabs(ja[3] + ja[4] + ja[5]);
-            const double vz = mu * (dz * dz) * fabs(ja[6] + ja[7] + ja[8]);
-            tv = dcfl * fabs(jg) / (vx + vy + vz);
-        }
+        hx::dt_limits(ph, q, ja, Jac[g], dc, de, dz, cfl, dcfl, tc, tv);
     }
     hxk::block_min_pair(tc, tv, part, threadIdx.x);
 }

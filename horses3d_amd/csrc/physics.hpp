@@ -38,6 +38,31 @@ HX_HD void laminar_mu_kappa(const Phys &ph, const double q[NV], double &mu, doub
     kappa = mu * ph.mu_to_kappa;
 }
 
+// MaxTimeStep limits of one node (DGSEMClass.f90:870-1035 with ComputeEigenvaluesForState,
+// Physics_NS.f90:927-962): convective cfl |J| / sum_d |Ja_d . lambda| dxi_d^-1 and viscous
+// dcfl |J| / sum_d mu(T) dxi_d^-2 |sum Ja_d| -- mu = SutherlandsLaw(T) without the 1/Re factor,
+// as the reference has it (DGSEMClass.f90:990, 999-1002).
+HX_HD void dt_limits(const Phys &ph, const double q[NV], const double ja[9], double jg, double dc, double de,
+                     double dz, double cfl, double dcfl, double &tc, double &tv)
+{
+    const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
+    const double p = pressure(ph, q);
+    const double a = sqrt(ph.gamma * p / q[0]);
+    const double e0 = u + a, e1 = v + a, e2 = w + a;
+    const double lx = fabs(ja[0] * e0 + ja[1] * e1 + ja[2] * e2) * dc;
+    const double ly = fabs(ja[3] * e0 + ja[4] * e1 + ja[5] * e2) * de;
+    const double lz = fabs(ja[6] * e0 + ja[7] * e1 + ja[8] * e2) * dz;
+    tc = cfl * fabs(jg) / (lx + ly + lz);
+    tv = INFINITY;
+    if (!ph.euler) {
+        const double mu = sutherland(ph, ph.gammaM2 * p / q[0]);
+        const double vx = mu * (dc * dc) * fabs(ja[0] + ja[1] + ja[2]);
+        const double vy = mu * (de * de) * fabs(ja[3] + ja[4] + ja[5]);
+        const double vz = mu * (dz * dz) * fabs(ja[6] + ja[7] + ja[8]);
+        tv = dcfl * fabs(jg) / (vx + vy + vz);
+    }
+}
+
 // navierstokes/Physics_NS.f90:51-97 EulerFlux, then the contravariant projection of
 // BaseClass_ComputeInnerFluxes (discretization/HyperbolicDiscretizationClass.f90:83-152):
 // out[d][v] = F_x[v] Ja^d_x + F_y[v] Ja^d_y + F_z[v] Ja^d_z

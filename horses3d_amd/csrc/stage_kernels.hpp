@@ -46,6 +46,16 @@ __device__ __forceinline__ double operator_entry(const double *src, int t)
     return src[(x / n1) * 8 + (x % n1)];
 }
 
+__device__ __forceinline__ void block_min_pair(double a, double b, double *part, int t);
+
+// MaxTimeStep of the state a stage starts from, accumulated by the stage's gradient kernel
+// (part == null: not this stage): block minima of the convective and viscous limits to
+// part[2*block + 0/1], reduced by the face kernel that follows (DtReduce).
+struct DtLimits {
+    double *part;
+    double cfl, dcfl, dcsi;
+};
+
 // Averaging selectors of the templated split-form kernel (RiemannSolvers_NS.f90:229-285)
 enum { AVG_STANDARD = 0, AVG_DUCROS = 1, AVG_KG = 2, AVG_PIROZZOLI = 3, FORM_STANDARD = -1 };
 
@@ -71,9 +81,9 @@ __device__ __forceinline__ long efx(int e, int slot, int a, int b)
 template <int N, int EPB>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
 __attribute__((amdgpu_waves_per_eu(HX3D_GRAD_WAVES)))
-k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ invJ,
+k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ Jac,
        const int *__restrict__ nbr, const double *__restrict__ egeo, double *__restrict__ vgrad,
-       int K, long S, long ES, Ops ops, Phys ph)
+       int K, long S, long ES, Ops ops, Phys ph, DtLimits dtl)
 {
     constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
     __shared__ double sQ[EPB][NV][ne];
@@ -102,7 +112,8 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
     for (int v = 0; v < NV; ++v) q[v] = Q[v * S + gc];
 #pragma unroll
     for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + gc];
-    const double iJ = invJ[gc];
+    const double jg = Jac[gc];
+    const double iJ = 1.0 / jg;   // the same bits as a stored 1/J
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const double a = egeo[c * ES + fx], b = egeo[c * ES + fy], d = egeo[c * ES + fz];
@@ -125,7 +136,8 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
         for (int v = 0; v < NV; ++v) sQ[el][v][n] = q[v];
     }
     __syncthreads();
-    if (!active) return;
+    double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this node (stage 1)
+    if (active) {
     double uxi[NV], ueta[NV], uzeta[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { uxi[v] = 0.0; ueta[v] = 0.0; uzeta[v] = 0.0; }
@@ -190,6 +202,9 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
     hx::viscous_state(ph, q, gr[0], gr[1], gr[2], vg);
 #pragma unroll
     for (int c = 0; c < 12; ++c) vgrad[c * S + g] = vg[c];
+    if (dtl.part) hx::dt_limits(ph, q, ja, jg, dtl.dcsi, dtl.dcsi, dtl.dcsi, dtl.cfl, dtl.dcfl, tc, tv);
+    }
+    if (dtl.part) block_min_pair(tc, tv, dtl.part, t);
 }
 
 // ------------------------------------------------------------------------------------
@@ -281,11 +296,9 @@ k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const dou
 // ------------------------------------------------------------------------------------
 struct DtArgs {
     const unsigned long long *in;   // [2]: convective, viscous limit bit patterns of this step's dt
-    double *part;                   // [2 * blocks]: this block's minima of the next step's limits
-                                    // (or null); reduced by k_dt_reduce
     double *time;                   // t += dt by one thread (or null)
-    DtReduce red;                   // consumed by the stage's face kernel (not the volume kernel)
-    double cfl, dcfl, dcsi;         // MaxTimeStep parameters (DGSEMClass.f90:870-1035)
+    DtLimits lim;                   // consumed by the stage's gradient kernel
+    DtReduce red;                   // consumed by the stage's face kernel
 };
 
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -338,6 +351,9 @@ __device__ __forceinline__ void reduce_dt_partials(const double *part, int n, un
 #ifndef HX3D_VOLUME_UNROLL
 #define HX3D_VOLUME_UNROLL 1
 #endif
+#ifndef HX3D_VOLUME_PAIRS
+#define HX3D_VOLUME_PAIRS 0
+#endif
 #define HX_STR(x) #x
 #define HX_UNROLL(n) _Pragma(HX_STR(unroll n))
 
@@ -368,6 +384,11 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     __shared__ double sF[EPB][3][NF][ne];
     __shared__ double sN[SPLIT ? EPB : 1][SPLIT ? ND : 1][SPLIT ? ne : 1];
     __shared__ double sJ[SPLIT ? EPB : 1][SPLIT ? 3 : 1][SPLIT ? ne : 1];
+    // pair variant: the two-point fluxes of the current direction, one per unordered node pair
+    // of each line and the diagonal (the inviscid contravariant flux), slot(lo, hi) major
+    constexpr bool PAIRS = SPLIT && HX3D_VOLUME_PAIRS && N <= 6;   // LDS limit
+    constexpr int NSLOT = n1 * (n1 + 1) / 2;
+    __shared__ double sP[PAIRS ? EPB : 1][PAIRS ? NSLOT : 1][PAIRS ? NV : 1][PAIRS ? nf : 1];
     __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
     const int nb = (K + EPB - 1) / EPB;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -477,52 +498,95 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             dt0 = bits_to_double(dta.in[0]);
             dt1 = bits_to_double(dta.in[1]);
         }
-        if (!active) continue;
         const int own = d == 0 ? i : (d == 1 ? j : k);
         const int stride = d == 0 ? 1 : (d == 1 ? n1 : nf);
         const int base = n - own * stride;
-        if (!SPLIT) {
+        // the two-point flux of nodes own and l of the line, lower index first
+        auto pair_flux = [&](int l, double fs[NV]) {
+            const int m = base + l * stride;
+            double a[ND], b[ND], jb[3];
 #pragma unroll
-            for (int l = 0; l < n1; ++l) {
-                const double hh = sHat[own * n1 + l];
-                const int m = base + l * stride;
+            for (int c = 0; c < ND; ++c) { a[c] = sN[el][c][n]; b[c] = sN[el][c][m]; }
 #pragma unroll
-                for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[el][d][v][m];
+            for (int c = 0; c < 3; ++c) jb[c] = sJ[el][c][m];
+            if (SYM) {
+                hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
+            } else {
+                const bool lo = own < l;
+                double x1[ND], x2[ND], j1[3], j2[3];
+#pragma unroll
+                for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
+                hx::two_point<AVG>(x1, j1, x2, j2, fs);
             }
-        } else {
+        };
+        if (!SPLIT) {
+            if (active) {
+#pragma unroll
+                for (int l = 0; l < n1; ++l) {
+                    const double hh = sHat[own * n1 + l];
+                    const int m = base + l * stride;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[el][d][v][m];
+                }
+            }
+        } else if (!PAIRS) {
             // SplitDG_ComputeSplitFormFluxes (HyperbolicSplitForm.f90:64-116): the diagonal is
             // the inviscid contravariant flux, the other entries the two-point fluxes of
             // (lower, higher) node index
-            double fi[NV];
-            hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
-            HX_UNROLL(HX3D_VOLUME_UNROLL)
-            for (int l = 0; l < n1; ++l) {
-                const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
-                const int m = base + l * stride;
-                double a[ND], b[ND], jb[3], fs[NV];
+            if (active) {
+                double fi[NV];
+                hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+                HX_UNROLL(HX3D_VOLUME_UNROLL)
+                for (int l = 0; l < n1; ++l) {
+                    const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
+                    const int m = base + l * stride;
+                    double fs[NV];
+                    pair_flux(l, fs);
+                    qd[0] = qd[0] + sh * (l == own ? fi[0] : fs[0]);
 #pragma unroll
-                for (int c = 0; c < ND; ++c) { a[c] = sN[el][c][n]; b[c] = sN[el][c][m]; }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) jb[c] = sJ[el][c][m];
-                if (SYM) {
-                    hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
-                } else {
-                    const bool lo = own < l;
-                    double x1[ND], x2[ND], j1[3], j2[3];
-#pragma unroll
-                    for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
-                    hx::two_point<AVG>(x1, j1, x2, j2, fs);
+                    for (int v = 1; v < NV; ++v)
+                        qd[v] = qd[v] + sh * (l == own ? fi[v] : fs[v]) + hh * sF[el][d][v - V0][m];
                 }
-                qd[0] = qd[0] + sh * (l == own ? fi[0] : fs[0]);
+            }
+        } else {
+            // each slot (lo, hi), lo <= hi, of a line is filled once: node own writes the
+            // diagonal and the pairs (own, own + s mod N+1), s = 1..(N+1)/2 (s = (N+1)/2 only for
+            // own < (N+1)/2 when N+1 is even)
+            const int line = d == 0 ? j + n1 * k : (d == 1 ? i + n1 * k : i + n1 * j);
+            auto slot = [&](int lo, int hi) { return lo * n1 - (lo * (lo - 1)) / 2 + (hi - lo); };
+            if (active) {
+                double fi[NV];
+                hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
 #pragma unroll
-                for (int v = 1; v < NV; ++v)
-                    qd[v] = qd[v] + sh * (l == own ? fi[v] : fs[v]) + hh * sF[el][d][v - V0][m];
+                for (int v = 0; v < NV; ++v) sP[el][slot(own, own)][v][line] = fi[v];
+#pragma unroll
+                for (int s = 1; 2 * s <= n1; ++s) {
+                    if (2 * s == n1 && own >= s) continue;
+                    const int l = own + s < n1 ? own + s : own + s - n1;
+                    double fs[NV];
+                    pair_flux(l, fs);
+                    const int sl = own < l ? slot(own, l) : slot(l, own);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) sP[el][sl][v][line] = fs[v];
+                }
+            }
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int l = 0; l < n1; ++l) {
+                    const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
+                    const int m = base + l * stride;
+                    const int sl = own < l ? slot(own, l) : slot(l, own);
+                    qd[0] = qd[0] + sh * sP[el][sl][0][line];
+#pragma unroll
+                    for (int v = 1; v < NV; ++v)
+                        qd[v] = qd[v] + sh * sP[el][sl][v][line] + hh * sF[el][d][v - V0][m];
+                }
             }
         }
     }
-    double tc = INFINITY, tv = INFINITY;   // next step's time-step limits of this node
     if (active) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -553,29 +617,8 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
                 q[v] = q[v] + cdt * gv;
                 Q[v * S + g] = q[v];
             }
-            if (dta.part) {
-                // MaxTimeStep of the new state (DGSEMClass.f90:870-1035 with
-                // ComputeEigenvaluesForState, Physics_NS.f90:927-962)
-                const double u = fabs(q[1] / q[0]), v = fabs(q[2] / q[0]), w = fabs(q[3] / q[0]);
-                const double p = hx::pressure(ph, q);
-                const double c = sqrt(ph.gamma * p / q[0]);
-                const double e0 = u + c, e1 = v + c, e2 = w + c;
-                const double lx = fabs(ja[0] * e0 + ja[1] * e1 + ja[2] * e2) * dta.dcsi;
-                const double ly = fabs(ja[3] * e0 + ja[4] * e1 + ja[5] * e2) * dta.dcsi;
-                const double lz = fabs(ja[6] * e0 + ja[7] * e1 + ja[8] * e2) * dta.dcsi;
-                tc = dta.cfl * fabs(jg) / (lx + ly + lz);
-                if (!ph.euler) {
-                    const double mu = hx::sutherland(ph, ph.gammaM2 * p / q[0]);
-                    const double d2 = dta.dcsi * dta.dcsi;
-                    const double vx = mu * d2 * fabs(ja[0] + ja[1] + ja[2]);
-                    const double vy = mu * d2 * fabs(ja[3] + ja[4] + ja[5]);
-                    const double vz = mu * d2 * fabs(ja[6] + ja[7] + ja[8]);
-                    tv = dta.dcfl * fabs(jg) / (vx + vy + vz);
-                }
-            }
         }
     }
-    if (update && dta.part) block_min_pair(tc, tv, dta.part, t);
 }
 
 }  // namespace hxk

@@ -155,3 +155,26 @@ def test_mixed_orders_box_matches_oracle(inviscid, averaging, riemann, les):
     floor = 4.0 * np.finfo(float).eps * np.abs(Qo).max(axis=0)
     assert np.all(np.abs(eng.get_state() - Qo).max(axis=0) <= TOL * dq + floor)
     eng.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_euler_advance_matches_oracle(mixed):
+    """Euler equations (no gradient sweep; dt from a standalone MaxTimeStep pass each step):
+    three device-dt RK3 steps against the oracle, uniform and mixed orders."""
+    case = (mesh.periodic_box_mixed(3, mesh.random_orders(27, 2, 5, 3)) if mixed
+            else mesh.periodic_box(3, 3, amplitude=0.1, phase_seed=2))
+    Q0 = mesh.taylor_green(case, 0.3, noise=1e-3, seed=4)
+    kw = dict(riemann="roe", inviscid="split-form", averaging="pirozzoli")
+    from horses3d_amd.engine import Engine
+    eng = Engine(case, mach=0.3, reynolds=1600.0, euler=True, **kw)
+    ora = po.Oracle(case, po.make_params(0.3, 1600.0, euler=True, **kw))
+    eng.set_state(Q0)
+    t = eng.advance(3, 0.4, 0.4, t=0.0)
+    Q, G, to = Q0.copy(), np.zeros_like(Q0), 0.0
+    for _ in range(3):
+        dt = ora.max_dt(Q, 0.4, 0.4)
+        ora.rk3_step(Q, G, to, dt)
+        to += dt
+    assert t == pytest.approx(to, rel=1e-13)
+    assert po.normwise_error(eng.get_state() - Q0, Q - Q0).max() <= TOL
+    eng.close()
