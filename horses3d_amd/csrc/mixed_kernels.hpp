@@ -485,11 +485,14 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = 0.0;
     for (int d = 0; d < 3; ++d) {
+        double jh[3];   // this node's metric row times metric_scale
+#pragma unroll
+        for (int c = 0; c < 3; ++c) jh[c] = hx::metric_scale(AVG) * ja[3 * d + c];
         if (SPLIT) {
             if (d) __syncthreads();
             if (active) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) sJ[c * ne + n] = ja[3 * d + c];
+                for (int c = 0; c < 3; ++c) sJ[c * ne + n] = jh[c];
             }
         }
         if (SPLIT || d == 0) __syncthreads();
@@ -524,14 +527,14 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
 #pragma unroll
                 for (int c = 0; c < 3; ++c) jb[c] = sJ[c * ne + m];
                 if (SYM) {
-                    hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
+                    hx::two_point<AVG>(a, jh, b, jb, fs);
                 } else {
                     const bool lo = own < l;
                     double x1[ND], x2[ND], j1[3], j2[3];
 #pragma unroll
                     for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
+                    for (int c = 0; c < 3; ++c) { j1[c] = lo ? jh[c] : jb[c]; j2[c] = lo ? jb[c] : jh[c]; }
                     hx::two_point<AVG>(x1, j1, x2, j2, fs);
                 }
                 qd[0] = qd[0] + sh * (l == own ? fi[0] : fs[0]);

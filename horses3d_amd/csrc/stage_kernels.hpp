@@ -348,21 +348,23 @@ __device__ __forceinline__ void reduce_dt_partials(const double *part, int n, un
     }
 }
 
-#ifndef HX3D_VOLUME_UNROLL
-#define HX3D_VOLUME_UNROLL 1
-#endif
-#ifndef HX3D_VOLUME_PAIRS
-#define HX3D_VOLUME_PAIRS 0
-#endif
-#define HX_STR(x) #x
-#define HX_UNROLL(n) _Pragma(HX_STR(unroll n))
+// Offset of the node pair (lo, hi), lo < hi, of an (N+1)-node line in the pair table (pairs
+// lo-major).
+__host__ __device__ constexpr int pair_slot(int n1, int lo, int hi)
+{
+    return lo * n1 - (lo * (lo + 1)) / 2 + (hi - lo - 1);
+}
 
 // One thread per node, EPB elements per block.  The contravariant fluxes of all three
 // directions (split form: the viscous ones, whose mass entry is zero; standard form: inviscid
-// minus viscous) and the two-point node data go to LDS once; the split form also stages the
-// metric row of the current direction, the only per-direction rewrite.  Per direction each node
-// accumulates, in l order, qd = (qd + sharpD F#) + hatD Fv (DGIntegrals.f90:92-129) or
-// qd = qd + hatD F (:56-87).
+// minus viscous) go to LDS once.  Per direction each node accumulates, in l order,
+// qd = (qd + sharpD F#) + hatD Fv (DGIntegrals.f90:92-129) or qd = qd + hatD F (:56-87).
+// Split form, per direction: the node data and the metric rows (times metric_scale) go to LDS;
+// each unordered node pair of a line is evaluated once, lower node index first as the reference
+// evaluates both (i, l) and (l, i) (HyperbolicSplitForm.f90:64-116), by node lo of the pair
+// (lo, lo + s mod N+1), s = 1..(N+1)/2 (for even N+1, s = (N+1)/2 only for lo < (N+1)/2); the
+// pair fluxes then replace the node data in LDS and every node reads its N off-diagonal
+// entries from there.  The diagonal entry is the node's inviscid contravariant flux.
 // Minimum waves per SIMD asked of the register allocator: 3 where the kernel fits in 168 VGPRs
 // without spilling (standard form, Kennedy-Gruber, Pirozzoli), else no constraint.
 constexpr int volume_waves(int form) { return (form == -1 || form == 2 || form == 3) ? 3 : 1; }
@@ -381,15 +383,15 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     constexpr int ND = hx::node_data_size(AVG);
     constexpr bool SYM = hx::two_point_symmetric(AVG);
     constexpr int NF = SPLIT ? 4 : 5, V0 = SPLIT ? 1 : 0;
+    // split form: node data [ND][ne] and metric rows [3][ne], then the pair fluxes
+    // [pair][NV][line] of the current direction, in one space per element
+    constexpr int NPAIR = n1 * (n1 - 1) / 2, NP = n1 / 2;
+    constexpr int RN = (ND + 3) * ne, RP = NPAIR * NV * nf;
+    constexpr int RSZ = SPLIT ? (RN > RP ? RN : RP) : 1;
     __shared__ double sF[EPB][3][NF][ne];
-    __shared__ double sN[SPLIT ? EPB : 1][SPLIT ? ND : 1][SPLIT ? ne : 1];
-    __shared__ double sJ[SPLIT ? EPB : 1][SPLIT ? 3 : 1][SPLIT ? ne : 1];
-    // pair variant: the two-point fluxes of the current direction, one per unordered node pair
-    // of each line and the diagonal (the inviscid contravariant flux), slot(lo, hi) major
-    constexpr bool PAIRS = SPLIT && HX3D_VOLUME_PAIRS && N <= 6;   // LDS limit
-    constexpr int NSLOT = n1 * (n1 + 1) / 2;
-    __shared__ double sP[PAIRS ? EPB : 1][PAIRS ? NSLOT : 1][PAIRS ? NV : 1][PAIRS ? nf : 1];
+    __shared__ double sR[EPB][RSZ];
     __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
+    __shared__ int sSlot[n1 * n1];   // pair table offset of (own, l); 0 on the diagonal
     const int nb = (K + EPB - 1) / EPB;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int t = threadIdx.x;
@@ -428,6 +430,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     for (int c = 0; c < 12; ++c) vg[c] = vgrad[c * S + gc];
     const double dl = edelta[ec];
     const double jg = Jac[gc];
+    double own[ND] = {};   // this node's two-point data (split form)
     if (active) {
         double Fvc[3][NV];
         if (!ph.euler) {
@@ -458,10 +461,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int v = 1; v < NV; ++v) sF[el][d][v - V0][n] = Fvc[d][v];
-            double a[ND];
-            hx::node_data<AVG>(ph, q, a);
-#pragma unroll
-            for (int c = 0; c < ND; ++c) sN[el][c][n] = a[c];
+            hx::node_data<AVG>(ph, q, own);
         } else {
             double Fi[3][NV];
             hx::euler_contravariant(ph, q, ja, Fi);
@@ -476,7 +476,12 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     double xs[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) xs[v] = ((0.0 + sx[v]) + sy[v]) + sz[v];
-    if (t < n1 * n1) { sHat[t] = hop; sSharp[t] = sop; }
+    if (t < n1 * n1) {
+        sHat[t] = hop;
+        sSharp[t] = sop;
+        const int a = t / n1, b = t - a * n1;
+        sSlot[t] = a == b ? 0 : pair_slot(n1, a < b ? a : b, a < b ? b : a) * NV * nf;
+    }
     double qd[NV], gk[NV], dt0 = 0.0, dt1 = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = 0.0;
@@ -484,11 +489,20 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     // padding) only skip the work between them.
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+        const int own_i = d == 0 ? i : (d == 1 ? j : k);
+        const int stride = d == 0 ? 1 : (d == 1 ? n1 : nf);
+        const int base = n - own_i * stride;
+        const int line = d == 0 ? j + n1 * k : (d == 1 ? i + n1 * k : i + n1 * j);
+        double jh[3];   // this node's metric row times metric_scale
+#pragma unroll
+        for (int c = 0; c < 3; ++c) jh[c] = hx::metric_scale(AVG) * ja[3 * d + c];
         if (SPLIT) {
-            if (d) __syncthreads();     // previous direction's metric rows are read
+            if (d) __syncthreads();     // the previous direction's pair fluxes are read
             if (active) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) sJ[el][c][n] = ja[3 * d + c];
+                for (int c = 0; c < ND; ++c) sR[el][c * ne + n] = own[c];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sR[el][(ND + c) * ne + n] = jh[c];
             }
         }
         if (SPLIT || d == 0) __syncthreads();
@@ -498,92 +512,67 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             dt0 = bits_to_double(dta.in[0]);
             dt1 = bits_to_double(dta.in[1]);
         }
-        const int own = d == 0 ? i : (d == 1 ? j : k);
-        const int stride = d == 0 ? 1 : (d == 1 ? n1 : nf);
-        const int base = n - own * stride;
-        // the two-point flux of nodes own and l of the line, lower index first
-        auto pair_flux = [&](int l, double fs[NV]) {
-            const int m = base + l * stride;
-            double a[ND], b[ND], jb[3];
-#pragma unroll
-            for (int c = 0; c < ND; ++c) { a[c] = sN[el][c][n]; b[c] = sN[el][c][m]; }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) jb[c] = sJ[el][c][m];
-            if (SYM) {
-                hx::two_point<AVG>(a, &ja[3 * d], b, jb, fs);
-            } else {
-                const bool lo = own < l;
-                double x1[ND], x2[ND], j1[3], j2[3];
-#pragma unroll
-                for (int c = 0; c < ND; ++c) { x1[c] = lo ? a[c] : b[c]; x2[c] = lo ? b[c] : a[c]; }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) { j1[c] = lo ? ja[3 * d + c] : jb[c]; j2[c] = lo ? jb[c] : ja[3 * d + c]; }
-                hx::two_point<AVG>(x1, j1, x2, j2, fs);
-            }
-        };
         if (!SPLIT) {
             if (active) {
 #pragma unroll
                 for (int l = 0; l < n1; ++l) {
-                    const double hh = sHat[own * n1 + l];
+                    const double hh = sHat[own_i * n1 + l];
                     const int m = base + l * stride;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[el][d][v][m];
                 }
             }
-        } else if (!PAIRS) {
-            // SplitDG_ComputeSplitFormFluxes (HyperbolicSplitForm.f90:64-116): the diagonal is
-            // the inviscid contravariant flux, the other entries the two-point fluxes of
-            // (lower, higher) node index
-            if (active) {
-                double fi[NV];
-                hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
-                HX_UNROLL(HX3D_VOLUME_UNROLL)
-                for (int l = 0; l < n1; ++l) {
-                    const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
-                    const int m = base + l * stride;
-                    double fs[NV];
-                    pair_flux(l, fs);
-                    qd[0] = qd[0] + sh * (l == own ? fi[0] : fs[0]);
+            continue;
+        }
+        // this node's pairs (own, own + s mod N+1)
+        double pf[NP][NV];
+        if (active) {
 #pragma unroll
-                    for (int v = 1; v < NV; ++v)
-                        qd[v] = qd[v] + sh * (l == own ? fi[v] : fs[v]) + hh * sF[el][d][v - V0][m];
+            for (int s = 1; s <= NP; ++s) {
+                const int l = own_i + s < n1 ? own_i + s : own_i + s - n1;
+                const int m = base + l * stride;
+                double b[ND], jb[3];
+#pragma unroll
+                for (int c = 0; c < ND; ++c) b[c] = sR[el][c * ne + m];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) jb[c] = sR[el][(ND + c) * ne + m];
+                if (SYM) {
+                    hx::two_point<AVG>(own, jh, b, jb, pf[s - 1]);
+                } else {
+                    const bool lo = own_i < l;
+                    double x1[ND], x2[ND], j1[3], j2[3];
+#pragma unroll
+                    for (int c = 0; c < ND; ++c) { x1[c] = lo ? own[c] : b[c]; x2[c] = lo ? b[c] : own[c]; }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { j1[c] = lo ? jh[c] : jb[c]; j2[c] = lo ? jb[c] : jh[c]; }
+                    hx::two_point<AVG>(x1, j1, x2, j2, pf[s - 1]);
                 }
             }
-        } else {
-            // each slot (lo, hi), lo <= hi, of a line is filled once: node own writes the
-            // diagonal and the pairs (own, own + s mod N+1), s = 1..(N+1)/2 (s = (N+1)/2 only for
-            // own < (N+1)/2 when N+1 is even)
-            const int line = d == 0 ? j + n1 * k : (d == 1 ? i + n1 * k : i + n1 * j);
-            auto slot = [&](int lo, int hi) { return lo * n1 - (lo * (lo - 1)) / 2 + (hi - lo); };
-            if (active) {
-                double fi[NV];
-                hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+        }
+        __syncthreads();   // node data read: the space takes the pair fluxes
+        if (active) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) sP[el][slot(own, own)][v][line] = fi[v];
+            for (int s = 1; s <= NP; ++s) {
+                if (2 * s == n1 && own_i >= s) continue;
+                const int l = own_i + s < n1 ? own_i + s : own_i + s - n1;
+                const int po = sSlot[own_i * n1 + l] + line;
 #pragma unroll
-                for (int s = 1; 2 * s <= n1; ++s) {
-                    if (2 * s == n1 && own >= s) continue;
-                    const int l = own + s < n1 ? own + s : own + s - n1;
-                    double fs[NV];
-                    pair_flux(l, fs);
-                    const int sl = own < l ? slot(own, l) : slot(l, own);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) sP[el][sl][v][line] = fs[v];
-                }
+                for (int v = 0; v < NV; ++v) sR[el][po + v * nf] = pf[s - 1][v];
             }
-            __syncthreads();
-            if (active) {
+        }
+        __syncthreads();
+        if (active) {
+            double fi[NV];
+            hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+#pragma unroll 1
+            for (int l = 0; l < n1; ++l) {
+                const double sh = sSharp[own_i * n1 + l], hh = sHat[own_i * n1 + l];
+                const int m = base + l * stride;
+                const int po = sSlot[own_i * n1 + l] + line;
+                qd[0] = qd[0] + sh * (l == own_i ? fi[0] : sR[el][po]);
 #pragma unroll
-                for (int l = 0; l < n1; ++l) {
-                    const double sh = sSharp[own * n1 + l], hh = sHat[own * n1 + l];
-                    const int m = base + l * stride;
-                    const int sl = own < l ? slot(own, l) : slot(l, own);
-                    qd[0] = qd[0] + sh * sP[el][sl][0][line];
-#pragma unroll
-                    for (int v = 1; v < NV; ++v)
-                        qd[v] = qd[v] + sh * sP[el][sl][v][line] + hh * sF[el][d][v - V0][m];
-                }
+                for (int v = 1; v < NV; ++v)
+                    qd[v] = qd[v] + sh * (l == own_i ? fi[v] : sR[el][po + v * nf]) + hh * sF[el][d][v - V0][m];
             }
         }
     }
