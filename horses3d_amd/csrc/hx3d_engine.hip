@@ -374,6 +374,15 @@ int dalloc(hx3d_engine *eng, T **p, size_t n)
 }
 
 // elements per block for the element kernels
+// The stage kernels address every array plane through a 32-bit byte offset (hxk::ldg), and the
+// neighbour table stores 2 * node + side in an int: each plane must hold fewer than 2^29
+// elements.
+inline bool offsets_fit(long S, long ES, long FS)
+{
+    const long lim = 1L << 29;
+    return S < lim && ES < lim && FS < lim;
+}
+
 constexpr int epb_for(int N) { return N == 1 ? 16 : N == 2 ? 8 : N == 3 ? 4 : N == 4 ? 2 : 1; }
 
 // blocks of the uniform-order gradient kernel
@@ -661,7 +670,8 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
             efoff[6 * e + s + 1] = efoff[6 * e + s] + (N3[3 * e + SLOT_U[s]] + 1) * (N3[3 * e + SLOT_V[s]] + 1);
     eng->ES = (((long)efoff[6 * K] + 63) / 64) * 64;
     const long ES = eng->ES;
-    if (2 * nodes > 0x7fffffffL || ES > 0x7fffffffL) return eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit indices");
+    if (!offsets_fit(eng->S, ES, 0))
+        return eng->fail(HX3D_EUNSUPPORTED, "mesh too large: more than 2^29 - 1 nodes (or face nodes) per array");
     // faces
     std::vector<hxm::FaceRec> faces(F);
     long fsm = 0;
@@ -899,8 +909,8 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     }
     for (size_t x = 0; x < nbr.size(); ++x)
         if (nbr[x] < 0) return bail(eng->fail(HX3D_EINVAL, "an element face node has no neighbour"));
-    if (2 * nodes > 0x7fffffffL || ES > 0x7fffffffL)
-        return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large for 32-bit node indices"));
+    if (!offsets_fit(S, ES, FS))
+        return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large: more than 2^29 - 1 nodes (or face nodes) per array"));
     // geometry SoA, J, 1/J, quadrature weights, LES filter widths, face frames
     std::vector<double> ja((size_t)9 * S, 0.0), jj((size_t)S, 1.0), wq((size_t)S, 0.0),
         edelta(K), fdelta(F), fgeo((size_t)10 * FS, 0.0), egeo((size_t)4 * ES, 0.0);

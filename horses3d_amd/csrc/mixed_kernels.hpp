@@ -516,8 +516,9 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
                 for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[(d * NF + v) * ne + m];
             }
         } else {
-            double fi[NV];
-            hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+            double fi[NV], pu, pv, pw, pp;
+            hx::euler_primitives(ph, q, pu, pv, pw, pp);
+            hx::euler_contravariant_row(q, pu, pv, pw, pp, &ja[3 * d], fi);
             for (int l = 0; l <= Nd; ++l) {
                 const double sh = sharpD[own * 8 + l], hh = hatD[own * 8 + l];
                 const int m = base + l * stride;

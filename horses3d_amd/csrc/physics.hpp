@@ -416,11 +416,19 @@ HX_HD void two_point(const double *a, const double *ja, const double *b, const d
     for (int q = 0; q < NV; ++q) fs[q] = f[q] * J[0] + g[q] * J[1] + h[q] * J[2];
 }
 
-// One direction of euler_contravariant (the same operations for row d).
-HX_HD void euler_contravariant_row(const Phys &ph, const double q[NV], const double jd[3], double out[NV])
+// The velocity and pressure euler_contravariant forms (the same operations).
+HX_HD void euler_primitives(const Phys &ph, const double q[NV], double &u, double &v, double &w, double &p)
 {
-    const double u = q[1] / q[0], v = q[2] / q[0], w = q[3] / q[0];
-    const double p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
+    u = q[1] / q[0];
+    v = q[2] / q[0];
+    w = q[3] / q[0];
+    p = ph.gm1 * (q[4] - 0.5 * (q[1] * u + q[2] * v + q[3] * w));
+}
+
+// One direction of euler_contravariant from its primitives (the same operations for row d).
+HX_HD void euler_contravariant_row(const double q[NV], double u, double v, double w, double p, const double jd[3],
+                                   double out[NV])
+{
     const double F[NV][3] = {{q[1], q[2], q[3]},
                              {q[1] * u + p, q[1] * v, q[1] * w},
                              {q[1] * v, q[2] * v + p, q[2] * w},

@@ -36,6 +36,37 @@ __device__ __forceinline__ void hold(double &x)
 #endif
 }
 
+// Element i of an array through its (wave-uniform) base and a 32-bit offset: the access takes
+// the base from scalar registers and one 32-bit vector offset, with no 64-bit address
+// arithmetic per access.  Every array plane the kernels index holds fewer than 2^29 elements
+// (checked at engine creation), so the byte offset fits 32 bits.
+// The base is made opaque in scalar registers (global address space kept), so that the
+// compiler does not fold the plane offset into a per-lane 64-bit address.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HX_GLOBAL __attribute__((address_space(1)))
+#else
+#define HX_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ HX_GLOBAL T *sbase(T *x)
+{
+    auto g = (HX_GLOBAL T *)x;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(g));
+#endif
+    return g;
+}
+template <class T>
+__device__ __forceinline__ T ldg(const T *x, unsigned i)
+{
+    return *(const HX_GLOBAL T *)((const HX_GLOBAL char *)sbase(x) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stg(T *x, unsigned i, T v)
+{
+    *(HX_GLOBAL T *)((HX_GLOBAL char *)sbase(x) + i * (unsigned)sizeof(T)) = v;
+}
+
 // Entry t of an (N+1) x (N+1) operator (row stride 8 in Ops); threads t >= (N+1)^2 get 0.
 // Loaded before the node data and stored to LDS after it has been issued, so the wait for it
 // does not hold back the other loads.
@@ -63,9 +94,9 @@ enum { AVG_STANDARD = 0, AVG_DUCROS = 1, AVG_KG = 2, AVG_PIROZZOLI = 3, FORM_STA
 // face quantities the element kernels read are stored in this order, so an element reads its
 // own contiguous range without an index lookup.
 template <int N>
-__device__ __forceinline__ long efx(int e, int slot, int a, int b)
+__device__ __forceinline__ unsigned efx(int e, int slot, int a, int b)
 {
-    return ((long)e * 6 + slot) * ((N + 1) * (N + 1)) + a + (N + 1) * b;
+    return ((unsigned)e * 6 + slot) * ((N + 1) * (N + 1)) + a + (N + 1) * b;
 }
 
 // ------------------------------------------------------------------------------------
@@ -94,29 +125,29 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
     const int el = t / ne, n = t - el * ne;
     const int e = lb * EPB + el;
     const bool active = lb < nb && el < EPB && e < K;
-    const long g = (long)e * ne + n;
+    const unsigned g = (unsigned)e * ne + n;
     const int i = n % n1, j = (n / n1) % n1, k = n / nf;
     // Every load is unconditional (padding threads read a clamped element and discard it; see
     // k_volume).  The node's (at most three) faces: x-face slot 5 / 3, y-face 0 / 1, z-face
     // 2 / 4; faces the node is not on get zero traces and zero geometry, so their lifting term
     // is an exact zero and the accumulation below is branch-free.
     const int ec = e < K ? e : K - 1;
-    const long gc = (long)ec * ne + n;
+    const unsigned gc = (unsigned)ec * ne + n;
     const bool onx = i == 0 || i == N, ony = j == 0 || j == N, onz = k == 0 || k == N;
-    const long fx = efx<N>(ec, i == 0 ? 5 : 3, j, k), fy = efx<N>(ec, j == 0 ? 0 : 1, i, k),
-               fz = efx<N>(ec, k == 0 ? 2 : 4, i, j);
+    const unsigned fx = efx<N>(ec, i == 0 ? 5 : 3, j, k), fy = efx<N>(ec, j == 0 ? 0 : 1, i, k),
+                   fz = efx<N>(ec, k == 0 ? 2 : 4, i, j);
     const double dop = operator_entry<n1>(ops.D, t);
-    const int cx = nbr[fx], cy = nbr[fy], cz = nbr[fz];
+    const int cx = ldg(nbr, fx), cy = ldg(nbr, fy), cz = ldg(nbr, fz);
     double q[NV], ja[9], gx[4], gy[4], gz[4];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) q[v] = Q[v * S + gc];
+    for (int v = 0; v < NV; ++v) q[v] = ldg(Q + v * S, gc);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + gc];
-    const double jg = Jac[gc];
+    for (int c = 0; c < 9; ++c) ja[c] = ldg(Ja + c * S, gc);
+    const double jg = ldg(Jac, gc);
     const double iJ = 1.0 / jg;   // the same bits as a stored 1/J
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const double a = egeo[c * ES + fx], b = egeo[c * ES + fy], d = egeo[c * ES + fz];
+        const double a = ldg(egeo + c * ES, fx), b = ldg(egeo + c * ES, fy), d = ldg(egeo + c * ES, fz);
         gx[c] = onx ? a : 0.0;
         gy[c] = ony ? b : 0.0;
         gz[c] = onz ? d : 0.0;
@@ -125,7 +156,8 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
     double qx[NV], qy[NV], qz[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        const double a = Q[v * S + (cx >> 1)], b = Q[v * S + (cy >> 1)], d = Q[v * S + (cz >> 1)];
+        const double a = ldg(Q + v * S, (unsigned)cx >> 1), b = ldg(Q + v * S, (unsigned)cy >> 1),
+                     d = ldg(Q + v * S, (unsigned)cz >> 1);
         qx[v] = onx ? a : 0.0;
         qy[v] = ony ? b : 0.0;
         qz[v] = onz ? d : 0.0;
@@ -201,7 +233,7 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
     double vg[12];
     hx::viscous_state(ph, q, gr[0], gr[1], gr[2], vg);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) vgrad[c * S + g] = vg[c];
+    for (int c = 0; c < 12; ++c) stg(vgrad + c * S, g, vg[c]);
     if (dtl.part) hx::dt_limits(ph, q, ja, jg, dtl.dcsi, dtl.dcsi, dtl.dcsi, dtl.cfl, dtl.dcfl, tc, tv);
     }
     if (dtl.part) block_min_pair(tc, tv, dtl.part, t);
@@ -223,31 +255,36 @@ struct DtReduce {
     unsigned long long *bits;
 };
 
+#ifndef HX3D_FACE_WAVES
+#define HX3D_FACE_WAVES 1
+#endif
 template <int N>
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(HX3D_FACE_WAVES)))
 k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ fgeo,
        const int4 *__restrict__ fnode, const double *__restrict__ fdelta, double *__restrict__ fe,
        long FS, long S, long ES, double b0, Phys ph, DtReduce dtr)
 {
     constexpr int n1 = N + 1, nf = n1 * n1;
     if (dtr.bits && blockIdx.x == 0) reduce_dt_partials(dtr.part, dtr.n, dtr.bits, threadIdx.x);
-    const long fq = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const unsigned fq = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (fq >= FS) return;
-    const int4 gg = fnode[fq];
+    const int4 gi = ldg(fnode, fq);
+    const unsigned gx = gi.x, gy = gi.y, gz = gi.z, gw = gi.w;
     // face frame: fgeo[c*FS + fq], c = n (0..2), t1 (3..5), t2 (6..8), |Ja| (9)
     double nrm[3], t1[3], t2[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) nrm[c] = fgeo[c * FS + fq];
-    const double jac = fgeo[9 * FS + fq];
+    for (int c = 0; c < 3; ++c) nrm[c] = ldg(fgeo + c * FS, fq);
+    const double jac = ldg(fgeo + 9 * FS, fq);
     // all loads unconditional (see k_volume)
     double qL[NV], qR[NV], vgL[12], vgR[12];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) { qL[v] = Q[v * S + gg.x]; qR[v] = Q[v * S + gg.y]; }
+    for (int v = 0; v < NV; ++v) { qL[v] = ldg(Q + v * S, gx); qR[v] = ldg(Q + v * S, gy); }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { t1[c] = fgeo[(3 + c) * FS + fq]; t2[c] = fgeo[(6 + c) * FS + fq]; }
+    for (int c = 0; c < 3; ++c) { t1[c] = ldg(fgeo + (3 + c) * FS, fq); t2[c] = ldg(fgeo + (6 + c) * FS, fq); }
 #pragma unroll
-    for (int c = 0; c < 12; ++c) { vgL[c] = vgrad[c * S + gg.x]; vgR[c] = vgrad[c * S + gg.y]; }
-    const double fd = fdelta[fq / nf];
+    for (int c = 0; c < 12; ++c) { vgL[c] = ldg(vgrad + c * S, gx); vgR[c] = ldg(vgrad + c * S, gy); }
+    const double fd = ldg(fdelta, fq / nf);
     double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (!ph.euler) {
         // BR1_RiemannSolver (EllipticBR1.f90:816-868) with the face viscosities of
@@ -284,8 +321,8 @@ k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const dou
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         const double fb = ((inv[v] - visc[v]) * jac) * b0;
-        fe[v * ES + gg.z] = fb;
-        fe[v * ES + gg.w] = -fb;
+        stg(fe + v * ES, gz, fb);
+        stg(fe + v * ES, gw, -fb);
     }
 }
 
@@ -398,7 +435,6 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     const int el = t / ne, n = t - el * ne;
     const int e = lb * EPB + el;
     const bool active = lb < nb && el < EPB && e < K;
-    const long g = (long)e * ne + n;
     const int i = n % n1, j = (n / n1) % n1, k = n / nf;
     const bool onx = i == 0 || i == N, ony = j == 0 || j == N, onz = k == 0 || k == N;
     // ---- Loads are unconditional (padding threads read a clamped element and discard it): a
@@ -406,16 +442,16 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     // of any earlier load.  The face values are issued first and summed once the node's own
     // data has arrived (loads complete in order); G and dt are issued at the last direction
     // and consumed after it.
-    const long gc = (long)(e < K ? e : K - 1) * ne + n;
-    const int ec = (int)(gc / ne);
+    const int ec = e < K ? e : K - 1;
+    const unsigned gc = (unsigned)ec * ne + n, gw = (unsigned)e * ne + n;
     const double hop = operator_entry<n1>(ops.hatD, t), sop = operator_entry<n1>(ops.sharpD, t);
     double sx[NV], sy[NV], sz[NV];
     {
-        const long xx = efx<N>(ec, i == 0 ? 5 : 3, j, k), xy = efx<N>(ec, j == 0 ? 0 : 1, i, k),
-                   xz = efx<N>(ec, k == 0 ? 2 : 4, i, j);
+        const unsigned xx = efx<N>(ec, i == 0 ? 5 : 3, j, k), xy = efx<N>(ec, j == 0 ? 0 : 1, i, k),
+                       xz = efx<N>(ec, k == 0 ? 2 : 4, i, j);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const double fx = fe[v * ES + xx], fy = fe[v * ES + xy], fz = fe[v * ES + xz];
+            const double fx = ldg(fe + v * ES, xx), fy = ldg(fe + v * ES, xy), fz = ldg(fe + v * ES, xz);
             sx[v] = onx ? fx : 0.0;   // faces the node is not on add an exact zero
             sy[v] = ony ? fy : 0.0;
             sz[v] = onz ? fz : 0.0;
@@ -423,13 +459,13 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     }
     double q[NV], ja[9], vg[12];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) q[v] = Q[v * S + gc];
+    for (int v = 0; v < NV; ++v) q[v] = ldg(Q + v * S, gc);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + gc];
+    for (int c = 0; c < 9; ++c) ja[c] = ldg(Ja + c * S, gc);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) vg[c] = vgrad[c * S + gc];
-    const double dl = edelta[ec];
-    const double jg = Jac[gc];
+    for (int c = 0; c < 12; ++c) vg[c] = ldg(vgrad + c * S, gc);
+    const double dl = ldg(edelta, (unsigned)ec);
+    const double jg = ldg(Jac, gc);
     double own[ND] = {};   // this node's two-point data (split form)
     if (active) {
         double Fvc[3][NV];
@@ -485,6 +521,8 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     double qd[NV], gk[NV], dt0 = 0.0, dt1 = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = 0.0;
+    double pu, pv, pw, pp;   // the diagonal's primitives, formed once for the three directions
+    hx::euler_primitives(ph, q, pu, pv, pw, pp);
     // xi, then eta, then zeta.  Every thread reaches every barrier; inactive threads (block
     // padding) only skip the work between them.
 #pragma unroll
@@ -508,7 +546,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
         if (SPLIT || d == 0) __syncthreads();
         if (d == 2) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) gk[v] = G[v * S + gc];
+            for (int v = 0; v < NV; ++v) gk[v] = ldg(G + v * S, gc);
             dt0 = bits_to_double(dta.in[0]);
             dt1 = bits_to_double(dta.in[1]);
         }
@@ -563,7 +601,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
         __syncthreads();
         if (active) {
             double fi[NV];
-            hx::euler_contravariant_row(ph, q, &ja[3 * d], fi);
+            hx::euler_contravariant_row(q, pu, pv, pw, pp, &ja[3 * d], fi);
 #pragma unroll 1
             for (int l = 0; l < n1; ++l) {
                 const double sh = sSharp[own_i * n1 + l], hh = sHat[own_i * n1 + l];
@@ -590,7 +628,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
         for (int v = 0; v < NV; ++v) qd[v] = qd[v] * ijg;
         if (qdot_out) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) qdot_out[v * S + g] = qd[v];
+            for (int v = 0; v < NV; ++v) stg(qdot_out + v * S, gw, qd[v]);
         }
         if (update) {
             hold(dt0);
@@ -602,9 +640,9 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             for (int v = 0; v < NV; ++v) {
                 hold(gk[v]);
                 const double gv = a_k * gk[v] + qd[v];
-                G[v * S + g] = gv;
+                stg(G + v * S, gw, gv);
                 q[v] = q[v] + cdt * gv;
-                Q[v * S + g] = q[v];
+                stg(Q + v * S, gw, q[v]);
             }
         }
     }
