@@ -167,18 +167,26 @@ def main():
     for k in kern.values():
         k["GBs"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
     dom = max(kern, key=lambda k: kern[k]["ms"])
-    traffic = None
+    traffic, fp64 = None, None
     tpath = os.path.join(REPO, "profiles", "traffic_tgv16_n4.json")
     if os.path.exists(tpath) and (args.n, args.order) == (16, 4):
         tj = json.load(open(tpath))
-        traffic = tj.get("k_volume_bytes_per_launch") if dom == "volume" else tj.get("k_%s_bytes_per_launch" % dom)
+        traffic = tj.get("k_%s_bytes_per_launch" % dom)
+        # FP64 vector FLOP per launch from the committed SQ counter run of the same workload
+        # (64 x ADD/MUL/TRANS + 128 x FMA wave instructions), over this run's launch time
+        flop = tj.get("k_%s_fp64_flop_per_launch" % dom)
+        if flop:
+            tf = flop / (kern[dom]["ms"] * 1e-3) / 1e12
+            fp64 = dict(flop_per_launch=flop, achieved=round(tf, 2), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                        frac=round(tf / FP64_PEAK_TFLOPS, 4))
     roof = dict(bound="hbm", kernel="k_" + dom, achieved=round(kern[dom]["GBs"], 1), peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=round(kern[dom]["GBs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                 bytes_per_launch=kern[dom]["bytes"], avg_launch_ms=round(kern[dom]["ms"], 5),
                 stage_ms=round(kt[3], 5),
                 stage_achieved_GBs=round(B["stage"] * nd / (kt[3] * 1e-3) / 1e9, 1),
                 stage_frac=round(B["stage"] * nd / (kt[3] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                kernels={k: dict(avg_launch_ms=round(v["ms"], 5), GBs=round(v["GBs"], 1)) for k, v in kern.items()})
+                kernels={k: dict(avg_launch_ms=round(v["ms"], 5), GBs=round(v["GBs"], 1)) for k, v in kern.items()},
+                fp64_vector=fp64)
     eng.close()
 
     cpu = None
