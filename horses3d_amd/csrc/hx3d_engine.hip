@@ -37,11 +37,18 @@
 #include "physics.hpp"
 #include "stage_kernels.hpp"
 #include "mixed_kernels.hpp"
+#include "spectral.hpp"
 
 using hx::NV;
 using hx::Phys;
 using hxk::DtArgs;
 using hxk::Ops;
+using hxs::almost_equal;
+using hxs::bary;
+using hxs::gl_nodes;
+using hxs::interp_matrix;
+using hxs::left_to_right;
+using hxs::tset_host;
 
 namespace {
 
@@ -240,43 +247,6 @@ __global__ void k_fill(double *__restrict__ X, long n, double val)
 // ------------------------------------------------------------------------------------
 // host side: spectral operators (restated from NodalStorageClass.f90 for the engine)
 // ------------------------------------------------------------------------------------
-void gl_nodes(int N, double *x, double *w)
-{
-    // LegendreLobattoNodesAndWeights (LegendreAlgorithms.f90:275-355)
-    const double tol = 4.0 * 2.220446049250313e-16;
-    auto qL = [&](double xx, double &q, double &dq, double &LN) {
-        double a2 = 1.0, da2 = 0.0, a1 = xx, da1 = 1.0, Lk = 0, dLk = 0;
-        for (int k = 2; k <= N; ++k) {
-            Lk = ((2 * k - 1) * xx * a1 - (k - 1) * a2) / k;
-            dLk = da2 + (2 * k - 1) * a1;
-            a2 = a1; a1 = Lk; da2 = da1; da1 = dLk;
-        }
-        int k = N + 1;
-        Lk = ((2 * k - 1) * xx * a1 - (k - 1) * a2) / k;
-        dLk = da2 + (2 * k - 1) * a1;
-        q = Lk - a2; dq = dLk - da2; LN = a1;
-    };
-    if (N == 1) { x[0] = -1; x[1] = 1; w[0] = w[1] = 1; return; }
-    x[0] = -1.0; x[N] = 1.0; w[0] = w[N] = 2.0 / (N * (N + 1));
-    for (int j = 1; j <= (N + 1) / 2 - 1; ++j) {
-        double xj = -std::cos((j + 0.25) * M_PI / N - 3.0 / (8 * N * M_PI * (j + 0.25)));
-        double q, dq, LN;
-        for (int k = 0; k <= 10; ++k) {
-            qL(xj, q, dq, LN);
-            double d = -q / dq;
-            xj += d;
-            if (std::fabs(d) <= tol * std::fabs(xj)) break;
-        }
-        qL(xj, q, dq, LN);
-        x[j] = xj; w[j] = 2.0 / (N * (N + 1) * LN * LN);
-        x[N - j] = -xj; w[N - j] = w[j];
-    }
-    if (N % 2 == 0) {
-        double a2 = 1.0, a1 = 0.0, L = (N == 0) ? 1.0 : 0.0;
-        for (int k = 2; k <= N; ++k) { L = ((2 * k - 1) * 0.0 * a1 - (k - 1) * a2) / k; a2 = a1; a1 = L; }
-        x[N / 2] = 0.0; w[N / 2] = 2.0 / (N * (N + 1) * L * L);
-    }
-}
 
 Ops make_ops(int N)
 {
@@ -306,20 +276,6 @@ Ops make_ops(int N)
     return o;
 }
 
-void left_to_right(int i, int j, int Nx, int Ny, int rot, int &ii, int &jj)
-{
-    // leftIndexes2Right (MeshTypes.f90:70-108)
-    switch (rot) {
-    case 0: ii = i; jj = j; break;
-    case 1: ii = Ny - j; jj = i; break;
-    case 2: ii = Nx - i; jj = Ny - j; break;
-    case 3: ii = j; jj = Nx - i; break;
-    case 4: ii = j; jj = i; break;
-    case 5: ii = Nx - i; jj = j; break;
-    case 6: ii = Ny - j; jj = Nx - i; break;
-    default: ii = i; jj = Ny - j; break;
-    }
-}
 
 #define HIPCHK(x)                                                                        \
     do {                                                                                 \
@@ -504,64 +460,6 @@ void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 // ------------------------------------------------------------------------------------
 // mixed orders
 // ------------------------------------------------------------------------------------
-bool almost_equal(double a, double b)
-{
-    // AlmostEqual (Utilities.f90:110-139), tolerance 2 epsilon
-    const double tol = 2.0 * 2.220446049250313e-16;
-    if (a == 0.0 || b == 0.0) return std::fabs(a - b) <= tol;
-    return std::fabs(b - a) <= tol * std::max(std::fabs(a), std::fabs(b));
-}
-
-void bary(int N, const double *x, double *wb)
-{
-    // BarycentricWeights (InterpolationAndDerivatives.f90:230-262)
-    for (int j = 0; j <= N; ++j) wb[j] = 1.0;
-    for (int j = 1; j <= N; ++j)
-        for (int k = 0; k < j; ++k) { wb[k] *= (x[k] - x[j]); wb[j] *= (x[j] - x[k]); }
-    for (int j = 0; j <= N; ++j) wb[j] = 1.0 / wb[j];
-}
-
-// PolynomialInterpolationMatrix (InterpolationAndDerivatives.f90:413-461): T (M+1)x(N+1)
-void interp_matrix(int N, int M, const double *xo, const double *wb, const double *xn, double *T)
-{
-    for (int k = 0; k <= M; ++k) {
-        bool match = false;
-        for (int j = 0; j <= N; ++j) {
-            T[k * (N + 1) + j] = 0.0;
-            if (almost_equal(xn[k], xo[j])) { match = true; T[k * (N + 1) + j] = 1.0; }
-        }
-        if (!match) {
-            double s = 0.0;
-            for (int j = 0; j <= N; ++j) {
-                const double t = wb[j] / (xn[k] - xo[j]);
-                T[k * (N + 1) + j] = t;
-                s = s + t;
-            }
-            for (int j = 0; j <= N; ++j) T[k * (N + 1) + j] = T[k * (N + 1) + j] / s;
-        }
-    }
-}
-
-// Tset(N -> M) (InterpolationMatrix_Construct, InterpolationMatrices.f90:42-108): Lagrange
-// interpolation when N < M, otherwise the transpose of the M -> N interpolation weighted by
-// w_N(j) / w_M(i) (the L2 projection)
-void tset_host(int N, int M, double *T)
-{
-    double xo[8], wo[8], xd[8], wd[8], wb[8];
-    gl_nodes(N, xo, wo);
-    gl_nodes(M, xd, wd);
-    if (N < M) {
-        bary(N, xo, wb);
-        interp_matrix(N, M, xo, wb, xd, T);
-    } else {
-        double Tin[64];
-        bary(M, xd, wb);
-        interp_matrix(M, N, xd, wb, xo, Tin);
-        for (int j = 0; j <= N; ++j)
-            for (int i = 0; i <= M; ++i) T[i * (N + 1) + j] = Tin[j * (M + 1) + i] * wo[j] / wd[i];
-    }
-}
-
 template <int FORM>
 void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {

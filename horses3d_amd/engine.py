@@ -46,7 +46,8 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d
            "hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device",
            "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_max_dt", "hx3d_advance", "hx3d_residuals",
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_set_timing",
-           "hx3d_kernel_times"]
+           "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
+           "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy"]
 
 
 def load_library(path=LIB_PATH):
@@ -76,12 +77,69 @@ def load_library(path=LIB_PATH):
     L.hx3d_synchronize.argtypes = [P]
     L.hx3d_set_timing.argtypes = [P, I]
     L.hx3d_kernel_times.argtypes = [P, P]
+    L.hx3d_mesh_read_specmesh.argtypes = [C.c_char_p, I, P, C.c_char_p, C.POINTER(C.c_void_p)]
+    L.hx3d_mesh_error.argtypes = [P]
+    L.hx3d_mesh_error.restype = C.c_char_p
+    L.hx3d_mesh_num_nodes.argtypes = [P]
+    L.hx3d_mesh_num_nodes.restype = C.c_long
+    L.hx3d_mesh_num_face_nodes.argtypes = [P]
+    L.hx3d_mesh_num_face_nodes.restype = C.c_long
+    L.hx3d_mesh_coords.argtypes = [P]
+    L.hx3d_mesh_coords.restype = C.POINTER(C.c_double)
+    L.hx3d_mesh_fill_desc.argtypes = [P, C.POINTER(CaseDesc)]
+    L.hx3d_mesh_destroy.argtypes = [P]
     _lib = L
     return L
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def read_specmesh(path, N=None, orders=None, periodic=None):
+    """A SpecMesh file as a case dict, through the C front end (hx3d_mesh_read_specmesh).
+
+    N: one order for every element, or orders: (K, 3) per-element orders.  periodic: the
+    periodic boundaries of the .control file as {name: coupled name} (or pairs).  Runs on the
+    host only."""
+    L = load_library()
+    if orders is not None:
+        ords = np.ascontiguousarray(orders, dtype=np.int32)
+        op = _p(ords)
+    else:
+        ords, op = None, None
+    if periodic is None:
+        spec = None
+    else:
+        items = periodic.items() if isinstance(periodic, dict) else periodic
+        spec = ";".join("%s=%s" % (a, b) for a, b in items).encode()
+    h = C.c_void_p()
+    rc = L.hx3d_mesh_read_specmesh(os.fsencode(path), int(N or 0), op, spec, C.byref(h))
+    try:
+        if rc != 0:
+            msg = L.hx3d_mesh_error(h).decode() if h else "out of memory"
+            raise HX3DError("hx3d_mesh_read_specmesh: %s (code %d)" % (msg, rc))
+        d = CaseDesc()
+        L.hx3d_mesh_fill_desc(h, C.byref(d))
+        nodes, fnodes, K, F = L.hx3d_mesh_num_nodes(h), L.hx3d_mesh_num_face_nodes(h), d.K, d.F
+
+        def arr(ptr, n, ty, shape):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ty)), shape=(n,)).copy().reshape(shape)
+
+        Nk = arr(d.N, 3 * K, C.c_int32, (K, 3))
+        case = dict(K=K, F=F, N=Nk, faceID=arr(d.faceID, 6 * K, C.c_int32, (K, 6)),
+                    faceSide=arr(d.faceSide, 6 * K, C.c_int32, (K, 6)),
+                    fElem=arr(d.fElem, 2 * F, C.c_int32, (F, 2)), fESide=arr(d.fESide, 2 * F, C.c_int32, (F, 2)),
+                    fRot=arr(d.fRot, F, C.c_int32, (F,)), Ja=arr(d.Ja, 9 * nodes, C.c_double, (nodes, 9)),
+                    J=arr(d.J, nodes, C.c_double, (nodes,)), normal=arr(d.fNormal, 3 * fnodes, C.c_double, (fnodes, 3)),
+                    t1=arr(d.fT1, 3 * fnodes, C.c_double, (fnodes, 3)), t2=arr(d.fT2, 3 * fnodes, C.c_double, (fnodes, 3)),
+                    fjac=arr(d.fJac, fnodes, C.c_double, (fnodes,)),
+                    x=arr(L.hx3d_mesh_coords(h), 3 * nodes, C.c_double, (nodes, 3)))
+        ne = np.prod(Nk + 1, axis=1)
+        case["off"] = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+        return case
+    finally:
+        L.hx3d_mesh_destroy(h)
 
 
 class Engine:

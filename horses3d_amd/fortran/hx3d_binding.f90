@@ -137,6 +137,41 @@ module hx3d_binding
          type(c_ptr), value          :: eng
          real(c_double), intent(out) :: out_ms(4)
       end function
+      ! mesh front end: path and periodic pairs as NUL-terminated strings
+      integer(c_int) function hx3d_mesh_read_specmesh(path, N, orders, periodic, mesh) &
+            bind(C, name="hx3d_mesh_read_specmesh")
+         import :: c_int, c_ptr, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value              :: N
+         type(c_ptr), value                 :: orders
+         character(kind=c_char), intent(in) :: periodic(*)
+         type(c_ptr), intent(out)           :: mesh
+      end function
+      type(c_ptr) function hx3d_mesh_error(mesh) bind(C, name="hx3d_mesh_error")
+         import :: c_ptr
+         type(c_ptr), value :: mesh
+      end function
+      integer(c_long) function hx3d_mesh_num_nodes(mesh) bind(C, name="hx3d_mesh_num_nodes")
+         import :: c_long, c_ptr
+         type(c_ptr), value :: mesh
+      end function
+      integer(c_long) function hx3d_mesh_num_face_nodes(mesh) bind(C, name="hx3d_mesh_num_face_nodes")
+         import :: c_long, c_ptr
+         type(c_ptr), value :: mesh
+      end function
+      type(c_ptr) function hx3d_mesh_coords(mesh) bind(C, name="hx3d_mesh_coords")
+         import :: c_ptr
+         type(c_ptr), value :: mesh
+      end function
+      integer(c_int) function hx3d_mesh_fill_desc(mesh, desc) bind(C, name="hx3d_mesh_fill_desc")
+         import :: c_int, c_ptr, hx3d_case_desc
+         type(c_ptr), value                  :: mesh
+         type(hx3d_case_desc), intent(inout) :: desc
+      end function
+      integer(c_int) function hx3d_mesh_destroy(mesh) bind(C, name="hx3d_mesh_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: mesh
+      end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr
          type(c_ptr), value :: s

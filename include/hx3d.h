@@ -89,8 +89,9 @@ typedef struct {
 typedef struct hx3d_engine hx3d_engine;
 
 /* Creates an engine on the current HIP device: uploads geometry, builds the
- * element/face gather tables.  Supported: Gauss-Lobatto nodes, uniform order
- * 1 <= N <= 7 in all elements and directions, conforming faces. */
+ * element/face gather tables.  Supported: Gauss-Lobatto nodes, orders 1 <= N <= 7 (one
+ * order everywhere, or per-element (Nx, Ny, Nz) with mortar faces, whose face frames must
+ * then be given at the face orders), every face interior or periodic. */
 int hx3d_engine_create(const hx3d_case_desc *desc, hx3d_engine **out);
 int hx3d_engine_destroy(hx3d_engine *eng);
 const char *hx3d_last_error(const hx3d_engine *eng);
@@ -127,6 +128,29 @@ int hx3d_synchronize(hx3d_engine *eng);
  * out[3] = their sum (one stage). */
 int hx3d_set_timing(hx3d_engine *eng, int on);
 int hx3d_kernel_times(hx3d_engine *eng, double out_ms[4]);
+
+/* ---- Mesh front end (host only, no GPU): a SpecMesh file read into the contract above with
+ * the reference's topology and geometry.
+ *   ConstructMesh_FromSpecMeshFile_ (Read_SpecMesh.f90:36-348) with ConstructFaces,
+ *   ConstructPeriodicFaces, DeletePeriodicMinusFaces (HexMesh.f90:356-944),
+ *   HexMesh_ConstructGeometry (HexMesh.f90:2677-3059), ConstructMappedGeometry and
+ *   ConstructMappedGeometryFace (MappedGeometry.f90:64-386, 482-760).
+ * orders: [3K] per-element (Nx, Ny, Nz) (the "polynomial order file") or NULL for N everywhere.
+ * periodic: the .control file's periodic boundaries as "name=coupled;name=coupled;..." (each
+ * "#define boundary name / type = periodic / coupled boundary = coupled" block); every other
+ * boundary makes the call fail with HX3D_EUNSUPPORTED.  On failure *out still holds the mesh
+ * object with the message (hx3d_mesh_error); destroy it. */
+typedef struct hx3d_mesh hx3d_mesh;
+int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const char *periodic, hx3d_mesh **out);
+const char *hx3d_mesh_error(const hx3d_mesh *m);
+long hx3d_mesh_num_nodes(const hx3d_mesh *m);
+long hx3d_mesh_num_face_nodes(const hx3d_mesh *m);
+/* node coordinates [3*nodes], x of node g at 3g + c (HexElement geom % x) */
+const double *hx3d_mesh_coords(const hx3d_mesh *m);
+/* sets the mesh and geometry fields of desc (K, N, faces, Ja, J, face frames) to arrays the
+ * mesh owns; the physics and discretisation fields are left to the caller */
+int hx3d_mesh_fill_desc(const hx3d_mesh *m, hx3d_case_desc *desc);
+int hx3d_mesh_destroy(hx3d_mesh *m);
 
 #ifdef __cplusplus
 }

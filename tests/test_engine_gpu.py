@@ -178,3 +178,25 @@ def test_euler_advance_matches_oracle(mixed):
     assert t == pytest.approx(to, rel=1e-13)
     assert po.normwise_error(eng.get_state() - Q0, Q - Q0).max() <= TOL
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["c2s_tgv_n3_p4_curved_split_pirozzoli_lxf", "x_noise_n2_mixed_p2to6_curved_split_pirozzoli_lxf"])
+def test_specmesh_file_to_qdot(name, tmp_path):
+    """End to end from a mesh file: the SpecMesh file the reference read, through the C front
+    end (hx3d_mesh_read_specmesh) into the engine; QDot against the reference's."""
+    import os
+    from horses3d_amd import specmesh
+    from horses3d_amd.engine import read_specmesh
+    case, meta = load_golden([p for p in GOLDEN if os.path.basename(p) == name + ".npz"][0])
+    path = os.path.join(str(tmp_path), "box.mesh")
+    specmesh.write_specmesh(path, meta["n"], meta["amp"], mesh.box_phases(meta["phase_seed"]),
+                            bface_order=max(4, meta["order"]))
+    mixed = len({tuple(r) for r in case["N"]}) > 1
+    fc = read_specmesh(path, N=meta["order"], orders=case["N"] if mixed else None,
+                       periodic=dict(specmesh.PERIODIC_PAIRS))
+    eng = engine_for(fc, physics_kwargs(meta))
+    eng.set_state(case["Q_step0"])
+    err = po.normwise_error(eng.qdot(), case["QDot_step0"])
+    print(name, err, flush=True)
+    assert err.max() <= TOL, err
+    eng.close()
