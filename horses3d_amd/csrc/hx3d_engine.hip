@@ -310,6 +310,8 @@ struct hx3d_engine {
     hxm::OpsAll *dOpsAll = nullptr;
     double *dU = nullptr, *dLift = nullptr;
     int bucket_first[9] = {0}, bucket_count[9] = {0};   // elements needing w waves: w = 1..8
+    std::vector<int> N3h;   // element orders (Nx, Ny, Nz), host copy for solution files
+    double mach = 0;
     bool qdot_valid = false, timing = false;
     double ktime[4] = {0, 0, 0, 0};
     std::vector<hipEvent_t> evpool;   // timing: 4 events per stage of the last hx3d_advance
@@ -752,6 +754,8 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     if (d->mach <= 0 || (!d->euler && d->reynolds <= 0))
         return bail(eng->fail(HX3D_EINVAL, "descriptor: mach and reynolds must be positive"));
     set_physics(eng, d);
+    eng->N3h.assign(d->N, d->N + 3L * d->K);
+    eng->mach = d->mach;
     if (!uniform) return bail(create_mixed(eng, d));
     eng->K = d->K; eng->N = N; eng->n1 = N + 1; eng->nf = eng->n1 * eng->n1; eng->ne = eng->nf * eng->n1;
     eng->F = d->F;
@@ -967,6 +971,32 @@ int hx3d_get_state(hx3d_engine *eng, double *Qh)
     HIPCHK(hipMemcpyAsync(Qh, eng->dStage, sizeof(double) * NV * eng->nodes, hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     return HX3D_OK;
+}
+
+// HexMesh_SaveSolution (HexMesh.f90:3376-3556) of the resident state: a solution file (type 2)
+// with the default reference values of the engine's Mach number
+int hx3d_save_solution(hx3d_engine *eng, const char *path, const char *header_name, int iter, double time)
+{
+    if (!eng || !path) return HX3D_EINVAL;
+    std::vector<double> Q((size_t)NV * eng->nodes);
+    int rc = hx3d_get_state(eng, Q.data());
+    if (rc) return rc;
+    double refs[6];
+    hx3d_hsol_reference_values(eng->mach, refs);
+    rc = hx3d_hsol_write(path, header_name, eng->K, eng->N3h.data(), iter, time, refs, Q.data(), nullptr);
+    if (rc) return eng->fail(rc, hx3d_hsol_error());
+    return HX3D_OK;
+}
+
+// LoadSolutionForRestart: the state of a solution file, checked against the engine's mesh
+int hx3d_load_solution(hx3d_engine *eng, const char *path, int *iter, double *time)
+{
+    if (!eng || !path) return HX3D_EINVAL;
+    std::vector<double> Q((size_t)NV * eng->nodes);
+    int rc = hx3d_hsol_read(path, eng->K, eng->N3h.data(), Q.data(), nullptr);
+    if (!rc) rc = hx3d_hsol_read_header(path, nullptr, nullptr, nullptr, iter, time, nullptr);
+    if (rc) return eng->fail(rc, hx3d_hsol_error());
+    return hx3d_set_state(eng, Q.data());
 }
 
 int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host)

@@ -47,7 +47,9 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d
            "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_max_dt", "hx3d_advance", "hx3d_residuals",
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_set_timing",
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
-           "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy"]
+           "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
+           "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
+           "hx3d_hsol_read", "hx3d_save_solution", "hx3d_load_solution"]
 
 
 def load_library(path=LIB_PATH):
@@ -88,6 +90,14 @@ def load_library(path=LIB_PATH):
     L.hx3d_mesh_coords.restype = C.POINTER(C.c_double)
     L.hx3d_mesh_fill_desc.argtypes = [P, C.POINTER(CaseDesc)]
     L.hx3d_mesh_destroy.argtypes = [P]
+    L.hx3d_hsol_error.argtypes = []
+    L.hx3d_hsol_error.restype = C.c_char_p
+    L.hx3d_hsol_reference_values.argtypes = [D, P]
+    L.hx3d_hsol_write.argtypes = [C.c_char_p, C.c_char_p, I, P, I, D, P, P, P]
+    L.hx3d_hsol_read_header.argtypes = [C.c_char_p, P, P, P, P, P, P]
+    L.hx3d_hsol_read.argtypes = [C.c_char_p, I, P, P, P]
+    L.hx3d_save_solution.argtypes = [P, C.c_char_p, C.c_char_p, I, D]
+    L.hx3d_load_solution.argtypes = [P, C.c_char_p, C.POINTER(I), C.POINTER(D)]
     _lib = L
     return L
 
@@ -235,6 +245,20 @@ class Engine:
         f = C.c_int()
         self._chk(load_library().hx3d_check_nan(self.h, C.byref(f)), "hx3d_check_nan")
         return bool(f.value)
+
+    def save_solution(self, path, iteration=0, time=None, header_name=None):
+        """HexMesh_SaveSolution of the resident state (a .hsol solution file)."""
+        L = load_library()
+        self._chk(L.hx3d_save_solution(self.h, os.fsencode(path), None if header_name is None else header_name.encode(),
+                                       int(iteration), self.t if time is None else float(time)), "hx3d_save_solution")
+
+    def load_solution(self, path):
+        """LoadSolutionForRestart: the state of a .hsol file; returns (iteration, time)."""
+        L = load_library()
+        it, tm = C.c_int(), C.c_double()
+        self._chk(L.hx3d_load_solution(self.h, os.fsencode(path), C.byref(it), C.byref(tm)), "hx3d_load_solution")
+        self.t = tm.value
+        return it.value, tm.value
 
     def synchronize(self):
         self._chk(load_library().hx3d_synchronize(self.h), "hx3d_synchronize")

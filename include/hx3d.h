@@ -129,6 +129,24 @@ int hx3d_synchronize(hx3d_engine *eng);
 int hx3d_set_timing(hx3d_engine *eng, int on);
 int hx3d_kernel_times(hx3d_engine *eng, double out_ms[4]);
 
+/* ---- Solution files (.hsol, SolutionFile.f90:61-213): the restart and output files of
+ * HexMesh_SaveSolution / HexMesh_LoadSolution (HexMesh.f90:3376-3556, 4582-4912).  Host
+ * arrays in the packed layout Q[5g+v]; gradients (type 3 files) as three such arrays U_x, U_y,
+ * U_z one after the other.  Errors: text from hx3d_hsol_error (per thread). */
+const char *hx3d_hsol_error(void);
+/* gamma, R, V, rho, T, Mach of the default reference state (PhysicsStorage_NS.f90:292-306) */
+int hx3d_hsol_reference_values(double mach, double refs[6]);
+/* header_name: the name stored in the file (NULL: path); grad may be NULL (type 2 file) */
+int hx3d_hsol_write(const char *path, const char *header_name, int K, const int *N, int iter, double time,
+                    const double refs[6], const double *Q, const double *grad);
+int hx3d_hsol_read_header(const char *path, int *type, int *node_type, int *K, int *iter, double *time,
+                          double refs[6]);
+/* checks the element count and each element's orders against N; grad may be NULL */
+int hx3d_hsol_read(const char *path, int K, const int *N, double *Q, double *grad);
+/* the resident state to / from a solution file (SaveSolution, LoadSolutionForRestart) */
+int hx3d_save_solution(hx3d_engine *eng, const char *path, const char *header_name, int iter, double time);
+int hx3d_load_solution(hx3d_engine *eng, const char *path, int *iter, double *time);
+
 /* ---- Mesh front end (host only, no GPU): a SpecMesh file read into the contract above with
  * the reference's topology and geometry.
  *   ConstructMesh_FromSpecMeshFile_ (Read_SpecMesh.f90:36-348) with ConstructFaces,
