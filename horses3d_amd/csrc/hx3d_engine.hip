@@ -202,6 +202,83 @@ k_kinetic(const double *__restrict__ Q, const double *__restrict__ Jac, const do
     }
 }
 
+// Volume integrals of the volume monitors (VolumeIntegrals.f90:179-375): sum_e sum_n w_i w_j w_k
+// J f(node) for the monitor's f, and the volume, as (value, volume) block partials.
+//   kind 1 kinetic energy rate  u . d(rho u)/dt - |u|^2/2 d(rho)/dt   (needs QDot)
+//        2 enstrophy            |curl u|^2                          (needs the gradients)
+//        3 mean velocity        |rho u| / rho
+//        4 internal energy      rho e
+//        5 entropy              log p - gamma log rho
+//        6 math entropy         -rho (log p - gamma log rho) / (gamma - 1)
+// The velocity gradients are the gradient kernel's viscous state (uniform orders) or formed from
+// the lifted state gradients U_x, U_y, U_z as getVelocityGradients_State does (mixed orders).
+__global__ void __launch_bounds__(256)
+k_monitor(int kind, const double *__restrict__ Q, const double *__restrict__ QD, const double *__restrict__ vgrad,
+          const double *__restrict__ U, const double *__restrict__ Jac, const double *__restrict__ wq, long nodes,
+          long S, Phys ph, double *__restrict__ part)
+{
+    __shared__ double sk[4], sv[4];
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double val = 0.0, vol = 0.0;
+    if (g < nodes) {
+        const double J = Jac[g], ww = wq[g];
+        double q[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
+        double f = 0.0;
+        if (kind == 1) {
+            double u = q[1] / q[0];
+            f = u * QD[S + g] - 0.5 * (u * u) * QD[g];
+            u = q[2] / q[0];
+            f = f + u * QD[2 * S + g] - 0.5 * (u * u) * QD[g];
+            u = q[3] / q[0];
+            f = f + u * QD[3 * S + g] - 0.5 * (u * u) * QD[g];
+        } else if (kind == 2) {
+            double ux[3], uy[3], uz[3];   // d(u, v, w)/dx, /dy, /dz
+            if (vgrad) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ux[c] = vgrad[c * S + g];
+                    uy[c] = vgrad[(3 + c) * S + g];
+                    uz[c] = vgrad[(6 + c) * S + g];
+                }
+            } else {
+                const double ir = 1.0 / q[0], ir2 = ir * ir;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double ud = q[1 + c] * ir2;
+                    ux[c] = ir * U[(0 * NV + 1 + c) * S + g] - ud * U[(0 * NV) * S + g];
+                    uy[c] = ir * U[(1 * NV + 1 + c) * S + g] - ud * U[(1 * NV) * S + g];
+                    uz[c] = ir * U[(2 * NV + 1 + c) * S + g] - ud * U[(2 * NV) * S + g];
+                }
+            }
+            const double a = uy[2] - uz[1], b = uz[0] - ux[2], c = ux[1] - uy[0];
+            f = a * a + b * b + c * c;
+        } else if (kind == 3) {
+            f = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        } else if (kind == 4) {
+            f = q[4];
+        } else {
+            const double p = hx::pressure(ph, q);
+            const double sv_ = log(p) - ph.gamma * log(q[0]);
+            f = kind == 5 ? sv_ : -q[0] * sv_ / ph.gm1;
+        }
+        // the reference's products: w w w J f, and for the mean velocity w w w |rho u| / rho J
+        val = kind == 3 ? ww * f / q[0] * J : ww * J * f;
+        vol = ww * J;
+    }
+    val = wave_sum(val);
+    vol = wave_sum(vol);
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0) { sk[w] = val; sv[w] = vol; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) { val += sk[x]; vol += sv[x]; }
+        part[2 * blockIdx.x] = val;
+        part[2 * blockIdx.x + 1] = vol;
+    }
+}
+
 __global__ void k_finish_sum2(const double *part, int nparts, double *out)
 {
     if (threadIdx.x == 0) {
@@ -1124,6 +1201,33 @@ int hx3d_kinetic_energy(hx3d_engine *eng, double *ke)
     HIPCHK(hipMemcpyAsync(r, eng->dStage, sizeof(r), hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     *ke = r[0] / r[1];
+    return HX3D_OK;
+}
+
+// VolumeMonitor_Update (VolumeMonitor.f90:278-330): the integral over the volume; the enstrophy
+// with its factor 1/2.  Rate and enstrophy use the derivative and gradients of the resident
+// state (computed here if the last derivative is stale).
+int hx3d_volume_monitor(hx3d_engine *eng, int kind, double *value)
+{
+    if (!eng || !value) return HX3D_EINVAL;
+    if (kind == HX3D_MON_KINETIC_ENERGY) return hx3d_kinetic_energy(eng, value);
+    if (kind < HX3D_MON_KINETIC_ENERGY_RATE || kind > HX3D_MON_MATH_ENTROPY)
+        return eng->fail(HX3D_EINVAL, "unknown volume monitor");
+    if (kind == HX3D_MON_ENSTROPHY && eng->ph.euler)
+        return eng->fail(HX3D_EUNSUPPORTED, "enstrophy needs the gradients of a Navier-Stokes case");
+    if ((kind == HX3D_MON_KINETIC_ENERGY_RATE || kind == HX3D_MON_ENSTROPHY) && !eng->qdot_valid) {
+        int rc = hx3d_compute_qdot(eng, 0.0, nullptr);
+        if (rc) return rc;
+    }
+    const int nb = (int)((eng->nodes + 255) / 256);
+    hipLaunchKernelGGL(k_monitor, dim3(nb), dim3(256), 0, eng->stream, kind, eng->dQ, eng->dQDot,
+                       eng->mixed ? nullptr : eng->dVgrad, eng->dU, eng->dJ, eng->dWq, eng->nodes, eng->S, eng->ph,
+                       eng->dPart);
+    hipLaunchKernelGGL(k_finish_sum2, dim3(1), dim3(64), 0, eng->stream, eng->dPart, nb, eng->dStage);
+    double r[2];
+    HIPCHK(hipMemcpyAsync(r, eng->dStage, sizeof(r), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    *value = (kind == HX3D_MON_ENSTROPHY ? 0.5 * r[0] : r[0]) / r[1];
     return HX3D_OK;
 }
 

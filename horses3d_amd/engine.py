@@ -22,6 +22,8 @@ INVISCID = {"standard": 0, "split-form": 1}
 AVERAGING = {"standard": 0, "ducros": 1, "kennedy-gruber": 2, "pirozzoli": 3, "morinishi": 4}
 LES = {None: 0, "none": 0, "smagorinsky": 1, "wale": 2}
 LES_DEFAULT_C = {0: 0.0, 1: 0.2, 2: 0.325}   # LESModels.f90:248, :352
+MONITORS = {"kinetic energy": 0, "kinetic energy rate": 1, "enstrophy": 2, "mean velocity": 3,
+            "internal energy": 4, "entropy": 5, "math entropy": 6}
 
 
 class CaseDesc(C.Structure):
@@ -49,7 +51,7 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
            "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
            "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
-           "hx3d_hsol_read", "hx3d_save_solution", "hx3d_load_solution"]
+           "hx3d_hsol_read", "hx3d_save_solution", "hx3d_load_solution", "hx3d_volume_monitor"]
 
 
 def load_library(path=LIB_PATH):
@@ -98,6 +100,7 @@ def load_library(path=LIB_PATH):
     L.hx3d_hsol_read.argtypes = [C.c_char_p, I, P, P, P]
     L.hx3d_save_solution.argtypes = [P, C.c_char_p, C.c_char_p, I, D]
     L.hx3d_load_solution.argtypes = [P, C.c_char_p, C.POINTER(I), C.POINTER(D)]
+    L.hx3d_volume_monitor.argtypes = [P, I, C.POINTER(D)]
     _lib = L
     return L
 
@@ -240,6 +243,15 @@ class Engine:
         ke = C.c_double()
         self._chk(load_library().hx3d_kinetic_energy(self.h, C.byref(ke)), "hx3d_kinetic_energy")
         return ke.value
+
+    def volume_monitor(self, kind):
+        """A volume monitor of the resident state (VolumeMonitor.f90): "kinetic energy",
+        "kinetic energy rate", "enstrophy", "mean velocity", "internal energy", "entropy" or
+        "math entropy"."""
+        L = load_library()
+        v = C.c_double()
+        self._chk(L.hx3d_volume_monitor(self.h, MONITORS[kind.lower()], C.byref(v)), "hx3d_volume_monitor")
+        return v.value
 
     def has_nan(self):
         f = C.c_int()

@@ -118,6 +118,14 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
 int hx3d_residuals(hx3d_engine *eng, double res[5]);
 /* Kinetic-energy monitor of the resident state. */
 int hx3d_kinetic_energy(hx3d_engine *eng, double *ke);
+/* Volume monitors (VolumeMonitor_Update, VolumeMonitor.f90:278-330, with the integrals of
+ * VolumeIntegrals.f90:179-375): the volume average of the selected quantity of the resident
+ * state (enstrophy: half the average of |curl u|^2).  The rate and the enstrophy use the time
+ * derivative and gradients of the resident state. */
+enum { HX3D_MON_KINETIC_ENERGY = 0, HX3D_MON_KINETIC_ENERGY_RATE = 1, HX3D_MON_ENSTROPHY = 2,
+       HX3D_MON_MEAN_VELOCITY = 3, HX3D_MON_INTERNAL_ENERGY = 4, HX3D_MON_ENTROPY = 5,
+       HX3D_MON_MATH_ENTROPY = 6 };
+int hx3d_volume_monitor(hx3d_engine *eng, int kind, double *value);
 /* checkForNan: *has_nan = 1 if any state value is NaN. */
 int hx3d_check_nan(hx3d_engine *eng, int *has_nan);
 /* Blocks until the engine's queued work is done. */

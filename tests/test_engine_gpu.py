@@ -230,3 +230,40 @@ def test_restart_from_reference_solution_file(name, tmp_path):
     eng.save_solution(out, it, t, header_name="RESULTS/case.hsol")
     assert open(out, "rb").read() == open(os.path.join(hs, name + "_final.hsol"), "rb").read()
     eng.close()
+
+
+MONITOR_KEYS = [("kinen", "kinetic energy"), ("kinrate", "kinetic energy rate"), ("enstr", "enstrophy"),
+                ("meanvel", "mean velocity"), ("inten", "internal energy"), ("entropy", "entropy"),
+                ("mathent", "math entropy")]
+
+
+@pytest.mark.parametrize("name", ["box3_p3_curved_split", "box2_mixed_p2to4"])
+def test_volume_monitors_match_reference(name, tmp_path):
+    """Each volume monitor of the reference run (VolumeMonitor.f90) at its last iteration, from the
+    state the reference held then (mesh from its mesh file through the front end): relative
+    difference <= 1e-12 (the sums run in another order; the rate cancels to 1e-3 of its terms)."""
+    import ast
+    import os
+    from horses3d_amd import specmesh
+    from horses3d_amd.engine import read_specmesh
+    md = os.path.join(os.path.dirname(GOLDEN[0]), "monitors")
+    z = np.load(os.path.join(md, name + ".npz"), allow_pickle=False)
+    spec = ast.literal_eval(str(z["meta"]))
+    path = os.path.join(str(tmp_path), "box.mesh")
+    specmesh.write_specmesh(path, spec["n"], spec["amp"], mesh.box_phases(spec["seed"]),
+                            bface_order=max(4, spec["N"]))
+    case = read_specmesh(path, N=spec["N"], orders=z["N"] if spec["orders"] else None,
+                         periodic=dict(specmesh.PERIODIC_PAIRS))
+    c = spec["ctl"]
+    eng = engine_for(case, dict(mach=spec["mach"], reynolds=spec["re"],
+                                riemann="lxf" if c.get("riemann") == "lax-friedrichs" else "roe",
+                                inviscid=c.get("inviscid", "standard"), averaging=c.get("averaging", "standard").lower(),
+                                les=None, les_c=None))
+    eng.set_state(z["Q_final"])
+    for key, kind in MONITOR_KEYS:
+        ref = z["mon_" + key][-1]
+        assert ref[0] == int(z["steps"]) and ref[1] == float(z["t_final"])
+        v = eng.volume_monitor(kind)
+        print(name, kind, v, ref[2], flush=True)
+        assert abs(v - ref[2]) <= 1e-12 * abs(ref[2]), (kind, v, ref[2])
+    eng.close()
