@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/hx3d.h"
+#include "frontend.hpp"
 #include "spectral.hpp"
 
 struct hx3d_mesh {
@@ -411,7 +412,8 @@ int fail(hx3d_mesh *m, int code, const std::string &msg)
     return code;
 }
 
-int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, const char *periodic)
+int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, long norders,
+                  const hx3d_front::ZoneResolver &resolve)
 {
     std::ifstream in(path);
     if (!in) return fail(m, HX3D_EINVAL, std::string("cannot open mesh file ") + path);
@@ -423,12 +425,15 @@ int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, c
     for (long j = 0; j < nn; ++j)
         if (!(in >> X[3 * j] >> X[3 * j + 1] >> X[3 * j + 2])) return fail(m, HX3D_EINVAL, "truncated node list");
     const int K = (int)ne;
+    if (orders && norders >= 0 && norders != ne)
+        return fail(m, HX3D_EINVAL, "the polynomial orders are given for " + std::to_string(norders) +
+                                        " elements, the mesh has " + std::to_string(ne));
     m->K = K;
     m->bface = bf;
     m->N.assign(3 * K, 0);
     for (int e = 0; e < K; ++e)
         for (int d = 0; d < 3; ++d) {
-            const int o = orders ? orders[3 * e + d] : Nuni;
+            const int o = !orders ? Nuni : orders[norders == hx3d_front::ORDERS_ONE_ROW ? d : 3 * e + d];
             if (o < 1 || o > 7) return fail(m, HX3D_EUNSUPPORTED, "orders must be 1..7");
             m->N[3 * e + d] = o;
         }
@@ -529,20 +534,13 @@ int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, c
         for (size_t z = 0; z < zones.size(); ++z)
             if (zones[z] == b) { faces[f].zone = (int)z; zfaces[z].push_back((int)f); break; }
     }
-    // periodic pairs "a=b;c=d": boundary a is periodic, coupled with b
+    // boundary conditions of the zones: periodic ones carry their coupled zone
     std::map<std::string, std::string> coupled;
-    if (periodic) {
-        std::stringstream ss(periodic);
-        std::string item;
-        while (std::getline(ss, item, ';')) {
-            const size_t eq = item.find('=');
-            if (eq == std::string::npos) continue;
-            auto trim = [](std::string s) {
-                size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
-                return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
-            };
-            coupled[lower(trim(item.substr(0, eq)))] = lower(trim(item.substr(eq + 1)));
-        }
+    for (const auto &z : zones) {
+        std::string c, err;
+        const int r = resolve(z, c, err);
+        if (r < 0) return fail(m, r, err);
+        if (r == 1) coupled[z] = c;
     }
     // ---- ConstructPeriodicFaces
     std::vector<char> deleted(zones.size(), 0);
@@ -903,9 +901,8 @@ int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, c
 
 }  // namespace
 
-extern "C" {
-
-int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const char *periodic, hx3d_mesh **out)
+int hx3d_front::read_specmesh(hx3d_mesh **out, const char *path, int N, const int *orders, long norders,
+                              const ZoneResolver &resolve)
 {
     if (!out) return HX3D_EINVAL;
     *out = nullptr;
@@ -914,12 +911,39 @@ int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const ch
     *out = m;
     if (!path) return fail(m, HX3D_EINVAL, "no mesh file name");
     try {
-        return read_specmesh(m, path, N, orders, periodic);
+        return ::read_specmesh(m, path, N, orders, norders, resolve);
     } catch (const std::bad_alloc &) {
         return fail(m, HX3D_ENOMEM, "out of host memory reading the mesh");
     } catch (...) {
         return fail(m, HX3D_EINVAL, "mesh file could not be read");
     }
+}
+
+extern "C" {
+
+int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const char *periodic, hx3d_mesh **out)
+{
+    // periodic pairs "a=b;c=d": boundary a is periodic, coupled with b; other zones are not periodic
+    std::map<std::string, std::string> pairs;
+    if (periodic) {
+        std::stringstream ss(periodic);
+        std::string item;
+        auto trim = [](const std::string &s) {
+            size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+            return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+        };
+        while (std::getline(ss, item, ';')) {
+            const size_t eq = item.find('=');
+            if (eq != std::string::npos) pairs[lower(trim(item.substr(0, eq)))] = lower(trim(item.substr(eq + 1)));
+        }
+    }
+    auto resolve = [&pairs](const std::string &zone, std::string &c, std::string &) {
+        auto it = pairs.find(zone);
+        if (it == pairs.end()) return 0;
+        c = it->second;
+        return 1;
+    };
+    return hx3d_front::read_specmesh(out, path, N, orders, hx3d_front::ORDERS_UNCHECKED, resolve);
 }
 
 const char *hx3d_mesh_error(const hx3d_mesh *m) { return m ? m->err.c_str() : "no mesh"; }

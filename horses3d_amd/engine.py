@@ -38,6 +38,11 @@ class CaseDesc(C.Structure):
                 ("les_c", C.c_double)]
 
 
+class ControlRun(C.Structure):
+    _fields_ = [("cfl", C.c_double), ("dcfl", C.c_double), ("dt", C.c_double), ("tolerance", C.c_double),
+                ("steps", C.c_int), ("output_interval", C.c_int), ("restart", C.c_int)]
+
+
 class HX3DError(RuntimeError):
     pass
 
@@ -51,7 +56,9 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
            "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
            "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
-           "hx3d_hsol_read", "hx3d_save_solution", "hx3d_load_solution", "hx3d_volume_monitor"]
+           "hx3d_hsol_read", "hx3d_save_solution", "hx3d_load_solution", "hx3d_volume_monitor",
+           "hx3d_control_read", "hx3d_control_error", "hx3d_control_value", "hx3d_control_boundary",
+           "hx3d_control_case", "hx3d_control_destroy"]
 
 
 def load_library(path=LIB_PATH):
@@ -101,6 +108,14 @@ def load_library(path=LIB_PATH):
     L.hx3d_save_solution.argtypes = [P, C.c_char_p, C.c_char_p, I, D]
     L.hx3d_load_solution.argtypes = [P, C.c_char_p, C.POINTER(I), C.POINTER(D)]
     L.hx3d_volume_monitor.argtypes = [P, I, C.POINTER(D)]
+    L.hx3d_control_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    L.hx3d_control_error.argtypes = [P]
+    L.hx3d_control_error.restype = C.c_char_p
+    L.hx3d_control_value.argtypes = [P, C.c_char_p]
+    L.hx3d_control_value.restype = C.c_char_p
+    L.hx3d_control_boundary.argtypes = [P, C.c_char_p, C.c_char_p, C.c_char_p, I]
+    L.hx3d_control_case.argtypes = [P, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(CaseDesc), C.POINTER(ControlRun)]
+    L.hx3d_control_destroy.argtypes = [P]
     _lib = L
     return L
 
@@ -134,25 +149,108 @@ def read_specmesh(path, N=None, orders=None, periodic=None):
             raise HX3DError("hx3d_mesh_read_specmesh: %s (code %d)" % (msg, rc))
         d = CaseDesc()
         L.hx3d_mesh_fill_desc(h, C.byref(d))
-        nodes, fnodes, K, F = L.hx3d_mesh_num_nodes(h), L.hx3d_mesh_num_face_nodes(h), d.K, d.F
-
-        def arr(ptr, n, ty, shape):
-            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ty)), shape=(n,)).copy().reshape(shape)
-
-        Nk = arr(d.N, 3 * K, C.c_int32, (K, 3))
-        case = dict(K=K, F=F, N=Nk, faceID=arr(d.faceID, 6 * K, C.c_int32, (K, 6)),
-                    faceSide=arr(d.faceSide, 6 * K, C.c_int32, (K, 6)),
-                    fElem=arr(d.fElem, 2 * F, C.c_int32, (F, 2)), fESide=arr(d.fESide, 2 * F, C.c_int32, (F, 2)),
-                    fRot=arr(d.fRot, F, C.c_int32, (F,)), Ja=arr(d.Ja, 9 * nodes, C.c_double, (nodes, 9)),
-                    J=arr(d.J, nodes, C.c_double, (nodes,)), normal=arr(d.fNormal, 3 * fnodes, C.c_double, (fnodes, 3)),
-                    t1=arr(d.fT1, 3 * fnodes, C.c_double, (fnodes, 3)), t2=arr(d.fT2, 3 * fnodes, C.c_double, (fnodes, 3)),
-                    fjac=arr(d.fJac, fnodes, C.c_double, (fnodes,)),
-                    x=arr(L.hx3d_mesh_coords(h), 3 * nodes, C.c_double, (nodes, 3)))
-        ne = np.prod(Nk + 1, axis=1)
-        case["off"] = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
-        return case
+        return _case_from_desc(L, h, d)
     finally:
         L.hx3d_mesh_destroy(h)
+
+
+def _case_from_desc(L, h, d):
+    """Copies of the mesh arrays a filled descriptor points to, as a case dict."""
+    nodes, fnodes, K, F = L.hx3d_mesh_num_nodes(h), L.hx3d_mesh_num_face_nodes(h), d.K, d.F
+
+    def arr(ptr, n, ty, shape):
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ty)), shape=(n,)).copy().reshape(shape)
+
+    Nk = arr(d.N, 3 * K, C.c_int32, (K, 3))
+    case = dict(K=K, F=F, N=Nk, faceID=arr(d.faceID, 6 * K, C.c_int32, (K, 6)),
+                faceSide=arr(d.faceSide, 6 * K, C.c_int32, (K, 6)),
+                fElem=arr(d.fElem, 2 * F, C.c_int32, (F, 2)), fESide=arr(d.fESide, 2 * F, C.c_int32, (F, 2)),
+                fRot=arr(d.fRot, F, C.c_int32, (F,)), Ja=arr(d.Ja, 9 * nodes, C.c_double, (nodes, 9)),
+                J=arr(d.J, nodes, C.c_double, (nodes,)), normal=arr(d.fNormal, 3 * fnodes, C.c_double, (fnodes, 3)),
+                t1=arr(d.fT1, 3 * fnodes, C.c_double, (fnodes, 3)), t2=arr(d.fT2, 3 * fnodes, C.c_double, (fnodes, 3)),
+                fjac=arr(d.fJac, fnodes, C.c_double, (fnodes,)),
+                x=arr(L.hx3d_mesh_coords(h), 3 * nodes, C.c_double, (nodes, 3)))
+    ne = np.prod(Nk + 1, axis=1)
+    case["off"] = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+    return case
+
+
+_NAMES = lambda table: {v: k for k, v in table.items() if k not in (None, "lxf")}
+
+
+class Control:
+    """A HORSES3D .control file read by the C reader (hx3d_control_read): key values with the
+    reference's parsing and defaults, and the boundary conditions of named zones.  Host only."""
+
+    def __init__(self, path):
+        L = load_library()
+        h = C.c_void_p()
+        rc = L.hx3d_control_read(os.fsencode(path), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = L.hx3d_control_error(h).decode() if h else "out of memory"
+            self.close()
+            raise HX3DError("hx3d_control_read: %s (code %d)" % (msg, rc))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().hx3d_control_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __getitem__(self, key):
+        v = load_library().hx3d_control_value(self.h, key.encode())
+        if v is None:
+            raise KeyError(key)
+        return v.decode()
+
+    def get(self, key, default=None):
+        try:
+            return self[key]
+        except KeyError:
+            return default
+
+    def boundary(self, zone):
+        """(type, coupled zone or "") of a mesh zone from the "#define boundary" blocks."""
+        L = load_library()
+        t, c = C.create_string_buffer(256), C.create_string_buffer(256)
+        rc = L.hx3d_control_boundary(self.h, zone.encode(), t, c, 256)
+        if rc != 0:
+            raise HX3DError("hx3d_control_boundary: %s (code %d)" % (L.hx3d_control_error(self.h).decode(), rc))
+        return t.value.decode(), c.value.decode()
+
+    def case(self, root=None):
+        """(case dict, engine keyword arguments, run settings) of the file (hx3d_control_case).
+        Relative file names are taken from root, else from the working directory."""
+        L = load_library()
+        m, d, r = C.c_void_p(), CaseDesc(), ControlRun()
+        rc = L.hx3d_control_case(self.h, os.fsencode(root) if root else None, C.byref(m), C.byref(d), C.byref(r))
+        try:
+            if rc != 0:
+                raise HX3DError("hx3d_control_case: %s (code %d)" % (L.hx3d_control_error(self.h).decode(), rc))
+            case = _case_from_desc(L, m, d)
+        finally:
+            if m:
+                L.hx3d_mesh_destroy(m)
+        physics = dict(mach=d.mach, reynolds=d.reynolds, prandtl=d.prandtl, euler=bool(d.euler),
+                       riemann=_NAMES(RIEMANN)[d.riemann], inviscid=_NAMES(INVISCID)[d.inviscid],
+                       averaging=_NAMES(AVERAGING)[d.averaging], lambda_stab=d.lambda_stab,
+                       les=_NAMES(LES)[d.les], les_c=d.les_c)
+        run = dict(cfl=r.cfl, dcfl=r.dcfl, dt=r.dt, tolerance=r.tolerance, steps=r.steps,
+                   output_interval=r.output_interval, restart=bool(r.restart))
+        return case, physics, run
+
+
+def read_control(path, root=None):
+    """(case dict, engine keyword arguments, run settings) of a .control file and the mesh and
+    order files it names: Engine(case, **physics) sets up the case it describes."""
+    c = Control(path)
+    try:
+        return c.case(root)
+    finally:
+        c.close()
 
 
 class Engine:

@@ -48,6 +48,19 @@ module hx3d_binding
       real(c_double) :: les_c = 0.0_c_double
    end type hx3d_case_desc
 
+   ! volume monitors (hx3d_volume_monitor kinds)
+   integer(c_int), parameter :: HX3D_MON_KINETIC_ENERGY = 0, HX3D_MON_KINETIC_ENERGY_RATE = 1, &
+                                HX3D_MON_ENSTROPHY = 2, HX3D_MON_MEAN_VELOCITY = 3, &
+                                HX3D_MON_INTERNAL_ENERGY = 4, HX3D_MON_ENTROPY = 5, &
+                                HX3D_MON_MATH_ENTROPY = 6
+
+   ! time-integration keys of a .control file (hx3d_control_case)
+   type, bind(C) :: hx3d_control_run
+      real(c_double) :: cfl = 0.0_c_double, dcfl = 0.0_c_double, dt = 0.0_c_double
+      real(c_double) :: tolerance = 0.0_c_double
+      integer(c_int) :: steps = 0, output_interval = 0, restart = 0
+   end type hx3d_control_run
+
    interface
       integer(c_int) function hx3d_engine_create(desc, eng) bind(C, name="hx3d_engine_create")
          import :: c_int, c_ptr, hx3d_case_desc
@@ -171,6 +184,93 @@ module hx3d_binding
       integer(c_int) function hx3d_mesh_destroy(mesh) bind(C, name="hx3d_mesh_destroy")
          import :: c_int, c_ptr
          type(c_ptr), value :: mesh
+      end function
+      integer(c_int) function hx3d_volume_monitor(eng, kind, value) bind(C, name="hx3d_volume_monitor")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: eng
+         integer(c_int), value       :: kind
+         real(c_double), intent(out) :: value
+      end function
+      ! solution files (.hsol); strings NUL-terminated, header_name may be c_null_char alone
+      type(c_ptr) function hx3d_hsol_error() bind(C, name="hx3d_hsol_error")
+         import :: c_ptr
+      end function
+      integer(c_int) function hx3d_hsol_reference_values(mach, refs) bind(C, name="hx3d_hsol_reference_values")
+         import :: c_int, c_double
+         real(c_double), value       :: mach
+         real(c_double), intent(out) :: refs(6)
+      end function
+      integer(c_int) function hx3d_hsol_write(path, header_name, K, N, iter, time, refs, Q, grad) &
+            bind(C, name="hx3d_hsol_write")
+         import :: c_int, c_ptr, c_char, c_double
+         character(kind=c_char), intent(in) :: path(*), header_name(*)
+         integer(c_int), value              :: K, iter
+         integer(c_int), intent(in)         :: N(3, *)
+         real(c_double), value              :: time
+         real(c_double), intent(in)         :: refs(6)
+         type(c_ptr), value                 :: Q, grad
+      end function
+      integer(c_int) function hx3d_hsol_read_header(path, type, node_type, K, iter, time, refs) &
+            bind(C, name="hx3d_hsol_read_header")
+         import :: c_int, c_char, c_double
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), intent(out)        :: type, node_type, K, iter
+         real(c_double), intent(out)        :: time, refs(6)
+      end function
+      integer(c_int) function hx3d_hsol_read(path, K, N, Q, grad) bind(C, name="hx3d_hsol_read")
+         import :: c_int, c_ptr, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value              :: K
+         integer(c_int), intent(in)         :: N(3, *)
+         type(c_ptr), value                 :: Q, grad
+      end function
+      integer(c_int) function hx3d_save_solution(eng, path, header_name, iter, time) bind(C, name="hx3d_save_solution")
+         import :: c_int, c_ptr, c_char, c_double
+         type(c_ptr), value                 :: eng
+         character(kind=c_char), intent(in) :: path(*), header_name(*)
+         integer(c_int), value              :: iter
+         real(c_double), value              :: time
+      end function
+      integer(c_int) function hx3d_load_solution(eng, path, iter, time) bind(C, name="hx3d_load_solution")
+         import :: c_int, c_ptr, c_char, c_double
+         type(c_ptr), value                 :: eng
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), intent(out)        :: iter
+         real(c_double), intent(out)        :: time
+      end function
+      ! .control files; root may be c_null_char alone (working directory)
+      integer(c_int) function hx3d_control_read(path, ctl) bind(C, name="hx3d_control_read")
+         import :: c_int, c_ptr, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         type(c_ptr), intent(out)           :: ctl
+      end function
+      type(c_ptr) function hx3d_control_error(ctl) bind(C, name="hx3d_control_error")
+         import :: c_ptr
+         type(c_ptr), value :: ctl
+      end function
+      type(c_ptr) function hx3d_control_value(ctl, key) bind(C, name="hx3d_control_value")
+         import :: c_ptr, c_char
+         type(c_ptr), value                 :: ctl
+         character(kind=c_char), intent(in) :: key(*)
+      end function
+      integer(c_int) function hx3d_control_boundary(ctl, zone, type, coupled, len) bind(C, name="hx3d_control_boundary")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value                  :: ctl
+         character(kind=c_char), intent(in)  :: zone(*)
+         character(kind=c_char), intent(out) :: type(*), coupled(*)
+         integer(c_int), value               :: len
+      end function
+      integer(c_int) function hx3d_control_case(ctl, root, mesh, desc, run) bind(C, name="hx3d_control_case")
+         import :: c_int, c_ptr, c_char, hx3d_case_desc, hx3d_control_run
+         type(c_ptr), value                  :: ctl
+         character(kind=c_char), intent(in)  :: root(*)
+         type(c_ptr), intent(out)            :: mesh
+         type(hx3d_case_desc), intent(out)   :: desc
+         type(hx3d_control_run), intent(out) :: run
+      end function
+      integer(c_int) function hx3d_control_destroy(ctl) bind(C, name="hx3d_control_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctl
       end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr

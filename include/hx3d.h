@@ -178,6 +178,46 @@ const double *hx3d_mesh_coords(const hx3d_mesh *m);
 int hx3d_mesh_fill_desc(const hx3d_mesh *m, hx3d_case_desc *desc);
 int hx3d_mesh_destroy(hx3d_mesh *m);
 
+/* ---- .control files (host only) ----------------------------------------------------------
+ * hx3d_control_read: ReadControlFile (ReadInputFile.f90:20-125): '!' starts a comment, a line
+ * starting with '/' is blank, "keyword = value" lines (keys lower-cased, quoted values taken
+ * between the quotes), "#define ... #end" blocks skipped; the first entry of a key is the one
+ * found (FTDictionaryClass.f90:371-535).  Then CheckInputIntegrity's defaults
+ * (NavierStokesSolver/main.f90:239-318).  On failure *out holds the object with the message. */
+typedef struct hx3d_control hx3d_control;
+int hx3d_control_read(const char *path, hx3d_control **out);
+const char *hx3d_control_error(const hx3d_control *c);
+/* stringValueForKey: the value of a key (any case), NULL when absent; valid until destroy */
+const char *hx3d_control_value(const hx3d_control *c, const char *key);
+/* boundary condition of one mesh zone from the "#define boundary a__b" blocks: GetZoneType
+ * (BoundaryConditions.f90:170-260) and ConstructPeriodicBC (PeriodicBC.f90:73-151); type and,
+ * for periodic zones, the coupled zone, lower-cased, each into len characters (NUL-ended) */
+int hx3d_control_boundary(hx3d_control *c, const char *zone, char *type, char *coupled, int len);
+
+/* the time-integration keys (constructTimeIntegrator, TimeIntegrator.f90:178-214) */
+typedef struct {
+    double cfl, dcfl;       /* "cfl", "dcfl" (dt from MaxTimeStep); 0 when "dt" is given */
+    double dt;              /* "dt" (fixed step); 0 when "cfl" is given */
+    double tolerance;       /* "convergence tolerance" */
+    int steps;              /* "number of time steps" */
+    int output_interval;    /* "output interval" */
+    int restart;            /* "restart" = .true.: start from "restart file name" */
+} hx3d_control_run;
+
+/* The whole case of a .control file: the physics and discretisation keys
+ * (ConstructPhysicsStorage_NS, SetRiemannSolver, the inviscid / viscous discretisation, the
+ * LES model, the node type), the polynomial orders (GetMeshPolynomialOrders,
+ * pAdaptationClass.f90:198-240: "polynomial order file", "polynomial order", or
+ * "polynomial order i/j/k"), and the mesh ("mesh file name", read by the front end with the
+ * boundary blocks' periodic pairs).  Relative file names are taken from root when it is given,
+ * else from the working directory (the reference's behaviour).  desc points into *mesh, which
+ * the caller destroys after hx3d_engine_create; *mesh is set whenever the mesh was read, also
+ * on failure.  Keys the engine does not implement give HX3D_EUNSUPPORTED, unknown names and
+ * missing keys HX3D_EINVAL, with the message in hx3d_control_error. */
+int hx3d_control_case(hx3d_control *c, const char *root, hx3d_mesh **mesh, hx3d_case_desc *desc,
+                      hx3d_control_run *run);
+int hx3d_control_destroy(hx3d_control *c);
+
 #ifdef __cplusplus
 }
 #endif

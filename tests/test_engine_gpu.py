@@ -202,6 +202,33 @@ def test_specmesh_file_to_qdot(name, tmp_path):
     eng.close()
 
 
+@pytest.mark.parametrize("name", ["syntax_split_kg_lxf", "orders_ijk_smagorinsky", "order_file_euler_pirozzoli"])
+def test_control_file_to_qdot(name, tmp_path):
+    """End to end from a .control file: the control file, mesh and orders file the reference ran,
+    through the C reader (hx3d_control_case) into the engine; QDot against the reference's."""
+    import ast
+    import os
+    import shutil
+    from horses3d_amd import specmesh
+    from horses3d_amd.engine import read_control
+    ctl = os.path.join(os.path.dirname(GOLDEN[0]), "control")
+    z = np.load(os.path.join(ctl, name + ".npz"), allow_pickle=False)
+    meta = ast.literal_eval(str(z["meta"]))
+    d = str(tmp_path)
+    shutil.copy(os.path.join(ctl, name + ".control"), os.path.join(d, "case.control"))
+    if os.path.exists(os.path.join(ctl, name + "_orders.txt")):
+        shutil.copy(os.path.join(ctl, name + "_orders.txt"), os.path.join(d, "orders.txt"))
+    specmesh.write_specmesh(os.path.join(d, "box.mesh"), meta["n"], meta["amp"], mesh.box_phases(meta["phase_seed"]),
+                            bface_order=meta["bface_order"])
+    case, phys, run = read_control(os.path.join(d, "case.control"), root=d)
+    eng = engine_for(case, phys)
+    eng.set_state(z["Q"])
+    err = po.normwise_error(eng.qdot(float(z["time"])), z["QDot"])
+    print(name, err, flush=True)
+    assert err.max() <= TOL, err
+    eng.close()
+
+
 @pytest.mark.parametrize("name", ["box2_p3_curved", "box2_mixed_p2to4"])
 def test_restart_from_reference_solution_file(name, tmp_path):
     """A reference run's mesh file and solution file (.hsol) into the engine: the loaded state's

@@ -33,9 +33,13 @@ def test_binding_drives_the_c_abi(tmp_path):
     exe = str(tmp_path / "abi_check")
     _run([FC, "-I", str(tmp_path), os.path.join(REPO, "tests", "fortran", "abi_check.f90"),
           str(tmp_path / "hx3d_binding.o"), LIB, "-Wl,-rpath," + os.path.dirname(LIB), "-o", exe], tmp_path)
+    from horses3d_amd import specmesh
+    from test_control import BASE, PERIODIC
+    specmesh.write_specmesh(str(tmp_path / "box.mesh"), 2, 0.0, None)
+    (tmp_path / "case.control").write_text(BASE + PERIODIC)
     env = dict(os.environ, HIP_VISIBLE_DEVICES="")
     out = _run([exe], tmp_path, env=env)
-    assert "abi_check ok" in out
+    assert "abi_check ok" in out and "control ok" in out
     assert "K <= 0" in out
 
 
