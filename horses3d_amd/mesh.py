@@ -135,6 +135,11 @@ def _map_and_grad(X0, h, amp, ph):
     return x, G
 
 
+def _along(M, A, axis):
+    """M applied along one axis of A: out[.., i, ..] = sum_l M[i, l] A[.., l, ..] (BLAS)."""
+    return np.moveaxis(np.tensordot(M, A, axes=(1, axis)), 0, axis)
+
+
 def periodic_box(n, N, amplitude=0.0, phase_seed=0, phases=None):
     """Periodic box [-pi,pi]^3 of n^3 hexahedra of order N (element e = i + n*(j + n*k))."""
     if phases is None:
@@ -165,14 +170,11 @@ def periodic_box(n, N, amplitude=0.0, phase_seed=0, phases=None):
     # the CGL derivative matrix.  Tensor-product derivatives commute, so the curl form gives
     # watertight face metrics and exact discrete metric identities.
     Xc, _ = _map_and_grad(base(xc), h, amplitude, phases)
-    Gc = np.stack([np.einsum("il,eljkd->eijkd", Dc, Xc), np.einsum("jl,eilkd->eijkd", Dc, Xc),
-                   np.einsum("kl,eijld->eijkd", Dc, Xc)], axis=-1)   # [..., d, m] = d x_d / d xi_m
+    Gc = np.stack([_along(Dc, Xc, 1), _along(Dc, Xc, 2), _along(Dc, Xc, 3)], axis=-1)   # [..., d, m] = d x_d / d xi_m
 
     def dcgl(P):
         # P (K, i, j, k, 3 comps over xi_m) -> aux[..., m, l] = d P_m / d xi_l
-        a = np.einsum("il,eljkm->eijkm", Dc, P)
-        b = np.einsum("jl,eilkm->eijkm", Dc, P)
-        c = np.einsum("kl,eijlm->eijkm", Dc, P)
+        a, b, c = _along(Dc, P, 1), _along(Dc, P, 2), _along(Dc, P, 3)
         return np.stack([a, b, c], axis=-1)
 
     Ja = np.zeros(Xc.shape[:-1] + (3, 3))    # [..., d (xi,eta,zeta), c (x,y,z)]
@@ -184,14 +186,38 @@ def periodic_box(n, N, amplitude=0.0, phase_seed=0, phases=None):
         Ja[..., 1, comp] = -0.5 * (aux[..., 0, 2] - aux[..., 2, 0])
         Ja[..., 2, comp] = -0.5 * (aux[..., 1, 0] - aux[..., 0, 1])
     Jc = np.einsum("...i,...i->...", Gc[..., :, 0], np.cross(Gc[..., :, 1], Gc[..., :, 2]))
-    Ja = np.einsum("ai,bj,ck,eijkdm->eabcdm", T, T, T, Ja)
-    J = np.einsum("ai,bj,ck,eijk->eabc", T, T, T, Jc)
+    # CGL -> GL interpolation, one direction at a time
+    Ja = _along(T, _along(T, _along(T, Ja, 1), 2), 3)
+    J = _along(T, _along(T, _along(T, Jc, 1), 2), 3)
     # node-major packing: g = e*ne + i + n1*(j + n1*k)
     pack = lambda A: np.ascontiguousarray(A.transpose(0, 3, 2, 1, *range(4, A.ndim)).reshape(K * ne, *A.shape[4:]))
-    Xp = pack(Xn)
-    Jap = pack(Ja).reshape(K * ne, 9)
-    Jp = pack(J).reshape(K * ne)
+    return _box_case(n, N, pack(Xn), pack(Ja).reshape(K * ne, 9), pack(J).reshape(K * ne), D, amplitude, phases)
 
+
+def straight_box(n, N):
+    """Straight periodic box [-pi,pi]^3 of n^3 cubes of order N, with the exact constant metrics
+    Ja = (h/2)^2 I, J = (h/2)^3 (no curl-form computation: fast at any size)."""
+    K, n1 = n ** 3, N + 1
+    ne = n1 ** 3
+    h = 2.0 * L / n
+    x, _ = gl_nodes(N)
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    lo = np.stack([(-L + h * ii).ravel(order="F"), (-L + h * jj).ravel(order="F"),
+                   (-L + h * kk).ravel(order="F")], axis=-1)
+    # node g = e*ne + i + n1*(j + n1*k): x from i, y from j, z from k
+    P = np.stack(np.meshgrid(x, x, x, indexing="ij"), axis=-1).transpose(2, 1, 0, 3).reshape(ne, 3)
+    X = (lo[:, None, :] + (P[None] + 1.0) * 0.5 * h).reshape(K * ne, 3)
+    Ja = np.zeros((K * ne, 9))
+    Ja[:, [0, 4, 8]] = (0.5 * h) ** 2
+    J = np.full(K * ne, (0.5 * h) ** 3)
+    return _box_case(n, N, X, Ja, J, deriv_matrix(x), 0.0, np.zeros(9))
+
+
+def _box_case(n, N, Xp, Jap, Jp, D, amplitude, phases):
+    """Case dict of an n^3 periodic box from packed node coordinates and metrics."""
+    K, n1 = n ** 3, N + 1
+    ne = n1 ** 3
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     # connectivity: face 3e+a joins element e's +axis face to its neighbour's -axis face (rotation 0)
     eid = lambda a, b, c: (a % n) + n * ((b % n) + n * (c % n))
     e_i, e_j, e_k = ii.ravel(order="F"), jj.ravel(order="F"), kk.ravel(order="F")

@@ -157,6 +157,77 @@ def test_mixed_orders_box_matches_oracle(inviscid, averaging, riemann, les):
     eng.close()
 
 
+def test_config3_size_matches_oracle():
+    """The p-anisotropic batching case at its full size: a 20^3 box, orders (Nx, Ny, Nz) on
+    {2..7} per element (seed 3; 1.32 M nodes, mortar faces), Taylor-Green state with 1e-3
+    velocity noise (seed 33), Mach 0.2, Re 1600, Roe + BR1.  QDot and two device-dt RK3 steps
+    against the oracle; periodic telescoping of the weighted QDot sum."""
+    case = mesh.periodic_box_mixed(20, mesh.random_orders(8000, 2, 7, 3))
+    Q0 = mesh.taylor_green(case, 0.2, noise=1e-3, seed=33)
+    kw = dict(mach=0.2, reynolds=1600.0, riemann="roe", inviscid="standard", averaging="standard",
+              les=None, les_c=None)
+    eng, ora = engine_for(case, kw), oracle_for(case, kw)
+    eng.set_state(Q0)
+    qd = eng.qdot()
+    err = po.normwise_error(qd, ora.qdot(Q0))
+    print("config3 qdot", err, flush=True)
+    assert err.max() <= TOL
+    # sum_e sum_n w J QDot = 0 per variable (interior and mortar faces telescope)
+    wJ = np.concatenate([case["J"][case["off"][e]:case["off"][e + 1]] *
+                         np.einsum("k,j,i->kji", mesh.gl_nodes(int(N[2]))[1], mesh.gl_nodes(int(N[1]))[1],
+                                   mesh.gl_nodes(int(N[0]))[1]).ravel()
+                         for e, N in enumerate(case["N"])])
+    tot, scale = (wJ[:, None] * qd).sum(axis=0), (wJ[:, None] * np.abs(qd)).sum(axis=0)
+    assert np.all(np.abs(tot) <= 1e-12 * scale), tot / scale
+    t = eng.advance(2, 0.4, 0.4, t=0.0)
+    Q, G, to = Q0.copy(), np.zeros_like(Q0), 0.0
+    for _ in range(2):
+        dt = ora.max_dt(Q, 0.4, 0.4)
+        ora.rk3_step(Q, G, to, dt)
+        to += dt
+    assert t == pytest.approx(to, rel=1e-13)
+    err = po.normwise_error(eng.get_state() - Q0, Q - Q0)
+    print("config3 two steps", err, flush=True)
+    assert err.max() <= TOL
+    eng.close()
+
+
+def test_config5_size_properties():
+    """The largest single-GPU case at its full size: 48^3 elements of order 5 (23.9 M nodes),
+    split form with Pirozzoli averaging, Roe, BR1, WALE, on the straight box (the curvilinear
+    map of the benchmark case changes no size or index).  Size-independent properties:
+      * QDot commutes with a periodic shift of the state by one element along each axis, bit
+        for bit (every element and face of this box has the same geometry);
+      * sum_e sum_n w J QDot = 0 per variable (periodic telescoping);
+      * two device-dt RK3 steps keep the state finite and lower the kinetic energy."""
+    n, N = 48, 5
+    case = mesh.straight_box(n, N)
+    Q = mesh.taylor_green(case, 0.1, noise=1e-3, seed=5)
+    print("config5 case built", flush=True)
+    eng = engine_for(case, dict(mach=0.1, reynolds=1600.0, riemann="roe", inviscid="split-form",
+                                averaging="pirozzoli", les="wale", les_c=None))
+    eng.set_state(Q)
+    qd = eng.qdot()
+    assert np.isfinite(qd).all()
+    ne = (N + 1) ** 3
+    Qe, qde = Q.reshape(n, n, n, ne, 5), qd.reshape(n, n, n, ne, 5)   # [k, j, i, node, var]
+    for axis in (2, 1, 0):
+        eng.set_state(np.roll(Qe, -1, axis=axis).reshape(-1, 5))
+        assert np.array_equal(eng.qdot().reshape(n, n, n, ne, 5), np.roll(qde, -1, axis=axis)), axis
+    w = mesh.gl_nodes(N)[1]
+    wJ = case["J"] * np.tile(np.einsum("k,j,i->kji", w, w, w).ravel(), n ** 3)
+    tot, scale = (wJ[:, None] * qd).sum(axis=0), (wJ[:, None] * np.abs(qd)).sum(axis=0)
+    assert np.all(np.abs(tot) <= 1e-12 * scale), tot / scale
+    eng.set_state(Q)
+    ke0 = eng.kinetic_energy()
+    eng.advance(2, 0.4, 0.4, t=0.0)
+    assert not eng.has_nan()
+    ke1 = eng.kinetic_energy()
+    print("config5 ke", ke0, ke1, flush=True)
+    assert 0.0 < ke1 < ke0
+    eng.close()
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_euler_advance_matches_oracle(mixed):
     """Euler equations (no gradient sweep; dt from a standalone MaxTimeStep pass each step):
