@@ -52,7 +52,7 @@ _lib = None
 EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
            "hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device",
            "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_max_dt", "hx3d_advance", "hx3d_residuals",
-           "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_set_timing",
+           "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_set_timing", "hx3d_set_step_graph",
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
            "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
            "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
@@ -87,6 +87,7 @@ def load_library(path=LIB_PATH):
     L.hx3d_check_nan.argtypes = [P, C.POINTER(I)]
     L.hx3d_synchronize.argtypes = [P]
     L.hx3d_set_timing.argtypes = [P, I]
+    L.hx3d_set_step_graph.argtypes = [P, I]
     L.hx3d_kernel_times.argtypes = [P, P]
     L.hx3d_mesh_read_specmesh.argtypes = [C.c_char_p, I, P, C.c_char_p, C.POINTER(C.c_void_p)]
     L.hx3d_mesh_error.argtypes = [P]
@@ -372,6 +373,10 @@ class Engine:
 
     def synchronize(self):
         self._chk(load_library().hx3d_synchronize(self.h), "hx3d_synchronize")
+
+    def set_step_graph(self, on):
+        """advance() replays a captured HIP graph per step (on) or issues the launches (default)."""
+        self._chk(load_library().hx3d_set_step_graph(self.h, 1 if on else 0), "hx3d_set_step_graph")
 
     def set_timing(self, on):
         self._chk(load_library().hx3d_set_timing(self.h, 1 if on else 0), "hx3d_set_timing")

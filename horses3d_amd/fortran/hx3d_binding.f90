@@ -140,6 +140,11 @@ module hx3d_binding
          import :: c_int, c_ptr
          type(c_ptr), value :: eng
       end function
+      integer(c_int) function hx3d_set_step_graph(eng, on) bind(C, name="hx3d_set_step_graph")
+         import :: c_int, c_ptr
+         type(c_ptr), value    :: eng
+         integer(c_int), value :: on
+      end function
       integer(c_int) function hx3d_set_timing(eng, on) bind(C, name="hx3d_set_timing")
          import :: c_int, c_ptr
          type(c_ptr), value    :: eng
