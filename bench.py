@@ -89,6 +89,38 @@ def cpu_baseline_port(case, kw, steps):
                 sample="C restatement (oracle), %d nodes, %d RK3 steps, 1 thread" % (nd, steps))
 
 
+def serve_bench(args):
+    """The serving stream, scaled to a run of seconds: cases of n^3 straight periodic hexes, n
+    uniform on {8..16} (the full stream goes to 24), order N uniform on {3..6}, Mach uniform on
+    [0.1, 0.8], Re log-uniform on [400, 5000], Taylor-Green state with seeded phase offsets,
+    Roe + BR1 + Smagorinsky (Cs 0.2), step counts uniform on [100, 1000], Poisson arrivals."""
+    import numpy as np
+    from horses3d_amd import mesh, serve
+    rng = np.random.default_rng(args.serve_seed)
+    reqs, t = [], 0.0
+    print("[bench] building %d serving cases" % args.serve_cases, file=sys.stderr, flush=True)
+    for k in range(args.serve_cases):
+        n, N = int(rng.integers(8, 17)), int(rng.integers(3, 7))
+        mach = float(rng.uniform(0.1, 0.8))
+        re = float(np.exp(rng.uniform(np.log(400.0), np.log(5000.0))))
+        case = mesh.straight_box(n, N)
+        Q0 = mesh.taylor_green(case, mach, phase=tuple(rng.uniform(0.0, 2.0 * np.pi, 3)))
+        t += float(rng.exponential(args.serve_gap))
+        reqs.append(serve.Request(case, dict(mach=mach, reynolds=re, riemann="roe", les="smagorinsky"), Q0,
+                                  steps=int(rng.integers(100, 1001)), arrival=t))
+    print("[bench] serving", file=sys.stderr, flush=True)
+    res, st = serve.serve(reqs, chunk=10, max_active=16, max_running=args.serve_running)
+    print("[bench] one at a time", file=sys.stderr, flush=True)
+    alone, st1 = serve.one_at_a_time(reqs)
+    same = all(r.t == a.t and np.array_equal(r.Q, a.Q) for r, a in zip(res, alone))
+    print(json.dumps({"serve": st, "one_at_a_time": st1, "results_identical": same,
+                      "throughput_ratio": st["dof_updates_per_s"] / st1["dof_updates_per_s"],
+                      "max_running": args.serve_running,
+                      "config": {"workload": "Poisson stream of %d periodic-box cases (n^3, n on 8..16; N on 3..6; "
+                                             "Mach 0.1-0.8; Re 400-5000; Roe+BR1+Smagorinsky; 100-1000 steps; "
+                                             "mean arrival gap %g s)" % (len(reqs), args.serve_gap)}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +135,14 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 5 times (the "
                          "known-size reference for calibrating FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--serve", action="store_true",
+                    help="only run the serving comparison: a Poisson stream of independent periodic-box "
+                         "cases stepped together (continuous batching) against the same stream one case "
+                         "at a time on the plain engine; prints one JSON line of both")
+    ap.add_argument("--serve-cases", type=int, default=32)
+    ap.add_argument("--serve-gap", type=float, default=0.05, help="mean arrival gap (s) of the serving stream")
+    ap.add_argument("--serve-seed", type=int, default=4)
+    ap.add_argument("--serve-running", type=int, default=2, help="cases fed at a time by the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the "
@@ -131,6 +171,9 @@ def main():
         # read + write bytes over the best of the five copies: the HBM rate a plain stream reaches
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8,
                           "copy_probe_best_GBs": round(2 * a.numel() * 8 / (min(ms) * 1e-3) / 1e9, 1)}))
+        return
+    if args.serve:
+        serve_bench(args)
         return
     print("[bench] building case %d^3 N=%d" % (args.n, args.order), file=sys.stderr, flush=True)
     case = mesh.periodic_box(args.n, args.order, amplitude=args.amplitude, phase_seed=2)

@@ -390,6 +390,7 @@ struct hx3d_engine {
     std::vector<int> N3h;   // element orders (Nx, Ny, Nz), host copy for solution files
     double mach = 0;
     bool qdot_valid = false, timing = false;
+    long dev_bytes = 0;   // device memory the engine holds
     // hx3d_advance's step as a HIP graph, captured for one (cfl, dcfl) and replayed per step
     // (off by default: replays measured 2-5% slower than the same launches issued directly)
     bool use_graph = false;
@@ -410,6 +411,7 @@ int dalloc(hx3d_engine *eng, T **p, size_t n)
     if (n == 0) n = 1;
     hipError_t e = hipMalloc((void **)p, n * sizeof(T));
     if (e != hipSuccess) return eng->fail(HX3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    eng->dev_bytes += (long)(n * sizeof(T));
     return 0;
 }
 
@@ -1279,6 +1281,32 @@ int hx3d_synchronize(hx3d_engine *eng)
     if (!eng) return HX3D_EINVAL;
     HIPCHK(hipStreamSynchronize(eng->stream));
     return HX3D_OK;
+}
+
+int hx3d_query(hx3d_engine *eng, int *done)
+{
+    if (!eng || !done) return HX3D_EINVAL;
+    const hipError_t e = hipStreamQuery(eng->stream);
+    if (e == hipSuccess) *done = 1;
+    else if (e == hipErrorNotReady) *done = 0;
+    else return eng->fail(HX3D_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    return HX3D_OK;
+}
+
+int hx3d_get_time(hx3d_engine *eng, double *t)
+{
+    if (!eng || !t) return HX3D_EINVAL;
+    HIPCHK(hipMemcpyAsync(t, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+long hx3d_engine_device_bytes(const hx3d_engine *eng) { return eng ? eng->dev_bytes : -1; }
+
+int hx3d_device_memory(size_t *free_bytes, size_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return HX3D_EINVAL;
+    return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? HX3D_OK : HX3D_EDEVICE;
 }
 
 int hx3d_set_step_graph(hx3d_engine *eng, int on)

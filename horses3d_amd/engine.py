@@ -52,7 +52,8 @@ _lib = None
 EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
            "hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device",
            "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_max_dt", "hx3d_advance", "hx3d_residuals",
-           "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_set_timing", "hx3d_set_step_graph",
+           "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_query", "hx3d_get_time", "hx3d_engine_device_bytes",
+           "hx3d_device_memory", "hx3d_set_timing", "hx3d_set_step_graph",
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
            "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
            "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
@@ -86,6 +87,11 @@ def load_library(path=LIB_PATH):
     L.hx3d_kinetic_energy.argtypes = [P, C.POINTER(D)]
     L.hx3d_check_nan.argtypes = [P, C.POINTER(I)]
     L.hx3d_synchronize.argtypes = [P]
+    L.hx3d_query.argtypes = [P, C.POINTER(I)]
+    L.hx3d_get_time.argtypes = [P, C.POINTER(D)]
+    L.hx3d_engine_device_bytes.argtypes = [P]
+    L.hx3d_engine_device_bytes.restype = C.c_long
+    L.hx3d_device_memory.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     L.hx3d_set_timing.argtypes = [P, I]
     L.hx3d_set_step_graph.argtypes = [P, I]
     L.hx3d_kernel_times.argtypes = [P, P]
@@ -244,6 +250,14 @@ class Control:
         return case, physics, run
 
 
+def device_memory():
+    """(free, total) bytes of the current HIP device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    if load_library().hx3d_device_memory(C.byref(f), C.byref(t)) != 0:
+        raise HX3DError("hx3d_device_memory failed")
+    return int(f.value), int(t.value)
+
+
 def read_control(path, root=None):
     """(case dict, engine keyword arguments, run settings) of a .control file and the mesh and
     order files it names: Engine(case, **physics) sets up the case it describes."""
@@ -332,6 +346,24 @@ class Engine:
         self._chk(load_library().hx3d_advance(self.h, nsteps, cfl, dcfl, C.byref(tt)), "hx3d_advance")
         self.t = tt.value
         return self.t
+
+    def advance_async(self, nsteps, cfl, dcfl):
+        """Queues nsteps steps on the engine's stream and returns at once (poll with done())."""
+        self._chk(load_library().hx3d_advance(self.h, nsteps, cfl, dcfl, None), "hx3d_advance")
+
+    def done(self):
+        d = C.c_int()
+        self._chk(load_library().hx3d_query(self.h, C.byref(d)), "hx3d_query")
+        return bool(d.value)
+
+    def time(self):
+        t = C.c_double()
+        self._chk(load_library().hx3d_get_time(self.h, C.byref(t)), "hx3d_get_time")
+        self.t = t.value
+        return self.t
+
+    def device_bytes(self):
+        return int(load_library().hx3d_engine_device_bytes(self.h))
 
     def residuals(self):
         r = np.zeros(5)

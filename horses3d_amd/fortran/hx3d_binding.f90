@@ -140,6 +140,24 @@ module hx3d_binding
          import :: c_int, c_ptr
          type(c_ptr), value :: eng
       end function
+      integer(c_int) function hx3d_query(eng, done) bind(C, name="hx3d_query")
+         import :: c_int, c_ptr
+         type(c_ptr), value          :: eng
+         integer(c_int), intent(out) :: done
+      end function
+      integer(c_int) function hx3d_get_time(eng, t) bind(C, name="hx3d_get_time")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: eng
+         real(c_double), intent(out) :: t
+      end function
+      integer(c_long) function hx3d_engine_device_bytes(eng) bind(C, name="hx3d_engine_device_bytes")
+         import :: c_long, c_ptr
+         type(c_ptr), value :: eng
+      end function
+      integer(c_int) function hx3d_device_memory(free_bytes, total_bytes) bind(C, name="hx3d_device_memory")
+         import :: c_int, c_size_t
+         integer(c_size_t), intent(out) :: free_bytes, total_bytes
+      end function
       integer(c_int) function hx3d_set_step_graph(eng, on) bind(C, name="hx3d_set_step_graph")
          import :: c_int, c_ptr
          type(c_ptr), value    :: eng

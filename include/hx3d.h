@@ -135,6 +135,16 @@ int hx3d_volume_monitor(hx3d_engine *eng, int kind, double *value);
 int hx3d_check_nan(hx3d_engine *eng, int *has_nan);
 /* Blocks until the engine's queued work is done. */
 int hx3d_synchronize(hx3d_engine *eng);
+/* Without blocking: *done = 1 when the engine's queued work is done, else 0.  With
+ * t_inout = NULL, hx3d_advance only queues the steps, so a host can keep several engines
+ * (each with its own stream) busy at once and poll them. */
+int hx3d_query(hx3d_engine *eng, int *done);
+/* The engine's time (advanced on the device by hx3d_advance); waits for queued work. */
+int hx3d_get_time(hx3d_engine *eng, double *t);
+/* Device memory held by the engine, in bytes. */
+long hx3d_engine_device_bytes(const hx3d_engine *eng);
+/* Free and total memory of the current HIP device (admission by free HBM). */
+int hx3d_device_memory(size_t *free_bytes, size_t *total_bytes);
 /* Average duration (ms) per launch of each stage kernel over the last hx3d_advance call,
  * measured with HIP events on the engine stream when timing was enabled with
  * hx3d_set_timing(eng, 1): out[0..2] = gradient+lifting, face flux, volume+surface+RK;

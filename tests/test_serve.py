@@ -1,0 +1,146 @@
+"""The serving loop's scheduling (horses3d_amd/serve.py) with stand-in engines and a stand-in
+clock: every request completes with the result it has alone, no case is admitted before it
+arrives, the number of running cases and the memory budget bound admission, and the latency
+figures are those of the recorded times.  Host only; the GPU form is in test_engine_gpu.py."""
+import time
+
+import numpy as np
+
+from horses3d_amd import serve
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1e-4   # every look at the clock costs 0.1 ms
+        return self.t
+
+    def sleep(self, s):
+        self.t += s
+
+
+class FakeEngine:
+    """Work of n steps takes n * 1 ms of the clock; the state counts the steps taken."""
+    live = 0
+    most = 0
+
+    def __init__(self, r, clock):
+        self.clock, self.busy_until, self.steps, self.r = clock, 0.0, 0, r
+        FakeEngine.live += 1
+        FakeEngine.most = max(FakeEngine.most, FakeEngine.live)
+
+    def set_state(self, Q):
+        self.Q = np.array(Q, dtype=float)
+
+    def device_bytes(self):
+        return 1000 * self.r.nodes
+
+    def done(self):
+        return self.clock.t >= self.busy_until
+
+    def advance_async(self, n, cfl, dcfl):
+        assert self.done()
+        self.busy_until = self.clock.t + 1e-3 * n
+        self.steps += n
+        self.Q = self.Q + n * cfl
+
+    def advance(self, n, cfl, dcfl, t=0.0):
+        self.advance_async(n, cfl, dcfl)
+        self.clock.t = self.busy_until
+        return 0.1 * self.steps
+
+    def time(self):
+        return 0.1 * self.steps
+
+    def get_state(self):
+        return self.Q.copy()
+
+    def close(self):
+        FakeEngine.live -= 1
+
+
+def requests(n=12, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nodes = int(rng.integers(100, 1000))
+        case = {"off": np.array([0, nodes])}
+        out.append(serve.Request(case, {}, np.full((nodes, 5), float(k)), steps=int(rng.integers(5, 60)),
+                                 cfl=0.5, arrival=float(rng.exponential(0.01) * k)))
+    return out
+
+
+def run(reqs, **kw):
+    clock = Clock()
+    FakeEngine.live = FakeEngine.most = 0
+    return serve.serve(reqs, engine_factory=lambda r: FakeEngine(r, clock), clock=clock, sleep=clock.sleep,
+                       poll_s=0.0, workers=0, **kw)
+
+
+def test_every_request_completes_as_alone():
+    reqs = requests()
+    res, st = run(reqs, chunk=7, max_active=4, device_memory=lambda: 1 << 40)
+    clock = Clock()
+    alone, _ = serve.one_at_a_time(reqs, engine_factory=lambda r: FakeEngine(r, clock), clock=clock,
+                                   sleep=clock.sleep)
+    assert st["cases"] == len(reqs) and FakeEngine.live == 0
+    for r, a, q in zip(res, alone, reqs):
+        assert np.array_equal(r.Q, a.Q) and r.t == a.t
+        assert np.array_equal(r.Q, q.Q0 + q.steps * 0.5)
+        assert r.admitted >= q.arrival and r.finished >= r.admitted
+    lat = np.array([r.latency for r in res])
+    assert st["p50_latency_s"] == np.percentile(lat, 50) and st["p99_latency_s"] == np.percentile(lat, 99)
+    assert st["dof_updates"] == sum(q.nodes * q.steps for q in reqs)
+
+
+def test_admission_bounds():
+    reqs = requests(16, seed=1)
+    for q in reqs:
+        q.arrival = 0.0
+    run(reqs, chunk=5, max_active=3, device_memory=lambda: 1 << 40)
+    assert FakeEngine.most == 3
+    # a memory budget of 2 MB beside a 100-byte reserve: the live cases never hold more
+    free = {"b": 2 * 1000 * 1000 + 100, "low": 1 << 60}
+    clock = Clock()
+
+    def factory(r):
+        e = FakeEngine(r, clock)
+        free["b"] -= 1000 * r.nodes
+        free["low"] = min(free["low"], free["b"])
+        close = e.close
+
+        def closing():
+            free["b"] += 1000 * r.nodes
+            close()
+        e.close = closing
+        return e
+
+    FakeEngine.live = FakeEngine.most = 0
+    res, st = serve.serve(reqs, engine_factory=factory, clock=clock, sleep=clock.sleep, poll_s=0.0, chunk=5,
+                          max_active=16, device_memory=lambda: free["b"], hbm_reserve=100, workers=0)
+    assert st["cases"] == len(reqs) and st["max_concurrent"] == FakeEngine.most > 1
+    assert free["low"] >= 100 and free["b"] == 2 * 1000 * 1000 + 100
+
+
+class WallClock:
+    """Stand-in engine clock on wall time (work of n steps takes n ms)."""
+
+    @property
+    def t(self):
+        return time.perf_counter()
+
+
+def test_threaded_setup_and_retirement():
+    """With set-up and retirement on worker threads (real clock), results are unchanged."""
+    reqs = requests(10, seed=3)
+    for q in reqs:
+        q.arrival *= 0.1
+    wall = WallClock()
+    res, st = serve.serve(reqs, engine_factory=lambda r: FakeEngine(r, wall), poll_s=1e-4, chunk=9,
+                          max_active=4, device_memory=lambda: 1 << 40, workers=2)
+    assert st["cases"] == len(reqs) and FakeEngine.live == 0
+    for r, q in zip(res, reqs):
+        assert np.array_equal(r.Q, q.Q0 + q.steps * 0.5)
+        assert r.admitted >= q.arrival
