@@ -487,17 +487,20 @@ void launch_volume(hx3d_engine *eng, double a_k, double c_k, int update, const D
     }
 }
 
+// One stage on the engine stream.  ev: NULL, or 4 events recorded around the three launches
+// (timing only); like a failed launch, a failed record surfaces through hipGetLastError, which
+// the calling entry point checks after its launches.
 template <int N>
 void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
              hipEvent_t *ev)
 {
-    if (ev) hipEventRecord(ev[0], eng->stream);
+    if (ev) (void)hipEventRecord(ev[0], eng->stream);
     if (!eng->ph.euler) launch_grad<N>(eng, dta.lim);
-    if (ev) hipEventRecord(ev[1], eng->stream);
+    if (ev) (void)hipEventRecord(ev[1], eng->stream);
     launch_face<N>(eng, dta.red);
-    if (ev) hipEventRecord(ev[2], eng->stream);
+    if (ev) (void)hipEventRecord(ev[2], eng->stream);
     launch_volume<N>(eng, a_k, c_k, update, dta, qdot);
-    if (ev) hipEventRecord(ev[3], eng->stream);
+    if (ev) (void)hipEventRecord(ev[3], eng->stream);
 }
 
 // ev: NULL, or 4 events recorded around the three launches of the stage
@@ -567,7 +570,7 @@ void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const 
 void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
                  hipEvent_t *ev)
 {
-    if (ev) hipEventRecord(ev[0], eng->stream);
+    if (ev) (void)hipEventRecord(ev[0], eng->stream);
     hxk::DtLimits lim = dta.lim;
     if (!eng->ph.euler) {
         hipLaunchKernelGGL(hxm::km_face_grad, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
@@ -582,11 +585,11 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
             if (lim.part) lim.part += 2 * (long)eng->bucket_count[w];
         }
     }
-    if (ev) hipEventRecord(ev[1], eng->stream);
+    if (ev) (void)hipEventRecord(ev[1], eng->stream);
     hipLaunchKernelGGL(hxm::km_face, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
                        eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll, eng->dFe, eng->F, eng->S,
                        eng->ES, eng->FSm, eng->ph, dta.red);
-    if (ev) hipEventRecord(ev[2], eng->stream);
+    if (ev) (void)hipEventRecord(ev[2], eng->stream);
     if (eng->inviscid != HX3D_INVISCID_SPLIT) launch_mvolume<hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
     else switch (eng->ph.averaging) {
         case HX3D_AVG_DUCROS: launch_mvolume<hxk::AVG_DUCROS>(eng, a_k, c_k, update, dta, qdot); break;
@@ -594,7 +597,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         case HX3D_AVG_PIROZZOLI: launch_mvolume<hxk::AVG_PIROZZOLI>(eng, a_k, c_k, update, dta, qdot); break;
         default: launch_mvolume<hxk::AVG_STANDARD>(eng, a_k, c_k, update, dta, qdot); break;
         }
-    if (ev) hipEventRecord(ev[3], eng->stream);
+    if (ev) (void)hipEventRecord(ev[3], eng->stream);
 }
 
 // MaxTimeStep limits of the resident state into the dt slot bits[0..1]
@@ -1002,16 +1005,17 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
 int hx3d_engine_destroy(hx3d_engine *eng)
 {
     if (!eng) return HX3D_OK;
-    if (eng->stream) hipStreamSynchronize(eng->stream);
+    // teardown: nothing is left to report an error to, so the calls' statuses are not checked
+    if (eng->stream) (void)hipStreamSynchronize(eng->stream);
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff, eng->dElist,
                     eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    for (auto &e : eng->evpool) hipEventDestroy(e);
-    if (eng->step_graph) hipGraphExecDestroy(eng->step_graph);
-    if (eng->stream) hipStreamDestroy(eng->stream);
+        if (p) (void)hipFree(p);
+    for (auto &e : eng->evpool) (void)hipEventDestroy(e);
+    if (eng->step_graph) (void)hipGraphExecDestroy(eng->step_graph);
+    if (eng->stream) (void)hipStreamDestroy(eng->stream);
     delete eng;
     return HX3D_OK;
 }
@@ -1177,7 +1181,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
             const hipError_t ce = hipStreamEndCapture(eng->stream, &g);
             if (ce != hipSuccess || !g) return eng->fail(HX3D_EDEVICE, std::string("step capture: ") + hipGetErrorString(ce));
             const hipError_t ie = hipGraphInstantiate(&eng->step_graph, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);   // the instantiated graph does not need it
             if (ie != hipSuccess) {
                 eng->step_graph = nullptr;
                 return eng->fail(HX3D_EDEVICE, std::string("step graph: ") + hipGetErrorString(ie));

@@ -402,13 +402,18 @@ __host__ __device__ constexpr int pair_slot(int n1, int lo, int hi)
 // (lo, lo + s mod N+1), s = 1..(N+1)/2 (for even N+1, s = (N+1)/2 only for lo < (N+1)/2); the
 // pair fluxes then replace the node data in LDS and every node reads its N off-diagonal
 // entries from there.  The diagonal entry is the node's inviscid contravariant flux.
-// Minimum waves per SIMD asked of the register allocator: 3 where the kernel fits in 168 VGPRs
-// without spilling (standard form, Kennedy-Gruber, Pirozzoli), else no constraint.
-constexpr int volume_waves(int form) { return (form == -1 || form == 2 || form == 3) ? 3 : 1; }
+// Minimum waves per SIMD asked of the register allocator: 3 for the standard form, and for
+// Kennedy-Gruber and Pirozzoli up to N = 5 (no spills in 168 VGPRs up to N = 4; at N = 5 the
+// 10 spilled registers cost less than a third wave gains), else no constraint (at N = 6 the
+// forced third wave measured 14% slower than two waves without spills).
+constexpr int volume_waves(int n, int form)
+{
+    return form == -1 ? 3 : (form == 2 || form == 3) && n <= 5 ? 3 : 1;
+}
 
 template <int N, int EPB, int FORM>
 __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64) * 64)
-__attribute__((amdgpu_waves_per_eu(volume_waves(FORM))))
+__attribute__((amdgpu_waves_per_eu(volume_waves(N, FORM))))
 k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ Ja,
          const double *__restrict__ Jac, const double *__restrict__ edelta, const double *__restrict__ fe,
          double *__restrict__ G, double *__restrict__ qdot_out, double a_k, double c_k, int update,
@@ -602,7 +607,10 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
         if (active) {
             double fi[NV];
             hx::euler_contravariant_row(q, pu, pv, pw, pp, &ja[3 * d], fi);
-#pragma unroll 1
+            // unrolled by two at N = 4 (0.8% faster there, same registers); other orders
+            // would spill
+            constexpr int VU = N == 4 ? 2 : 1;
+#pragma unroll VU
             for (int l = 0; l < n1; ++l) {
                 const double sh = sSharp[own_i * n1 + l], hh = sHat[own_i * n1 + l];
                 const int m = base + l * stride;
