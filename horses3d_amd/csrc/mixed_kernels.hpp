@@ -80,52 +80,115 @@ __device__ __forceinline__ long slot_node(const int *N3, const long *off, int e,
     return off[e] + idx[0] + (Ne[0] + 1) * (idx[1] + (Ne[1] + 1) * idx[2]);
 }
 
-// Face_AdaptSolutionToFace for one value at face node (i, j) of order Nf from the side's trace
-// in the face orientation, tr[c][m + (Ns0+1) l] (stride 64 between values).
-__device__ __forceinline__ double adapt(const OpsAll &ops, const FaceRec &r, int side, const double *tr,
-                                        int i, int j)
+// Face_AdaptSolutionToFace (side trace -> face order Nf) and Face_ProjectFluxToElements (face ->
+// the side's element face) at face node (i, j).  Projection type pt: 0 the orders agree (a
+// copy), 1 Tset along the first face index, 2 along the second, 3 along both, accumulated as
+// acc += T0[i][m] * T1[j][l] * value in the loop order l (outer), m (inner) of the reference.
+
+// The face's mortar matrices in LDS, m[side][0..3] = Tset(N0s -> n0), Tset(N1s -> n1) (side to
+// face) and Tset(n0 -> N0s), Tset(n1 -> N1s) (face to side), one entry per lane: the
+// interpolation loops then read them at LDS latency instead of through the cache from the
+// operator table.
+__device__ __forceinline__ void stage_mortar(const OpsAll &ops, const FaceRec &r, double (*m)[4][64], int lane)
 {
-    const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1], n0 = r.Nf[0], n1 = r.Nf[1];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int N0 = r.Nfs[s][0], N1 = r.Nfs[s][1], n0 = r.Nf[0], n1 = r.Nf[1];
+        m[s][0][lane] = ops.T[N0][n0][lane];
+        m[s][1][lane] = ops.T[N1][n1][lane];
+        m[s][2][lane] = ops.T[n0][N0][lane];
+        m[s][3][lane] = ops.T[n1][N1][lane];
+    }
+}
+
+// Face_AdaptSolutionToFace for K values at once (values k at tr[k*64 + q], the side's trace in
+// the face orientation, q = m + (Ns0+1) l), with the matrices staged by stage_mortar: each
+// source node's weight is formed once, and every value accumulates in the reference's order
+// with the reference's products.
+template <int K>
+__device__ __forceinline__ void adapt_k(const double (*m)[64], const FaceRec &r, int side, const double *tr,
+                                        int i, int j, double *out)
+{
+    const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1], n0 = r.Nf[0];
     const int pt = r.proj[side];
-    if (pt == 0) return tr[i + (n0 + 1) * j];
-    const double *T0 = ops.T[N0][n0], *T1 = ops.T[N1][n1];
-    double acc = 0.0;
+    if (pt == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = tr[k * 64 + i + (n0 + 1) * j];
+        return;
+    }
+    const double *T0 = m[0], *T1 = m[1];
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = 0.0;
     if (pt == 1) {
-        for (int l = 0; l <= N0; ++l) acc += T0[i * (N0 + 1) + l] * tr[l + (N0 + 1) * j];
+        for (int l = 0; l <= N0; ++l) {
+            const double w = T0[i * (N0 + 1) + l];
+            const int q = l + (N0 + 1) * j;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += w * tr[k * 64 + q];
+        }
     } else if (pt == 2) {
-        for (int l = 0; l <= N1; ++l) acc += T1[j * (N1 + 1) + l] * tr[i + (N0 + 1) * l];
+        for (int l = 0; l <= N1; ++l) {
+            const double w = T1[j * (N1 + 1) + l];
+            const int q = i + (N0 + 1) * l;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += w * tr[k * 64 + q];
+        }
     } else {
         for (int l = 0; l <= N1; ++l)
-            for (int m = 0; m <= N0; ++m) acc += T0[i * (N0 + 1) + m] * T1[j * (N1 + 1) + l] * tr[m + (N0 + 1) * l];
+            for (int mm = 0; mm <= N0; ++mm) {
+                const double w = T0[i * (N0 + 1) + mm] * T1[j * (N1 + 1) + l];
+                const int q = mm + (N0 + 1) * l;
+#pragma unroll
+                for (int k = 0; k < K; ++k) out[k] += w * tr[k * 64 + q];
+            }
     }
-    return acc;
 }
 
-// Face_ProjectFluxToElements for one value at node (i, j) of the side's face-orientation grid
-// from the face-order values fl[q] (q = m + (Nf0+1) l).
-__device__ __forceinline__ double project(const OpsAll &ops, const FaceRec &r, int side, const double *fl,
-                                          int i, int j)
+// Face_ProjectFluxToElements for K values at once (values k at fl[k*64 + q], q = m + (Nf0+1) l)
+template <int K>
+__device__ __forceinline__ void project_k(const double (*m)[64], const FaceRec &r, int side, const double *fl,
+                                          int i, int j, double *out)
 {
-    const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1], n0 = r.Nf[0], n1 = r.Nf[1];
+    const int n0 = r.Nf[0], n1 = r.Nf[1];
     const int pt = r.proj[side];
-    if (pt == 0) return fl[i + (n0 + 1) * j];
-    const double *T0 = ops.T[n0][N0], *T1 = ops.T[n1][N1];
-    double acc = 0.0;
+    if (pt == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = fl[k * 64 + i + (n0 + 1) * j];
+        return;
+    }
+    const double *T0 = m[2], *T1 = m[3];
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = 0.0;
     if (pt == 1) {
-        for (int l = 0; l <= n0; ++l) acc += T0[i * (n0 + 1) + l] * fl[l + (n0 + 1) * j];
+        for (int l = 0; l <= n0; ++l) {
+            const double w = T0[i * (n0 + 1) + l];
+            const int q = l + (n0 + 1) * j;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += w * fl[k * 64 + q];
+        }
     } else if (pt == 2) {
-        for (int l = 0; l <= n1; ++l) acc += T1[j * (n1 + 1) + l] * fl[i + (n0 + 1) * l];
+        for (int l = 0; l <= n1; ++l) {
+            const double w = T1[j * (n1 + 1) + l];
+            const int q = i + (n0 + 1) * l;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += w * fl[k * 64 + q];
+        }
     } else {
         for (int l = 0; l <= n1; ++l)
-            for (int m = 0; m <= n0; ++m) acc += T0[i * (n0 + 1) + m] * T1[j * (n1 + 1) + l] * fl[m + (n0 + 1) * l];
+            for (int mm = 0; mm <= n0; ++mm) {
+                const double w = T0[i * (n0 + 1) + mm] * T1[j * (n1 + 1) + l];
+                const int q = mm + (n0 + 1) * l;
+#pragma unroll
+                for (int k = 0; k < K; ++k) out[k] += w * fl[k * 64 + q];
+            }
     }
-    return acc;
 }
 
-// Stage the side's element trace of nv node arrays src[c*S + g] into tr[c*64 + q] in the face
+// Stage the side's element trace of K node arrays src[c*S + g] into tr[c*64 + q] in the face
 // orientation (the right side is rotated, Face_AdaptSolutionToFace).  One wave, lane = q.
-__device__ __forceinline__ void gather_trace(const FaceRec &r, int side, const int *N3, const long *off,
-                                             const double *src, int nv, long S, double *tr, int lane)
+template <int K>
+__device__ __forceinline__ void gather_trace_k(const FaceRec &r, int side, const int *N3, const long *off,
+                                               const double *src, long S, double *tr, int lane)
 {
     const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
     const int nq = (N0 + 1) * (N1 + 1);
@@ -134,7 +197,8 @@ __device__ __forceinline__ void gather_trace(const FaceRec &r, int side, const i
         int a = i, b = j;
         if (side == 1) left_to_right_d(i, j, N0, N1, r.rot, a, b);
         const long g = slot_node(N3, off, r.e[side], r.slot[side], a, b);
-        for (int c = 0; c < nv; ++c) tr[c * 64 + lane] = src[c * S + g];
+#pragma unroll
+        for (int c = 0; c < K; ++c) tr[c * 64 + lane] = src[c * S + g];
     }
 }
 
@@ -158,23 +222,25 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
 {
     __shared__ double sT[2][NV * 64];
     __shared__ double sU[3 * NV][64];
+    __shared__ double sM[2][4][64];
     const int f = blockIdx.x, lane = threadIdx.x;
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
-    const OpsAll &ops = *opsp;
-    gather_trace(r, 0, N3, off, Q, NV, S, sT[0], lane);
-    gather_trace(r, 1, N3, off, Q, NV, S, sT[1], lane);
+    stage_mortar(*opsp, r, sM, lane);
+    gather_trace_k<NV>(r, 0, N3, off, Q, S, sT[0], lane);
+    gather_trace_k<NV>(r, 1, N3, off, Q, S, sT[1], lane);
     __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
     if (lane < nq) {
         const int i = lane % (n0 + 1), j = lane / (n0 + 1);
         const long fq = r.foff + lane;
         const double nx = fgeo[fq], ny = fgeo[FSm + fq], nz = fgeo[2 * FSm + fq], jac = fgeo[9 * FSm + fq];
+        double qL[NV], qR[NV];
+        adapt_k<NV>(sM[0], r, 0, sT[0], i, j, qL);
+        adapt_k<NV>(sM[1], r, 1, sT[1], i, j, qR);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const double qL = adapt(ops, r, 0, &sT[0][v * 64], i, j);
-            const double qR = adapt(ops, r, 1, &sT[1][v * 64], i, j);
-            const double uS = 0.5 * (qR - qL) * jac;
+            const double uS = 0.5 * (qR[v] - qL[v]) * jac;
             sU[0 * NV + v][lane] = uS * nx;
             sU[1 * NV + v][lane] = uS * ny;
             sU[2 * NV + v][lane] = uS * nz;
@@ -186,7 +252,10 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
         if (lane < (N0 + 1) * (N1 + 1)) {
             const int i = lane % (N0 + 1), j = lane / (N0 + 1);
             const long x = elem_face_node(r, side, efoff, i, j);
-            for (int c = 0; c < 3 * NV; ++c) lift[c * ES + x] = project(ops, r, side, sU[c], i, j);
+            double lv[3 * NV];
+            project_k<3 * NV>(sM[side], r, side, sU[0], i, j, lv);
+#pragma unroll
+            for (int c = 0; c < 3 * NV; ++c) lift[c * ES + x] = lv[c];
         }
     }
 }
@@ -293,71 +362,75 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         const double *__restrict__ fgeo, const double *__restrict__ fdelta, const OpsAll *__restrict__ opsp,
         double *__restrict__ fe, int F, long S, long ES, long FSm, Phys ph, hxk::DtReduce dtr)
 {
-    constexpr int NT = NV + 3 * NV;   // Q and grad Q
-    __shared__ double sT[2][NT * 64];
-    __shared__ double sF[NV][64];
+    // LDS: both sides' Q traces; one side's grad Q trace at a time, then the face flux
+    __shared__ double sQ[2][NV * 64];
+    __shared__ double sG[3 * NV * 64];
+    __shared__ double sM[2][4][64];
     const int f = blockIdx.x, lane = threadIdx.x;
     if (dtr.bits && f == 0) hxk::reduce_dt_partials(dtr.part, dtr.n, dtr.bits, lane);
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
-    const OpsAll &ops = *opsp;
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        gather_trace(r, side, N3, off, Q, NV, S, sT[side], lane);
-        if (!ph.euler) gather_trace(r, side, N3, off, U, 3 * NV, S, sT[side] + NV * 64, lane);
-    }
+    stage_mortar(*opsp, r, sM, lane);
+    gather_trace_k<NV>(r, 0, N3, off, Q, S, sQ[0], lane);
+    gather_trace_k<NV>(r, 1, N3, off, Q, S, sQ[1], lane);
+    if (!ph.euler) gather_trace_k<3 * NV>(r, 0, N3, off, U, S, sG, lane);
     __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
-    if (lane < nq) {
-        const int i = lane % (n0 + 1), j = lane / (n0 + 1);
-        const long fq = r.foff + lane;
-        double nrm[3], t1[3], t2[3];
+    // lanes past the face's nodes work on node (0, 0) and discard the result (every index stays
+    // inside the face's arrays)
+    const bool on = lane < nq;
+    const int i = on ? lane % (n0 + 1) : 0, j = on ? lane / (n0 + 1) : 0;
+    const long fq = r.foff + (on ? lane : 0);
+    double nrm[3], t1[3], t2[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            nrm[c] = fgeo[c * FSm + fq];
-            t1[c] = fgeo[(3 + c) * FSm + fq];
-            t2[c] = fgeo[(6 + c) * FSm + fq];
-        }
-        const double jac = fgeo[9 * FSm + fq];
-        double qs[2][NV];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qs[s][v] = adapt(ops, r, s, &sT[s][v * 64], i, j);
-        double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (!ph.euler) {
-            // BR1_RiemannSolver (EllipticBR1.f90:816-868)
-            double Fs[2][NV][3];
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                double gx[NV], gy[NV], gz[NV], vg[12];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    gx[v] = adapt(ops, r, s, &sT[s][(NV + v) * 64], i, j);
-                    gy[v] = adapt(ops, r, s, &sT[s][(2 * NV + v) * 64], i, j);
-                    gz[v] = adapt(ops, r, s, &sT[s][(3 * NV + v) * 64], i, j);
-                }
-                hx::viscous_state(ph, qs[s], gx, gy, gz, vg);
-                double mu, kappa;
-                hx::laminar_mu_kappa(ph, qs[s], mu, kappa);
-                if (ph.les) {
-                    const double mt = hx::les_viscosity(ph, fdelta[f], qs[s], vg);
-                    mu = mu + mt;
-                    kappa = kappa + mt * ph.mu_to_kappa;
-                }
-                const double ir = 1.0 / qs[s][0];
-                hx::viscous_flux(mu, kappa, qs[s][1] * ir, qs[s][2] * ir, qs[s][3] * ir, vg, Fs[s]);
+    for (int c = 0; c < 3; ++c) {
+        nrm[c] = fgeo[c * FSm + fq];
+        t1[c] = fgeo[(3 + c) * FSm + fq];
+        t2[c] = fgeo[(6 + c) * FSm + fq];
+    }
+    const double jac = fgeo[9 * FSm + fq];
+    double qs[2][NV];
+    adapt_k<NV>(sM[0], r, 0, sQ[0], i, j, qs[0]);
+    adapt_k<NV>(sM[1], r, 1, sQ[1], i, j, qs[1]);
+    double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!ph.euler) {
+        // BR1_RiemannSolver (EllipticBR1.f90:816-868): each side's viscous flux from its
+        // gradients at the face order, the sides one after the other through sG
+        auto side_flux = [&](int s, double (*Fs)[3]) {
+            double g[3 * NV], vg[12];
+            adapt_k<3 * NV>(sM[s], r, s, sG, i, j, g);
+            hx::viscous_state(ph, qs[s], g, g + NV, g + 2 * NV, vg);
+            double mu, kappa;
+            hx::laminar_mu_kappa(ph, qs[s], mu, kappa);
+            if (ph.les) {
+                const double mt = hx::les_viscosity(ph, fdelta[f], qs[s], vg);
+                mu = mu + mt;
+                kappa = kappa + mt * ph.mu_to_kappa;
             }
+            const double ir = 1.0 / qs[s][0];
+            hx::viscous_flux(mu, kappa, qs[s][1] * ir, qs[s][2] * ir, qs[s][3] * ir, vg, Fs);
+        };
+        double Fs[2][NV][3];
+        if (on) side_flux(0, Fs[0]);
+        __syncthreads();   // side 0's gradients consumed
+        gather_trace_k<3 * NV>(r, 1, N3, off, U, S, sG, lane);
+        __syncthreads();
+        if (on) {
+            side_flux(1, Fs[1]);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 visc[v] = 0.5 * (Fs[0][v][0] + Fs[1][v][0]) * nrm[0] + 0.5 * (Fs[0][v][1] + Fs[1][v][1]) * nrm[1] +
                           0.5 * (Fs[0][v][2] + Fs[1][v][2]) * nrm[2];
         }
+        __syncthreads();   // sG takes the face flux next
+    }
+    double *sF = sG;   // face flux [NV][64]
+    if (on) {
         double inv[NV];
         if (ph.riemann == 1) hx::lxf_flux(ph, qs[0], qs[1], nrm, t1, t2, inv);
         else hx::roe_flux(ph, qs[0], qs[1], nrm, inv);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sF[v][lane] = (inv[v] - visc[v]) * jac;
+        for (int v = 0; v < NV; ++v) sF[v * 64 + lane] = (inv[v] - visc[v]) * jac;
     }
     __syncthreads();
     for (int side = 0; side < 2; ++side) {
@@ -366,12 +439,11 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
             const int i = lane % (N0 + 1), j = lane / (N0 + 1);
             const long x = elem_face_node(r, side, efoff, i, j);
             const int e = r.e[side], s = r.slot[side];
-            const double b = ops.b0[N3[3 * e + M_SLOT_AXIS[s]]];
+            const double b = opsp->b0[N3[3 * e + M_SLOT_AXIS[s]]];
+            double a[NV];
+            project_k<NV>(sM[side], r, side, sF, i, j, a);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const double a = project(ops, r, side, sF[v], i, j);
-                fe[v * ES + x] = (side == 0 ? a : -1.0 * a) * b;
-            }
+            for (int v = 0; v < NV; ++v) fe[v * ES + x] = (side == 0 ? a[v] : -1.0 * a[v]) * b;
         }
     }
 }
