@@ -558,7 +558,7 @@ void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const 
     DtArgs a = dta;   // time and the slot reset are advanced by the first launch only
     for (int w = 1; w <= 8; ++w) {
         if (!eng->bucket_count[w]) continue;
-        const size_t lds = sizeof(double) * per_node * 64 * w;
+        const size_t lds = sizeof(double) * (per_node * 64 * w + 6 * 64);
         hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->bucket_count[w]), dim3(64 * w), lds, eng->stream,
                            eng->dElist + eng->bucket_first[w], eng->dQ, eng->dU, eng->dN3, eng->dOff,
                            eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG,
@@ -578,7 +578,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
                            eng->S, eng->ES, eng->FSm);
         for (int w = 1; w <= 8; ++w) {
             if (!eng->bucket_count[w]) continue;
-            hipLaunchKernelGGL(hxm::km_grad, dim3(eng->bucket_count[w]), dim3(64 * w), sizeof(double) * NV * 64 * w,
+            hipLaunchKernelGGL(hxm::km_grad, dim3(eng->bucket_count[w]), dim3(64 * w), sizeof(double) * (NV * 64 * w + 3 * 64),
                                eng->stream, eng->dElist + eng->bucket_first[w], eng->dQ, eng->dN3, eng->dOff,
                                eng->dEfoff, eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES,
                                eng->ph, lim);

@@ -288,28 +288,21 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
     for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
     const double jg = Jac[g];
     const double iJ = 1.0 / jg;   // the same bits as a stored 1/J
-    // lifting values of the node's faces (slot order 5/3, 0/1, 2/4); absent faces give 0
     const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
     const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
     const long xy = efoff[6 * e + (j == 0 ? 0 : 1)] + i + nx * k;
     const long xz = efoff[6 * e + (k == 0 ? 2 : 4)] + i + nx * j;
-    double hx_[3 * NV], hy_[3 * NV], hz_[3 * NV];
-#pragma unroll
-    for (int c = 0; c < 3 * NV; ++c) {
-        const double a = lift[c * ES + xx], b = lift[c * ES + xy], d = lift[c * ES + xz];
-        hx_[c] = onx ? a : 0.0;
-        hy_[c] = ony ? b : 0.0;
-        hz_[c] = onz ? d : 0.0;
-    }
-    double *sQ = smem;   // [NV][ne]
+    double *sQ = smem;             // [NV][ne]
+    double *sD = smem + NV * ne;   // D of the three orders, [3][64]
     if (active) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) sQ[v * ne + n] = q[v];
     }
+    for (int t = n; t < 3 * 64; t += blockDim.x) sD[t] = ops.D[N3[3 * e + t / 64]][t % 64];
     __syncthreads();
     double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this node (stage 1)
     if (active) {
-    const double *Dx = ops.D[Nx], *Dy = ops.D[Ny], *Dz = ops.D[Nz];
+    const double *Dx = sD, *Dy = sD + 64, *Dz = sD + 128;
     double uxi[NV], ueta[NV], uzeta[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { uxi[v] = 0.0; ueta[v] = 0.0; uzeta[v] = 0.0; }
@@ -329,6 +322,16 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
         for (int v = 0; v < NV; ++v) uzeta[v] = uzeta[v] + sQ[v * ne + i + nx * (j + ny * l)] * dk;
     }
     const double bx = ops.b0[Nx], by = ops.b0[Ny], bz = ops.b0[Nz];
+    // lifting values of the node's faces (slot order 5/3, 0/1, 2/4); absent faces give 0.
+    // Loaded after the derivatives, so that they do not hold registers through them.
+    double hx_[3 * NV], hy_[3 * NV], hz_[3 * NV];
+#pragma unroll
+    for (int c = 0; c < 3 * NV; ++c) {
+        const double a = lift[c * ES + xx], b = lift[c * ES + xy], d = lift[c * ES + xz];
+        hx_[c] = onx ? a : 0.0;
+        hy_[c] = ony ? b : 0.0;
+        hz_[c] = onz ? d : 0.0;
+    }
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         double gr[3];
@@ -451,7 +454,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
 // ------------------------------------------------------------------------------------
 // km_volume: as hxk::k_volume with the element's own orders per direction, the viscous
 // state formed from grad Q (ViscousFlux_STATE), and MaxTimeStep's 1/|x_1 - x_0| per direction.
-// LDS (dynamic): sF[3][NF][ne], sN[ND][ne], sJ[3][ne].
+// LDS (dynamic): sF[3][NF][ne], sN[ND][ne], sJ[3][ne], then hatD and sharpD [2][3][64].
 // ------------------------------------------------------------------------------------
 template <int FORM>
 __global__ void __launch_bounds__(512)
@@ -481,6 +484,13 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
     double *sF = smem;                        // [3][NF][ne]
     double *sN = sF + 3 * NF * ne;            // [ND][ne]
     double *sJ = sN + (SPLIT ? ND * ne : 0);  // [3][ne]
+    // hatD and sharpD of the element's three orders, [2][3][64] (read at LDS latency in the
+    // contractions below; made visible by the first barrier of the direction loop)
+    double *sOp = sJ + (SPLIT ? 3 * ne : 0);
+    for (int tt = t; tt < 6 * 64; tt += blockDim.x) {
+        const int dd = (tt / 64) % 3, c = tt % 64;
+        sOp[tt] = tt < 3 * 64 ? ops.hatD[N3[3 * e + dd]][c] : ops.sharpD[N3[3 * e + dd]][c];
+    }
     // loads, unconditional (see hxk::k_volume)
     double sx[NV], sy[NV], sz[NV];
     {
@@ -579,7 +589,7 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
         const int Nd = d == 0 ? Nx : (d == 1 ? Ny : Nz);
         const int stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
         const int base = n - own * stride;
-        const double *hatD = ops.hatD[Nd], *sharpD = ops.sharpD[Nd];
+        const double *hatD = sOp + 64 * d, *sharpD = sOp + 3 * 64 + 64 * d;
         if (!SPLIT) {
             for (int l = 0; l <= Nd; ++l) {
                 const double hh = hatD[own * 8 + l];
