@@ -140,6 +140,16 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
                     sleep(min(wait, 0.05))
             elif busy:
                 sleep(poll_s)
+    except BaseException:
+        # release every engine still held before passing the error on
+        if pool:
+            pool.shutdown(wait=True)
+            pool = None
+        held = [sl[1] for sl in active]
+        held += [f.result() for f, _ in preparing.values() if f.done() and f.exception() is None]
+        for eng in held:
+            eng.close()
+        raise
     finally:
         if pool:
             pool.shutdown(wait=True)
@@ -160,6 +170,9 @@ class _Done:
 
     def result(self):
         return self.v
+
+    def exception(self):
+        return None
 
 
 def _run_now(fn, *a):

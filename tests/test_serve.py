@@ -144,3 +144,30 @@ def test_threaded_setup_and_retirement():
     for r, q in zip(res, reqs):
         assert np.array_equal(r.Q, q.Q0 + q.steps * 0.5)
         assert r.admitted >= q.arrival
+
+
+def test_failed_setup_releases_engines():
+    """A case whose set-up fails ends the loop with its error, and the engines of the other
+    cases are closed."""
+    reqs = requests(6, seed=5)
+    for q in reqs:
+        q.arrival = 0.0
+    clock = Clock()
+    made = []
+
+    def factory(r):
+        if r is reqs[3]:
+            raise RuntimeError("no memory for this case")
+        e = FakeEngine(r, clock)
+        made.append(e)
+        return e
+
+    FakeEngine.live = 0
+    try:
+        serve.serve(reqs, engine_factory=factory, clock=clock, sleep=clock.sleep, poll_s=0.0, chunk=5,
+                    max_active=16, device_memory=lambda: 1 << 40, workers=0)
+    except RuntimeError as e:
+        assert "no memory" in str(e)
+    else:
+        raise AssertionError("the set-up error was not passed on")
+    assert made and FakeEngine.live == 0
