@@ -135,6 +135,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 5 times (the "
                          "known-size reference for calibrating FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--largest", action="store_true",
+                    help="the largest single-GPU case instead: 48^3 straight periodic hexes x N=5 (23.9 M "
+                         "nodes), split-form Kennedy-Gruber + Roe + BR1 + WALE, 20 timed steps, no CPU baseline "
+                         "(the curvilinear map changes no array size and no kernel)")
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving comparison: a Poisson stream of independent periodic-box "
                          "cases stepped together (continuous batching) against the same stream one case "
@@ -175,9 +179,20 @@ def main():
     if args.serve:
         serve_bench(args)
         return
+    if args.largest:
+        args.n, args.order, args.no_cpu_baseline = 48, 5, True
+        if args.steps == 200:
+            args.steps = 20
     print("[bench] building case %d^3 N=%d" % (args.n, args.order), file=sys.stderr, flush=True)
-    case = mesh.periodic_box(args.n, args.order, amplitude=args.amplitude, phase_seed=2)
-    kw = dict(mach=0.1, reynolds=1600.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli")
+    if args.largest:
+        case = mesh.straight_box(args.n, args.order)
+        kw = dict(mach=0.1, reynolds=1600.0, riemann="roe", inviscid="split-form", averaging="kennedy-gruber",
+                  les="wale")
+        form = "split-form Kennedy-Gruber + Roe + BR1 + WALE"
+    else:
+        case = mesh.periodic_box(args.n, args.order, amplitude=args.amplitude, phase_seed=2)
+        kw = dict(mach=0.1, reynolds=1600.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli")
+        form = "split-form Pirozzoli + LxF + BR1"
     Q0 = mesh.taylor_green(case, kw["mach"])
     nd = mesh.ndof(case)
     eng = Engine(case, **kw)
@@ -255,10 +270,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (Taylor-Green vortex on a seeded curvilinear periodic box)",
-        "config": {"workload": "plain DGSEM compressible NS, %d^3 curvilinear hexes x N=%d (%d nodes), "
-                               "split-form Pirozzoli + LxF + BR1, LS-RK3, CFL 0.4"
-                               % (args.n, args.order, nd),
+        "data": "synthetic (Taylor-Green vortex on a %s periodic box)"
+                % ("straight" if args.largest else "seeded curvilinear"),
+        "config": {"workload": "plain DGSEM compressible NS, %d^3 %s hexes x N=%d (%d nodes), "
+                               "%s, LS-RK3, CFL 0.4"
+                               % (args.n, "straight" if args.largest else "curvilinear", args.order, nd, form),
                    "elements": case["K"], "order": args.order, "ndof": nd, "mach": 0.1,
                    "reynolds": 1600.0, "parallelism": "single GPU"},
         "roofline": roof,
