@@ -98,6 +98,9 @@ def test_config2_size_matches_oracle(inviscid, averaging, riemann, les):
     (6, "pirozzoli", "lxf", None),
     (6, "kennedy-gruber", "roe", "smagorinsky"),
     (7, "pirozzoli", "roe", None),
+    (5, "ducros", "lxf", None),
+    (6, "standard", "roe", "wale"),
+    (7, "kennedy-gruber", "lxf", "smagorinsky"),
 ])
 def test_high_order_split_matches_oracle(N, averaging, riemann, les):
     """Split-form volume kernels of orders 5-7 (each compiled with its own register budget) on
