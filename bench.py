@@ -89,6 +89,37 @@ def cpu_baseline_port(case, kw, steps):
                 sample="C restatement (oracle), %d nodes, %d RK3 steps, 1 thread" % (nd, steps))
 
 
+def mixed_bench(args):
+    """The p-anisotropic case: one JSON line with its rate and per-kernel-group times."""
+    import torch
+    from horses3d_amd import mesh
+    from horses3d_amd.engine import Engine
+    steps = 30 if args.steps == 200 else args.steps
+    print("[bench] building the p-anisotropic 20^3 case", file=sys.stderr, flush=True)
+    case = mesh.periodic_box_mixed(20, mesh.random_orders(8000, 2, 7, 3))
+    Q0 = mesh.taylor_green(case, 0.2, noise=1e-3, seed=33)
+    nd = len(Q0)
+    eng = Engine(case, mach=0.2, reynolds=1600.0, riemann="roe")
+    eng.set_state(Q0)
+    eng.advance(args.warmup, 0.4, 0.4, t=0.0)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.advance(steps, 0.4, 0.4)
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    assert not eng.has_nan()
+    eng.set_timing(True)
+    eng.advance(steps, 0.4, 0.4)
+    kt = eng.kernel_times()
+    eng.close()
+    print(json.dumps({"metric": "DOF-updates/s (FP64)", "value": nd * steps / el, "unit": "DOF-updates/s",
+                      "steps": steps, "ms_per_step": el / steps * 1e3, "dtype": "f64",
+                      "stage_ms": {"gradient (face unStar + lifting)": kt[0], "face flux": kt[1], "volume": kt[2]},
+                      "config": {"workload": "p-anisotropic 20^3 periodic box, (Nx,Ny,Nz) uniform on {2..7} (seed 3), "
+                                             "%d nodes, TGV + 1e-3 noise, Mach 0.2, Re 1600, Roe + BR1" % nd}}))
+
+
 def serve_bench(args):
     """The serving stream, scaled to a run of seconds: cases of n^3 straight periodic hexes, n
     uniform on {8..16} (the full stream goes to 24), order N uniform on {3..6}, Mach uniform on
@@ -139,6 +170,10 @@ def main():
                     help="the largest single-GPU case instead: 48^3 straight periodic hexes x N=5 (23.9 M "
                          "nodes), split-form Kennedy-Gruber + Roe + BR1 + WALE, 20 timed steps, no CPU baseline "
                          "(the curvilinear map changes no array size and no kernel)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="the p-anisotropic case instead: 20^3 straight periodic hexes with orders (Nx, Ny, Nz) "
+                         "uniform on {2..7} per element (seed 3, 1.32 M nodes, mortar faces), Taylor-Green state "
+                         "with 1e-3 velocity noise, Mach 0.2, Roe + BR1, 30 timed steps, no CPU baseline")
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving comparison: a Poisson stream of independent periodic-box "
                          "cases stepped together (continuous batching) against the same stream one case "
@@ -178,6 +213,9 @@ def main():
         return
     if args.serve:
         serve_bench(args)
+        return
+    if args.mixed:
+        mixed_bench(args)
         return
     if args.largest:
         args.n, args.order, args.no_cpu_baseline = 48, 5, True
