@@ -387,6 +387,11 @@ struct hx3d_engine {
     hxm::OpsAll *dOpsAll = nullptr;
     double *dU = nullptr, *dLift = nullptr;
     int bucket_first[9] = {0}, bucket_count[9] = {0};   // elements needing w waves: w = 1..8
+    // side streams for the element-bucket launches (the buckets touch disjoint elements, so
+    // they run side by side), forked from and joined back to the engine stream with events
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
+    bool capturing = false;   // a step graph is being captured: buckets stay on the engine stream
     std::vector<int> N3h;   // element orders (Nx, Ny, Nz), host copy for solution files
     double mach = 0;
     bool qdot_valid = false, timing = false;
@@ -549,22 +554,50 @@ void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 // ------------------------------------------------------------------------------------
 // mixed orders
 // ------------------------------------------------------------------------------------
+// The bucket launches of one element kernel: forked to the side streams, bucket b on stream
+// b mod 4 (0: the engine stream), joined back before the next kernel of the stage.
+bool buckets_spread(const hx3d_engine *eng) { return eng->side[0] && !eng->capturing; }
+
+void buckets_fork(hx3d_engine *eng)
+{
+    if (!buckets_spread(eng)) return;
+    (void)hipEventRecord(eng->fork_ev, eng->stream);
+    for (int k = 0; k < 3; ++k) (void)hipStreamWaitEvent(eng->side[k], eng->fork_ev, 0);
+}
+
+hipStream_t bucket_stream(const hx3d_engine *eng, int b)
+{
+    return buckets_spread(eng) && b % 4 ? eng->side[b % 4 - 1] : eng->stream;
+}
+
+void buckets_join(hx3d_engine *eng)
+{
+    if (!buckets_spread(eng)) return;
+    for (int k = 0; k < 3; ++k) {
+        (void)hipEventRecord(eng->join_ev[k], eng->side[k]);
+        (void)hipStreamWaitEvent(eng->stream, eng->join_ev[k], 0);
+    }
+}
+
 template <int FORM>
 void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
     constexpr bool SPLIT = FORM >= 0;
     constexpr int ND = hx::node_data_size(SPLIT ? FORM : 0);
     const int per_node = 3 * (SPLIT ? 4 : 5) + (SPLIT ? ND + 3 : 0);
-    DtArgs a = dta;   // time and the slot reset are advanced by the first launch only
+    DtArgs a = dta;   // the time is advanced by the first launch only
+    buckets_fork(eng);
+    int b = 0;
     for (int w = 1; w <= 8; ++w) {
         if (!eng->bucket_count[w]) continue;
         const size_t lds = sizeof(double) * (per_node * 64 * w + 6 * 64);
-        hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->bucket_count[w]), dim3(64 * w), lds, eng->stream,
+        hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->bucket_count[w]), dim3(64 * w), lds, bucket_stream(eng, b++),
                            eng->dElist + eng->bucket_first[w], eng->dQ, eng->dU, eng->dN3, eng->dOff,
                            eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG,
                            qdot, a_k, c_k, update, a, eng->S, eng->ES, eng->ph);
         a.time = nullptr;
     }
+    buckets_join(eng);
 }
 
 void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
@@ -576,14 +609,17 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         hipLaunchKernelGGL(hxm::km_face_grad, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
                            eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
                            eng->S, eng->ES, eng->FSm);
+        buckets_fork(eng);
+        int b = 0;
         for (int w = 1; w <= 8; ++w) {
             if (!eng->bucket_count[w]) continue;
             hipLaunchKernelGGL(hxm::km_grad, dim3(eng->bucket_count[w]), dim3(64 * w), sizeof(double) * (NV * 64 * w + 3 * 64),
-                               eng->stream, eng->dElist + eng->bucket_first[w], eng->dQ, eng->dN3, eng->dOff,
-                               eng->dEfoff, eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES,
-                               eng->ph, lim);
-            if (lim.part) lim.part += 2 * (long)eng->bucket_count[w];
+                               bucket_stream(eng, b++), eng->dElist + eng->bucket_first[w], eng->dQ, eng->dN3,
+                               eng->dOff, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S,
+                               eng->ES, eng->ph, lim);
+            if (lim.part) lim.part += 2 * (long)eng->bucket_count[w];   // each bucket's own partials
         }
+        buckets_join(eng);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
     hipLaunchKernelGGL(hxm::km_face, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
@@ -780,6 +816,12 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         return rc;
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "hipStreamCreate failed");
+    for (int k = 0; k < 3; ++k)
+        if (hipStreamCreateWithFlags(&eng->side[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&eng->join_ev[k], hipEventDisableTiming) != hipSuccess)
+            return eng->fail(HX3D_EDEVICE, "hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&eng->fork_ev, hipEventDisableTiming) != hipSuccess)
+        return eng->fail(HX3D_EDEVICE, "hipEventCreate failed");
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
@@ -1015,6 +1057,11 @@ int hx3d_engine_destroy(hx3d_engine *eng)
         if (p) (void)hipFree(p);
     for (auto &e : eng->evpool) (void)hipEventDestroy(e);
     if (eng->step_graph) (void)hipGraphExecDestroy(eng->step_graph);
+    for (int k = 0; k < 3; ++k) {
+        if (eng->side[k]) (void)hipStreamDestroy(eng->side[k]);
+        if (eng->join_ev[k]) (void)hipEventDestroy(eng->join_ev[k]);
+    }
+    if (eng->fork_ev) (void)hipEventDestroy(eng->fork_ev);
     if (eng->stream) (void)hipStreamDestroy(eng->stream);
     delete eng;
     return HX3D_OK;
@@ -1177,7 +1224,9 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
             }
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
+            eng->capturing = true;
             issue_step(nullptr);
+            eng->capturing = false;
             const hipError_t ce = hipStreamEndCapture(eng->stream, &g);
             if (ce != hipSuccess || !g) return eng->fail(HX3D_EDEVICE, std::string("step capture: ") + hipGetErrorString(ce));
             const hipError_t ie = hipGraphInstantiate(&eng->step_graph, g, nullptr, nullptr, 0);
