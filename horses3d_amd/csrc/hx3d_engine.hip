@@ -1162,6 +1162,26 @@ int hx3d_rk3_step(hx3d_engine *eng, double t, double dt)
     return HX3D_OK;
 }
 
+int hx3d_advance_dt(hx3d_engine *eng, int nsteps, double dt, double *t_inout)
+{
+    if (!eng || nsteps < 0 || !(dt > 0.0)) return HX3D_EINVAL;
+    if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
+    const DtArgs a = host_dt(eng, dt);
+    for (int s = 0; s < nsteps; ++s)
+        for (int k = 0; k < 3; ++k) {
+            DtArgs b = a;
+            if (k == 2) b.time = eng->dScal + 1;   // t += dt once per step, on the device
+            stage(eng, RK_A[k], RK_C[k], 1, b, nullptr);
+        }
+    HIPCHK(hipGetLastError());
+    eng->qdot_valid = false;
+    if (t_inout) {
+        HIPCHK(hipMemcpyAsync(t_inout, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+    }
+    return HX3D_OK;
+}
+
 int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt)
 {
     if (!eng || !dt) return HX3D_EINVAL;

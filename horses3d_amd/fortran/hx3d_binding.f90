@@ -121,6 +121,13 @@ module hx3d_binding
          real(c_double), value         :: cfl, dcfl
          real(c_double), intent(inout) :: t
       end function
+      integer(c_int) function hx3d_advance_dt(eng, nsteps, dt, t) bind(C, name="hx3d_advance_dt")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value            :: eng
+         integer(c_int), value         :: nsteps
+         real(c_double), value         :: dt
+         real(c_double), intent(inout) :: t
+      end function
       integer(c_int) function hx3d_residuals(eng, res) bind(C, name="hx3d_residuals")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value          :: eng

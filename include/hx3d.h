@@ -114,6 +114,9 @@ int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt);
 /* The step loop: nsteps RK3 steps, each with dt = MaxTimeStep(cfl, dcfl) of the state at
  * the start of the step (computed on the device; no host round trip).  t_inout advances. */
 int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *t_inout);
+/* The step loop with a fixed step: nsteps RK3 steps of size dt (the .control file's "dt" key
+ * instead of "cfl", TimeIntegrator.f90:196-198); t_inout advances by dt per step. */
+int hx3d_advance_dt(hx3d_engine *eng, int nsteps, double dt, double *t_inout);
 /* on = 1: hx3d_advance issues each step as a HIP graph (captured on the first call for a given
  * (cfl, dcfl), replayed once per step; kernel launches only; not while timing is on).  on = 0
  * (the default): the same launches issued one by one, which measured 2-5% faster on MI355X

@@ -252,6 +252,29 @@ def test_config5_size_properties():
     eng.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_fixed_dt_advance_matches_single_steps(mixed):
+    """hx3d_advance_dt (the "dt" key's loop) against the same number of hx3d_rk3_step calls,
+    bit for bit, and the time it reaches against the sum of the steps."""
+    case = (mesh.periodic_box_mixed(3, mesh.random_orders(27, 2, 5, 9)) if mixed
+            else mesh.periodic_box(3, 3, amplitude=0.1, phase_seed=4))
+    Q0 = mesh.taylor_green(case, 0.3, noise=1e-3, seed=6)
+    kw = dict(mach=0.3, reynolds=900.0, riemann="roe", inviscid="split-form", averaging="kennedy-gruber")
+    ea, eb = engine_for(case, kw), engine_for(case, kw)
+    ea.set_state(Q0)
+    eb.set_state(Q0)
+    dt = 0.5 * ea.max_dt(0.4, 0.4)
+    t = ea.advance_dt(5, dt, t=0.25)
+    tb = 0.25
+    for _ in range(5):
+        eb.rk3_step(tb, dt)
+        tb += dt
+    assert t == tb
+    assert np.array_equal(ea.get_state(), eb.get_state())
+    ea.close()
+    eb.close()
+
+
 @pytest.mark.parametrize("kind", ["uniform", "mixed", "euler"])
 def test_step_graph_matches_eager(kind):
     """hx3d_advance's captured step graph against the same launches issued one by one: replays
