@@ -120,6 +120,47 @@ def mixed_bench(args):
                                              "%d nodes, TGV + 1e-3 noise, Mach 0.2, Re 1600, Roe + BR1" % nd}}))
 
 
+def batch_bench(args):
+    """Many small cases: one engine over their union against one engine per case, same steps."""
+    import numpy as np
+    import torch
+    from horses3d_amd import batch, mesh
+    from horses3d_amd.engine import Engine
+    rng = np.random.default_rng(7)
+    ncase, steps, dt = 64, 100, 1e-3
+    cases = [mesh.straight_box(4, 3) for _ in range(ncase)]
+    Q0 = [mesh.taylor_green(c, 0.1, phase=tuple(rng.uniform(0.0, 2.0 * np.pi, 3))) for c in cases]
+    kw = dict(mach=0.1, reynolds=1600.0, riemann="roe")
+    nodes = sum(len(q) for q in Q0)
+    print("[bench] one engine per case", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    alone = []
+    for c, q in zip(cases, Q0):
+        e = Engine(c, **kw)
+        e.set_state(q)
+        e.advance_dt(steps, dt, t=0.0)
+        alone.append(e.get_state())
+        e.close()
+    t_alone = time.perf_counter() - t0
+    print("[bench] one engine for all cases", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = batch.Batch(cases, **kw)
+    b.set_states(Q0)
+    b.advance_dt(steps, dt, t=0.0)
+    got = b.get_states()
+    b.close()
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a, g) for a, g in zip(alone, got))
+    print(json.dumps({"cases": ncase, "steps": steps, "nodes": nodes,
+                      "batched_dof_updates_per_s": nodes * steps / t_batch,
+                      "one_engine_per_case_dof_updates_per_s": nodes * steps / t_alone,
+                      "ratio": t_alone / t_batch, "results_identical": same,
+                      "config": {"workload": "%d independent 4^3 x N=3 periodic boxes, TGV with seeded phases, "
+                                             "Roe + BR1, Mach 0.1, Re 1600, %d steps of dt %g; wall time incl. "
+                                             "engine set-up, upload and read-back" % (ncase, steps, dt)}}))
+
+
 def serve_bench(args):
     """The serving stream, scaled to a run of seconds: cases of n^3 straight periodic hexes, n
     uniform on {8..16} (the full stream goes to 24), order N uniform on {3..6}, Mach uniform on
@@ -174,6 +215,10 @@ def main():
                     help="the p-anisotropic case instead: 20^3 straight periodic hexes with orders (Nx, Ny, Nz) "
                          "uniform on {2..7} per element (seed 3, 1.32 M nodes, mortar faces), Taylor-Green state "
                          "with 1e-3 velocity noise, Mach 0.2, Roe + BR1, 30 timed steps, no CPU baseline")
+    ap.add_argument("--batch", action="store_true",
+                    help="only run the batching comparison: 64 independent 4^3 x N=3 periodic boxes (the "
+                         "reference's CPU case size; seeded Taylor-Green phases; Roe + BR1, Mach 0.1, Re 1600) "
+                         "for 100 fixed-dt steps, one engine over all of them against one engine per case")
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving comparison: a Poisson stream of independent periodic-box "
                          "cases stepped together (continuous batching) against the same stream one case "
@@ -216,6 +261,9 @@ def main():
         return
     if args.mixed:
         mixed_bench(args)
+        return
+    if args.batch:
+        batch_bench(args)
         return
     if args.largest:
         args.n, args.order, args.no_cpu_baseline = 48, 5, True

@@ -252,6 +252,36 @@ def test_config5_size_properties():
     eng.close()
 
 
+@pytest.mark.parametrize("orders", ["same", "different"])
+def test_batch_matches_cases_alone(orders):
+    """Independent cases in one engine (horses3d_amd/batch.py), stepped with a fixed dt: each
+    case's slice equals the case run alone, bit for bit when the cases share one order (the same
+    kernels run), to 1e-12 of the increment when the batch runs the mixed-order kernels."""
+    from horses3d_amd import batch
+    Ns = (3, 3, 3) if orders == "same" else (2, 3, 4)
+    cases = [mesh.periodic_box(2 + k, N, amplitude=0.1, phase_seed=k + 1) for k, N in enumerate(Ns)]
+    Q0 = [mesh.taylor_green(c, 0.2, noise=1e-3, seed=k) for k, c in enumerate(cases)]
+    kw = dict(mach=0.2, reynolds=1200.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli",
+              les="smagorinsky")
+    b = batch.Batch(cases, **kw)
+    b.set_states(Q0)
+    dt = 2e-3
+    tb = b.advance_dt(6, dt, t=0.0)
+    got = b.get_states()
+    b.close()
+    for c, q0, q in zip(cases, Q0, got):
+        e = engine_for(c, kw)
+        e.set_state(q0)
+        t = e.advance_dt(6, dt, t=0.0)
+        alone = e.get_state()
+        e.close()
+        assert t == tb
+        if orders == "same":
+            assert np.array_equal(q, alone)
+        else:
+            assert po.normwise_error(q - q0, alone - q0).max() <= TOL
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_fixed_dt_advance_matches_single_steps(mixed):
     """hx3d_advance_dt (the "dt" key's loop) against the same number of hx3d_rk3_step calls,
