@@ -368,7 +368,6 @@ struct hx3d_engine {
     Phys ph{};
     Ops ops{};
     int inviscid = 0;
-    int vol_tile = 0;   // 0: default elements per block of the volume kernel, 1: half
     double dcsi = 0;
     hipStream_t stream = nullptr;
     double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr;
@@ -470,14 +469,10 @@ void launch_volume_e(hx3d_engine *eng, double a_k, double c_k, int update, const
                        qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->ES, eng->ops, eng->ph);
 }
 
-// Two compiled tilings of the volume kernel per order: the default elements per block
-// (epb_for) and a half-size block (vol_tile = 1), selectable per engine.
 template <int N, int FORM>
 void launch_volume_f(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
-    constexpr int E0 = epb_for(N), E1 = E0 > 1 ? E0 / 2 : 1;
-    if (eng->vol_tile == 1) launch_volume_e<N, FORM, E1>(eng, a_k, c_k, update, dta, qdot);
-    else launch_volume_e<N, FORM, E0>(eng, a_k, c_k, update, dta, qdot);
+    launch_volume_e<N, FORM, epb_for(N)>(eng, a_k, c_k, update, dta, qdot);
 }
 
 template <int N>
@@ -893,7 +888,6 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     eng->FS = (long)eng->F * eng->nf;
     eng->ES = (((long)eng->K * 6 * eng->nf + 63) / 64) * 64;
     eng->inviscid = d->inviscid;
-    if (const char *vt = std::getenv("HX3D_VOLUME_TILE")) eng->vol_tile = std::atoi(vt) == 1 ? 1 : 0;
     eng->ops = make_ops(N);
     {
         double x[8], w[8];
