@@ -25,20 +25,20 @@ import pyoracle as po  # noqa: E402
 
 # name: (mesh n, N, amp, phase seed, steps, dump steps, control kwargs, ic noise amp, noise seed)
 CASES = {
-    # config 1: 4^3 straight, N=3, Roe, BR1, standard form, M=0.1, Re=1600, 100 steps
+    # the reference's CPU case: 4^3 straight, N=3, Roe, BR1, standard form, M=0.1, Re=1600, 100 steps
     "c1_tgv_n4_p3_roe_standard": dict(n=4, N=3, amp=0.0, seed=0, steps=100, dumps=[0, 1, 10],
                                       ctl=dict(mach=0.1, re=1600.0)),
-    # config 2 in small: curvilinear map, split form Pirozzoli (KEP), LxF
+    # the 16^3 x N=4 benchmark case in small: curvilinear map, split form Pirozzoli (KEP), LxF
     "c2s_tgv_n3_p4_curved_split_pirozzoli_lxf": dict(
         n=3, N=4, amp=0.1, seed=2, steps=5, dumps=[0, 2],
         ctl=dict(mach=0.1, re=1600.0, riemann="lax-friedrichs", inviscid="split-form",
                  averaging="Pirozzoli")),
-    # config 5 in small: curvilinear (amplitude 0.15), split KEP, Roe, WALE, N=5
+    # the 48^3 x N=5 case in small: curvilinear (amplitude 0.15), split KEP, Roe, WALE, N=5
     "c5s_tgv_n2_p5_curved_split_kg_roe_wale": dict(
         n=2, N=5, amp=0.15, seed=5, steps=4, dumps=[0, 2],
         ctl=dict(mach=0.1, re=1600.0, riemann="roe", inviscid="split-form",
                  averaging="Kennedy-Gruber", les="Wale")),
-    # config 4 in small: Smagorinsky LES, Mach 0.5, Re 800, N=3
+    # a serving-stream case in small: Smagorinsky LES, Mach 0.5, Re 800, N=3
     "c4s_tgv_n3_p3_smagorinsky": dict(
         n=3, N=3, amp=0.0, seed=0, steps=4, dumps=[0, 2],
         ctl=dict(mach=0.5, re=800.0, les="Smagorinsky", les_intensity=0.2)),
@@ -55,7 +55,7 @@ CASES = {
         n=2, N=7, amp=0.1, seed=9, steps=2, dumps=[0], noise=0.05, noise_seed=91,
         ctl=dict(mach=0.4, re=2000.0, riemann="lax-friedrichs", inviscid="standard",
                  averaging="Kennedy-Gruber")),
-    # config 3 in small: p-anisotropic orders (Nx, Ny, Nz) uniform on {2..5} per element, mortar
+    # the p-anisotropic 20^3 case in small: p-anisotropic orders (Nx, Ny, Nz) uniform on {2..5} per element, mortar
     # faces between mismatched orders, velocity noise, Mach 0.2, Roe + BR1
     "c3s_tgv_n3_mixed_p2to5_roe": dict(
         n=3, N=5, orders=(2, 5, 33), amp=0.0, seed=0, steps=3, dumps=[0, 2], noise=1e-3, noise_seed=333,

@@ -55,8 +55,8 @@ def test_rk3_step_and_dt_match_oracle(golden):
     eng.close()
 
 
-def test_config1_100_steps_kinetic_energy():
-    """config 1: 4^3, N=3, Roe/BR1/standard, CFL 0.4, 100 device-side dt steps."""
+def test_box4_n3_100_steps_kinetic_energy():
+    """4^3 box, N=3, Roe/BR1/standard, CFL 0.4: 100 device-side dt steps against the reference."""
     case, meta = load_golden([p for p in GOLDEN if "c1_tgv" in p][0])
     eng = engine_for(case, physics_kwargs(meta))
     eng.set_state(case["Q_step0"])
@@ -79,8 +79,8 @@ def test_config1_100_steps_kinetic_energy():
     ("standard", "standard", "roe", None),
     ("split-form", "kennedy-gruber", "roe", "wale"),
 ])
-def test_config2_size_matches_oracle(inviscid, averaging, riemann, les):
-    """config 2 size (16^3 x N=4, curvilinear) with a perturbed Taylor-Green state."""
+def test_box16_n4_curved_matches_oracle(inviscid, averaging, riemann, les):
+    """16^3 x N=4 curvilinear box (the benchmark case) with a perturbed Taylor-Green state."""
     case = mesh.periodic_box(16, 4, amplitude=0.1, phase_seed=2)
     Q = mesh.taylor_green(case, 0.1, noise=1e-3, seed=22)
     kw = dict(mach=0.1, reynolds=1600.0, riemann=riemann, inviscid=inviscid, averaging=averaging,
@@ -181,7 +181,7 @@ def test_mixed_orders_box_matches_oracle(inviscid, averaging, riemann, les):
     eng.close()
 
 
-def test_config3_size_matches_oracle():
+def test_box20_p2to7_matches_oracle():
     """The p-anisotropic batching case at its full size: a 20^3 box, orders (Nx, Ny, Nz) on
     {2..7} per element (seed 3; 1.32 M nodes, mortar faces), Taylor-Green state with 1e-3
     velocity noise (seed 33), Mach 0.2, Re 1600, Roe + BR1.  QDot and two device-dt RK3 steps
@@ -194,7 +194,7 @@ def test_config3_size_matches_oracle():
     eng.set_state(Q0)
     qd = eng.qdot()
     err = po.normwise_error(qd, ora.qdot(Q0))
-    print("config3 qdot", err, flush=True)
+    print("20^3 p-anisotropic qdot", err, flush=True)
     assert err.max() <= TOL
     # sum_e sum_n w J QDot = 0 per variable (interior and mortar faces telescope)
     wJ = np.concatenate([case["J"][case["off"][e]:case["off"][e + 1]] *
@@ -211,12 +211,12 @@ def test_config3_size_matches_oracle():
         to += dt
     assert t == pytest.approx(to, rel=1e-13)
     err = po.normwise_error(eng.get_state() - Q0, Q - Q0)
-    print("config3 two steps", err, flush=True)
+    print("20^3 p-anisotropic two steps", err, flush=True)
     assert err.max() <= TOL
     eng.close()
 
 
-def test_config5_size_properties():
+def test_box48_n5_properties():
     """The largest single-GPU case at its full size: 48^3 elements of order 5 (23.9 M nodes),
     split form with Pirozzoli averaging, Roe, BR1, WALE, on the straight box (the curvilinear
     map of the benchmark case changes no size or index).  Size-independent properties:
@@ -227,7 +227,7 @@ def test_config5_size_properties():
     n, N = 48, 5
     case = mesh.straight_box(n, N)
     Q = mesh.taylor_green(case, 0.1, noise=1e-3, seed=5)
-    print("config5 case built", flush=True)
+    print("48^3 x N=5 case built", flush=True)
     eng = engine_for(case, dict(mach=0.1, reynolds=1600.0, riemann="roe", inviscid="split-form",
                                 averaging="pirozzoli", les="wale", les_c=None))
     eng.set_state(Q)
@@ -247,7 +247,7 @@ def test_config5_size_properties():
     eng.advance(2, 0.4, 0.4, t=0.0)
     assert not eng.has_nan()
     ke1 = eng.kinetic_energy()
-    print("config5 ke", ke0, ke1, flush=True)
+    print("48^3 x N=5 kinetic energy", ke0, ke1, flush=True)
     assert 0.0 < ke1 < ke0
     eng.close()
 

@@ -39,7 +39,7 @@ def test_max_dt_matches_reference(golden):
 
 
 def test_rk3_trajectory_matches_reference():
-    """config 1 (4^3, N=3, Roe, BR1, standard form): 100 CFL-bounded RK3 steps."""
+    """4^3 box, N=3, Roe, BR1, standard form: 100 CFL-bounded RK3 steps."""
     path = [p for p in GOLDEN if "c1_tgv" in p][0]
     case, meta = load_golden(path)
     o = make_oracle(case, meta)
