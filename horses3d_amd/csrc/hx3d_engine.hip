@@ -601,7 +601,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
     if (ev) (void)hipEventRecord(ev[0], eng->stream);
     hxk::DtLimits lim = dta.lim;
     if (!eng->ph.euler) {
-        hipLaunchKernelGGL(hxm::km_face_grad, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
+        hipLaunchKernelGGL(hxm::km_face_grad, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
                            eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
                            eng->S, eng->ES, eng->FSm);
         buckets_fork(eng);
@@ -617,7 +617,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         buckets_join(eng);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
-    hipLaunchKernelGGL(hxm::km_face, dim3(eng->F), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
+    hipLaunchKernelGGL(hxm::km_face, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
                        eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll, eng->dFe, eng->F, eng->S,
                        eng->ES, eng->FSm, eng->ph, dta.red);
     if (ev) (void)hipEventRecord(ev[2], eng->stream);

@@ -223,7 +223,9 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     __shared__ double sT[2][NV * 64];
     __shared__ double sU[3 * NV][64];
     __shared__ double sM[2][4][64];
-    const int f = blockIdx.x, lane = threadIdx.x;
+    // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
+    // mostly run on one XCD and find the elements' traces in its L2
+    const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
@@ -369,7 +371,9 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     __shared__ double sQ[2][NV * 64];
     __shared__ double sG[3 * NV * 64];
     __shared__ double sM[2][4][64];
-    const int f = blockIdx.x, lane = threadIdx.x;
+    // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
+    // mostly run on one XCD and find the elements' traces in its L2
+    const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
     if (dtr.bits && f == 0) hxk::reduce_dt_partials(dtr.part, dtr.n, dtr.bits, lane);
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
