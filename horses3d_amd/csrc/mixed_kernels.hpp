@@ -220,8 +220,10 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
              const int *__restrict__ efoff, const FaceRec *__restrict__ faces, const double *__restrict__ fgeo,
              const OpsAll *__restrict__ opsp, double *__restrict__ lift, int F, long S, long ES, long FSm)
 {
+    // LDS: both sides' Q traces, one direction's unStar, the mortar matrices (11.5 KB: the
+    // block count per CU is then bound by registers, not LDS)
     __shared__ double sT[2][NV * 64];
-    __shared__ double sU[3 * NV][64];
+    __shared__ double sU[NV][64];
     __shared__ double sM[2][4][64];
     // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
@@ -233,31 +235,38 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     gather_trace_k<NV>(r, 1, N3, off, Q, S, sT[1], lane);
     __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
+    double uS[NV], nrm[3];
     if (lane < nq) {
         const int i = lane % (n0 + 1), j = lane / (n0 + 1);
         const long fq = r.foff + lane;
-        const double nx = fgeo[fq], ny = fgeo[FSm + fq], nz = fgeo[2 * FSm + fq], jac = fgeo[9 * FSm + fq];
+        nrm[0] = fgeo[fq];
+        nrm[1] = fgeo[FSm + fq];
+        nrm[2] = fgeo[2 * FSm + fq];
+        const double jac = fgeo[9 * FSm + fq];
         double qL[NV], qR[NV];
         adapt_k<NV>(sM[0], r, 0, sT[0], i, j, qL);
         adapt_k<NV>(sM[1], r, 1, sT[1], i, j, qR);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double uS = 0.5 * (qR[v] - qL[v]) * jac;
-            sU[0 * NV + v][lane] = uS * nx;
-            sU[1 * NV + v][lane] = uS * ny;
-            sU[2 * NV + v][lane] = uS * nz;
-        }
+        for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
     }
-    __syncthreads();
-    for (int side = 0; side < 2; ++side) {
-        const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
-        if (lane < (N0 + 1) * (N1 + 1)) {
-            const int i = lane % (N0 + 1), j = lane / (N0 + 1);
-            const long x = elem_face_node(r, side, efoff, i, j);
-            double lv[3 * NV];
-            project_k<3 * NV>(sM[side], r, side, sU[0], i, j, lv);
+    // unStar(v) n_d, one direction at a time, projected onto both sides: lift[(5d + v)*ES + x]
+    for (int d = 0; d < 3; ++d) {
+        if (d) __syncthreads();   // the previous direction's projections have read sU
+        if (lane < nq) {
 #pragma unroll
-            for (int c = 0; c < 3 * NV; ++c) lift[c * ES + x] = lv[c];
+            for (int v = 0; v < NV; ++v) sU[v][lane] = uS[v] * nrm[d];
+        }
+        __syncthreads();
+        for (int side = 0; side < 2; ++side) {
+            const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
+            if (lane < (N0 + 1) * (N1 + 1)) {
+                const int i = lane % (N0 + 1), j = lane / (N0 + 1);
+                const long x = elem_face_node(r, side, efoff, i, j);
+                double lv[NV];
+                project_k<NV>(sM[side], r, side, sU[0], i, j, lv);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) lift[(d * NV + v) * ES + x] = lv[v];
+            }
         }
     }
 }
