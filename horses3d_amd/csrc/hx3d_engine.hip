@@ -649,10 +649,12 @@ void set_physics(hx3d_engine *eng, const hx3d_case_desc *d)
 {
     Phys &ph = eng->ph;
     ph.gamma = 1.4; ph.gm1 = 1.4 - 1.0;
-    ph.gammaM2 = ph.gamma * d->mach * d->mach;
+    // gamma * POW2(Mach) and 1 / (gm1 * POW2(Mach) * Pr) with the square formed first, as the
+    // reference does (PhysicsStorage_NS.f90:250, 285): (gamma * M) * M rounds differently
+    ph.gammaM2 = ph.gamma * (d->mach * d->mach);
     ph.euler = d->euler ? 1 : 0;
     ph.mu = d->euler ? 0.0 : 1.0 / d->reynolds;
-    ph.mu_to_kappa = d->euler ? 0.0 : 1.0 / (ph.gm1 * d->mach * d->mach * d->prandtl);
+    ph.mu_to_kappa = d->euler ? 0.0 : 1.0 / (ph.gm1 * (d->mach * d->mach) * d->prandtl);
     const double Tref = 520.0 * 5.0 / 9.0, Ssu = 198.6 * 5.0 / 9.0;
     ph.S_div_T = Ssu / Tref;
     ph.Trenorm = Tref / Tref;
