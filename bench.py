@@ -42,10 +42,44 @@ def alg_bytes(N):
                 stage=600.0 + 512.0 * fpn)
 
 
+def host_cpu():
+    """(model name, physical cores, logical CPUs) of this host from /proc/cpuinfo."""
+    model, cores, logical = "unknown", set(), 0
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                model = v
+                logical += 1
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return model, len(cores) or logical, logical
+
+
+def cpu_threads():
+    """The threads this process may run: its CPU affinity, capped by OMP_NUM_THREADS when the
+    environment sets it (a GPU box grants each GPU a share of its cores that way)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_baseline_reference(n, N, steps, threads):
-    """Time the reference solver (oracle/_ref/bin/horses3d.ns) on the same box, its own
-    'Solver efficiency' line (seconds per million DOF and iteration, main.f90:404-409)."""
-    exe = os.path.join(REPO, "oracle", "_ref", "bin", "horses3d.ns")
+    """Time the reference solver on the same box: the -O3 build oracle/_ref/bin/horses3d_O3.ns
+    (the -O2 golden-vector build if that is absent), its own 'Solver efficiency' line (seconds
+    per million DOF and iteration, main.f90:404-409)."""
+    exe, flags = os.path.join(REPO, "oracle", "_ref", "bin", "horses3d_O3.ns"), "-O3"
+    if not os.path.exists(exe):
+        exe, flags = os.path.join(REPO, "oracle", "_ref", "bin", "horses3d.ns"), "-O2"
     if not os.path.exists(exe):
         return None
     sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
@@ -57,17 +91,22 @@ def cpu_baseline_reference(n, N, steps, threads):
                      riemann="lax-friedrichs", inviscid="split-form", averaging="Pirozzoli",
                      ctd_after=False)
     t0 = time.time()
-    out = rc.run_reference(work, threads=threads, timeout=900)
+    out = rc.run_reference(work, threads=threads, timeout=900, exe=exe)
     wall = time.time() - t0
     m = re.search(r"Solver efficiency:\s*([0-9.Ee+-]+)", out)
     if not m:
         return None
     eff = float(m.group(1))   # seconds / (1e6 DOF * iteration)
+    model, phys, logical = host_cpu()
     return dict(value=1e6 / eff, unit="DOF-updates/s", cores=threads, kind="reference",
-                sample="reference OpenMP build (amdflang -O2 from the reference sources), "
-                       "%d^3 straight periodic box, N=%d, split-form Pirozzoli + LxF + BR1, %d RK3 steps, "
-                       "OMP_NUM_THREADS=%d; its own 'Solver efficiency' line (%.3e s/(MDOF*iter)), "
-                       "wall %.1f s incl. setup" % (n, N, steps, threads, eff, wall))
+                cpu=model, host_physical_cores=phys, host_logical_cpus=logical,
+                sample="reference OpenMP build (amdflang %s -fopenmp from the reference sources) on %s, "
+                       "%d threads (OMP_NUM_THREADS=%d, OMP_SCHEDULE=static: this process's CPU share; the host "
+                       "has %d physical cores), %d^3 straight periodic box, N=%d, split-form Pirozzoli + LxF + BR1, "
+                       "%d RK3 steps; a straight box does the same per-step work as the GPU line's curvilinear one "
+                       "(the map changes no size, no loop and no flux); its own 'Solver efficiency' line "
+                       "(%.3e s/(MDOF*iter)), wall %.1f s incl. setup"
+                       % (flags, model, threads, threads, phys, n, N, steps, eff, wall))
 
 
 def cpu_baseline_port(case, kw, steps):
@@ -202,7 +241,8 @@ def main():
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--amplitude", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: this process's CPU share, see cpu_threads)")
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 5 times (the "
@@ -337,7 +377,7 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         try:
-            cpu = cpu_baseline_reference(args.n, args.order, args.cpu_steps, args.cpu_threads)
+            cpu = cpu_baseline_reference(args.n, args.order, args.cpu_steps, args.cpu_threads or cpu_threads())
         except Exception as e:   # the reference build is optional on a box
             print("[bench] reference CPU baseline failed: %s" % e, file=sys.stderr, flush=True)
         if cpu is None:

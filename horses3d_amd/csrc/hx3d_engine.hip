@@ -320,6 +320,10 @@ __global__ void k_fill(double *__restrict__ X, long n, double val)
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) X[g] = val;
 }
+__global__ void k_fill_int(int *__restrict__ X, int n, int val)
+{
+    if ((int)threadIdx.x < n) X[threadIdx.x] = val;
+}
 
 // ------------------------------------------------------------------------------------
 // host side: spectral operators (restated from NodalStorageClass.f90 for the engine)
@@ -394,6 +398,13 @@ struct hx3d_engine {
     std::vector<int> N3h;   // element orders (Nx, Ny, Nz), host copy for solution files
     double mach = 0;
     bool qdot_valid = false, timing = false;
+    // CTD_AFTER_STEPS (ExplicitMethods.f90:784): the time derivative is recomputed at the state a
+    // step leaves, instead of keeping the last stage's
+    bool ctd_after = false;
+    // dFlag[0]: hx3d_check_nan's scan; dFlag[1]: sticky divergence flag set by the RK update
+    // kernels when a NaN appears.  hFlag: pinned host copy of dFlag[1], refreshed in stream
+    // order after each queued step loop (read by hx3d_query once the stream is idle).
+    int *hFlag = nullptr;
     long dev_bytes = 0;   // device memory the engine holds
     // hx3d_advance's step as a HIP graph, captured for one (cfl, dcfl) and replayed per step
     // (off by default: replays measured 2-5% slower than the same launches issued directly)
@@ -483,6 +494,7 @@ void launch_volume(hx3d_engine *eng, double a_k, double c_k, int update, const D
     case HX3D_AVG_DUCROS: return launch_volume_f<N, hxk::AVG_DUCROS>(eng, a_k, c_k, update, dta, qdot);
     case HX3D_AVG_KENNEDYGRUBER: return launch_volume_f<N, hxk::AVG_KG>(eng, a_k, c_k, update, dta, qdot);
     case HX3D_AVG_PIROZZOLI: return launch_volume_f<N, hxk::AVG_PIROZZOLI>(eng, a_k, c_k, update, dta, qdot);
+    case HX3D_AVG_MORINISHI: return launch_volume_f<N, hxk::AVG_MORINISHI>(eng, a_k, c_k, update, dta, qdot);
     default: return launch_volume_f<N, hxk::AVG_STANDARD>(eng, a_k, c_k, update, dta, qdot);
     }
 }
@@ -626,6 +638,7 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         case HX3D_AVG_DUCROS: launch_mvolume<hxk::AVG_DUCROS>(eng, a_k, c_k, update, dta, qdot); break;
         case HX3D_AVG_KENNEDYGRUBER: launch_mvolume<hxk::AVG_KG>(eng, a_k, c_k, update, dta, qdot); break;
         case HX3D_AVG_PIROZZOLI: launch_mvolume<hxk::AVG_PIROZZOLI>(eng, a_k, c_k, update, dta, qdot); break;
+        case HX3D_AVG_MORINISHI: launch_mvolume<hxk::AVG_MORINISHI>(eng, a_k, c_k, update, dta, qdot); break;
         default: launch_mvolume<hxk::AVG_STANDARD>(eng, a_k, c_k, update, dta, qdot); break;
         }
     if (ev) (void)hipEventRecord(ev[3], eng->stream);
@@ -643,6 +656,27 @@ void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long 
                            eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dDtPart);
     hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, eng->stream, eng->dDtPart, (int)nb, bits);
 }
+
+// the divergence flag's pinned host copy, and both device flags cleared
+int init_flags(hx3d_engine *eng)
+{
+    if (hipHostMalloc((void **)&eng->hFlag, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return eng->fail(HX3D_ENOMEM, "hipHostMalloc failed");
+    *eng->hFlag = 0;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, eng->stream, eng->dFlag, 2, 0);
+    return HX3D_OK;
+}
+
+// hFlag <- dFlag[1] in stream order (read by the host once the stream is idle)
+void queue_flag_readback(hx3d_engine *eng)
+{
+    (void)hipMemcpyAsync(eng->hFlag, eng->dFlag + 1, sizeof(int), hipMemcpyDeviceToHost, eng->stream);
+}
+
+// After a queued step loop: the divergence flag's read-back, and the time derivative the
+// reference's storage holds after TakeRK3Step -- the last stage's (already written to dQDot by
+// the caller's last stage), or with CTD_AFTER_STEPS the one of the new state.
+void after_steps(hx3d_engine *eng);
 
 // physics constants (PhysicsStorage_NS.f90:145-439, ThermodynamicsAir :115-132)
 void set_physics(hx3d_engine *eng, const hx3d_case_desc *d)
@@ -806,7 +840,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         (rc = dalloc(eng, &eng->dPart, (size_t)NV * ((S + 255) / 256 + 1))) ||
         (rc = dalloc(eng, &eng->dDtPart, (size_t)2 * (std::max((long)K, (S + 255) / 256) + 64))) ||
         (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
-        (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dN3, N3.size())) ||
+        (rc = dalloc(eng, &eng->dFlag, 2)) || (rc = dalloc(eng, &eng->dN3, N3.size())) ||
         (rc = dalloc(eng, &eng->dOff, off.size())) || (rc = dalloc(eng, &eng->dEfoff, efoff.size())) ||
         (rc = dalloc(eng, &eng->dElist, elist.size())) || (rc = dalloc(eng, &eng->dEofn, eofn.size())) ||
         (rc = dalloc(eng, &eng->dFaces, faces.size())) || (rc = dalloc(eng, &eng->dOpsAll, 1)))
@@ -836,6 +870,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dG, (long)NV * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3((15 * S + 255) / 256), dim3(256), 0, eng->stream, eng->dU, 15L * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, eng->stream, eng->dScal, 4L, 0.0);
+    if (int rc2 = init_flags(eng)) return rc2;
     if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "initialisation failed");
     return HX3D_OK;
 }
@@ -844,6 +879,17 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
 const double RK_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 const double RK_C[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
+}  // namespace
+
+extern "C" int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host);
+
+namespace {
+void after_steps(hx3d_engine *eng)
+{
+    eng->qdot_valid = true;
+    if (eng->ctd_after) (void)hx3d_compute_qdot(eng, 0.0, nullptr);
+    queue_flag_readback(eng);
+}
 }  // namespace
 
 // ------------------------------------------------------------------------------------
@@ -875,8 +921,6 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     if (d->riemann < 0 || d->riemann > 1 || d->inviscid < 0 || d->inviscid > 1 || d->averaging < 0 ||
         d->averaging > 4 || d->les < 0 || d->les > 2)
         return bail(eng->fail(HX3D_EINVAL, "descriptor: selector out of range"));
-    if (d->inviscid == HX3D_INVISCID_SPLIT && d->averaging == HX3D_AVG_MORINISHI)
-        return bail(eng->fail(HX3D_EUNSUPPORTED, "split form with Morinishi averaging is not implemented"));
     if (d->mach <= 0 || (!d->euler && d->reynolds <= 0))
         return bail(eng->fail(HX3D_EINVAL, "descriptor: mach and reynolds must be positive"));
     set_physics(eng, d);
@@ -1016,7 +1060,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         (rc = dalloc(eng, &eng->dDtPart, (size_t)2 * (std::max((long)K, (S + 255) / 256) + 64))) ||
         (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
         (rc = dalloc(eng, &eng->dNbr, nbr.size())) ||
-        (rc = dalloc(eng, &eng->dFlag, 1)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
+        (rc = dalloc(eng, &eng->dFlag, 2)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
         return bail(rc);
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(eng->fail(HX3D_EDEVICE, "hipStreamCreate failed"));
@@ -1034,6 +1078,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, (long)NV * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dG, (long)NV * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, eng->stream, eng->dScal, 4L, 0.0);
+    if (int rc2 = init_flags(eng)) return bail(rc2);
     if (hipStreamSynchronize(eng->stream) != hipSuccess)
         return bail(eng->fail(HX3D_EDEVICE, "initialisation failed"));
     *out = eng;
@@ -1058,6 +1103,7 @@ int hx3d_engine_destroy(hx3d_engine *eng)
         if (eng->join_ev[k]) (void)hipEventDestroy(eng->join_ev[k]);
     }
     if (eng->fork_ev) (void)hipEventDestroy(eng->fork_ev);
+    if (eng->hFlag) (void)hipHostFree(eng->hFlag);
     if (eng->stream) (void)hipStreamDestroy(eng->stream);
     delete eng;
     return HX3D_OK;
@@ -1071,6 +1117,8 @@ int hx3d_set_state_device(hx3d_engine *eng, const double *Qd)
     // a new state restarts the RK register (the reference allocates G_NS zeroed)
     const long nb2 = (NV * eng->S + 255) / 256;
     hipLaunchKernelGGL(k_fill, dim3(nb2), dim3(256), 0, eng->stream, eng->dG, (long)NV * eng->S, 0.0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, eng->stream, eng->dFlag + 1, 1, 0);
+    queue_flag_readback(eng);
     eng->qdot_valid = false;
     HIPCHK(hipGetLastError());
     return HX3D_OK;
@@ -1151,10 +1199,44 @@ int hx3d_rk3_step(hx3d_engine *eng, double t, double dt)
 {
     (void)t;
     if (!eng) return HX3D_EINVAL;
-    const DtArgs a = host_dt(eng, dt);
+    DtArgs a = host_dt(eng, dt);
+    a.nan = eng->dFlag + 1;
     for (int k = 0; k < 3; ++k) stage(eng, RK_A[k], RK_C[k], 1, a, k == 2 ? eng->dQDot : nullptr);
-    eng->qdot_valid = false;
+    after_steps(eng);
     HIPCHK(hipGetLastError());
+    return HX3D_OK;
+}
+
+int hx3d_set_ctd_after_steps(hx3d_engine *eng, int on)
+{
+    if (!eng) return HX3D_EINVAL;
+    eng->ctd_after = on != 0;
+    return HX3D_OK;
+}
+
+int hx3d_get_qdot(hx3d_engine *eng, double *QDot_host)
+{
+    if (!eng || !QDot_host) return HX3D_EINVAL;
+    if (!eng->qdot_valid) {
+        int rc = hx3d_compute_qdot(eng, 0.0, nullptr);
+        if (rc) return rc;
+    }
+    const long nb = (eng->nodes + 255) / 256;
+    hipLaunchKernelGGL(k_pack_out, dim3(nb), dim3(256), 0, eng->stream, eng->dQDot, eng->dStage, eng->nodes, eng->S);
+    HIPCHK(hipMemcpyAsync(QDot_host, eng->dStage, sizeof(double) * NV * eng->nodes, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+// The end of a step loop: with t_inout, wait, return the time, and report a divergence
+// (checkForNan, ExplicitMethods.f90:786: the reference stops there) as HX3D_ENAN.
+static int finish_steps(hx3d_engine *eng, double *t_inout)
+{
+    HIPCHK(hipGetLastError());
+    if (!t_inout) return HX3D_OK;
+    HIPCHK(hipMemcpyAsync(t_inout, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    if (*eng->hFlag) return eng->fail(HX3D_ENAN, "numerical divergence: a NaN appeared in the state");
     return HX3D_OK;
 }
 
@@ -1162,20 +1244,17 @@ int hx3d_advance_dt(hx3d_engine *eng, int nsteps, double dt, double *t_inout)
 {
     if (!eng || nsteps < 0 || !(dt > 0.0)) return HX3D_EINVAL;
     if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
-    const DtArgs a = host_dt(eng, dt);
+    DtArgs a = host_dt(eng, dt);
+    a.nan = eng->dFlag + 1;
     for (int s = 0; s < nsteps; ++s)
         for (int k = 0; k < 3; ++k) {
             DtArgs b = a;
             if (k == 2) b.time = eng->dScal + 1;   // t += dt once per step, on the device
-            stage(eng, RK_A[k], RK_C[k], 1, b, nullptr);
+            stage(eng, RK_A[k], RK_C[k], 1, b, k == 2 && s == nsteps - 1 ? eng->dQDot : nullptr);
         }
     HIPCHK(hipGetLastError());
-    eng->qdot_valid = false;
-    if (t_inout) {
-        HIPCHK(hipMemcpyAsync(t_inout, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
-        HIPCHK(hipStreamSynchronize(eng->stream));
-    }
-    return HX3D_OK;
+    if (nsteps > 0) after_steps(eng);
+    return finish_steps(eng, t_inout);
 }
 
 int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt)
@@ -1202,7 +1281,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
     // a standalone pass writes them), which block 0 of the first stage's face kernel reduces into
     // the slot before any volume kernel of the step reads dt.  Every step issues the same
     // launches with the same arguments: ev, when given, holds 12 events for the step.
-    auto issue_step = [eng, cfl, dcfl](hipEvent_t *ev) {
+    auto issue_step = [eng, cfl, dcfl](hipEvent_t *ev, double *qdot) {
         unsigned long long *slot = eng->dBits;
         int nparts;
         if (eng->ph.euler) {
@@ -1226,11 +1305,13 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
                 a.red = hxk::DtReduce{eng->dDtPart, nparts, slot};
             }
             if (k == 2) a.time = eng->dScal + 1;
-            stage(eng, RK_A[k], RK_C[k], 1, a, nullptr, ev ? ev + 4 * k : nullptr);
+            a.nan = eng->dFlag + 1;
+            stage(eng, RK_A[k], RK_C[k], 1, a, k == 2 ? qdot : nullptr, ev ? ev + 4 * k : nullptr);
         }
     };
     if (eng->timing || !eng->use_graph) {
-        for (int s = 0; s < nsteps; ++s) issue_step(eng->timing ? &eng->evpool[(size_t)12 * s] : nullptr);
+        for (int s = 0; s < nsteps; ++s)
+            issue_step(eng->timing ? &eng->evpool[(size_t)12 * s] : nullptr, s == nsteps - 1 ? eng->dQDot : nullptr);
     } else if (nsteps > 0) {
         // capture the step once per (cfl, dcfl); the graph holds kernel launches only
         if (!eng->step_graph || eng->graph_cfl != cfl || eng->graph_dcfl != dcfl) {
@@ -1241,7 +1322,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
             eng->capturing = true;
-            issue_step(nullptr);
+            issue_step(nullptr, nullptr);
             eng->capturing = false;
             const hipError_t ce = hipStreamEndCapture(eng->stream, &g);
             if (ce != hipSuccess || !g) return eng->fail(HX3D_EDEVICE, std::string("step capture: ") + hipGetErrorString(ce));
@@ -1254,7 +1335,9 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
             eng->graph_cfl = cfl;
             eng->graph_dcfl = dcfl;
         }
-        for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(eng->step_graph, eng->stream));
+        // the last step is issued directly: it also leaves its last stage's time derivative
+        for (int s = 0; s + 1 < nsteps; ++s) HIPCHK(hipGraphLaunch(eng->step_graph, eng->stream));
+        issue_step(nullptr, eng->dQDot);
     }
     HIPCHK(hipGetLastError());
     if (eng->timing && nsteps > 0) {
@@ -1270,12 +1353,8 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
         for (int x = 0; x < 3; ++x) eng->ktime[x] = acc[x] / (3.0 * nsteps);
         eng->ktime[3] = (eng->ktime[0] + eng->ktime[1] + eng->ktime[2]);
     }
-    eng->qdot_valid = false;
-    if (t_inout) {
-        HIPCHK(hipMemcpyAsync(t_inout, eng->dScal + 1, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
-        HIPCHK(hipStreamSynchronize(eng->stream));
-    }
-    return HX3D_OK;
+    if (nsteps > 0) after_steps(eng);
+    return finish_steps(eng, t_inout);
 }
 
 int hx3d_residuals(hx3d_engine *eng, double res[5])
@@ -1359,6 +1438,7 @@ int hx3d_query(hx3d_engine *eng, int *done)
     if (e == hipSuccess) *done = 1;
     else if (e == hipErrorNotReady) *done = 0;
     else return eng->fail(HX3D_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    if (*done && *eng->hFlag) return eng->fail(HX3D_ENAN, "numerical divergence: a NaN appeared in the state");
     return HX3D_OK;
 }
 

@@ -668,6 +668,8 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
                 q[v] = q[v] + cdt * gv;
                 Q[v * S + g] = q[v];
             }
+            if (dta.nan && (isnan(q[0]) || isnan(q[1]) || isnan(q[2]) || isnan(q[3]) || isnan(q[4])))
+                atomicOr(dta.nan, 1);
         }
     }
 }

@@ -190,55 +190,59 @@ HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], con
     return 0.0;
 }
 
-// navierstokes/RiemannSolvers_NS.f90:1541-1656 RoeRiemannSolver
+// navierstokes/RiemannSolvers_NS.f90:1541-1656 RoeRiemannSolver: the one-sided Roe flux with
+// the entropy fix of the one wave that can change sign, the reference's operation order.
 HX_HD void roe_flux(const Phys &ph, const double L[NV], const double R[NV], const double n[3], double f[NV])
 {
     const double g = ph.gamma;
-    const double rho = L[0], rhou = L[1], rhov = L[2], rhow = L[3], rhoe = L[4];
-    const double rhon = R[0], rhoun = R[1], rhovn = R[2], rhown = R[3], rhoen = R[4];
-    const double ul = rhou / rho, vl = rhov / rho, wl = rhow / rho;
-    const double pleft = (g - 1.0) * (rhoe - 0.5 / rho * (rhou * rhou + rhov * rhov + rhow * rhow));
-    const double ur = rhoun / rhon, vr = rhovn / rhon, wr = rhown / rhon;
-    const double pright = (g - 1.0) * (rhoen - 0.5 / rhon * (rhoun * rhoun + rhovn * rhovn + rhown * rhown));
-    const double ql = n[0] * ul + n[1] * vl + n[2] * wl;
-    const double qr = n[0] * ur + n[1] * vr + n[2] * wr;
-    const double hl = 0.5 * (ul * ul + vl * vl + wl * wl) + g / (g - 1.0) * pleft / rho;
-    const double hr = 0.5 * (ur * ur + vr * vr + wr * wr) + g / (g - 1.0) * pright / rhon;
-    const double rtd = sqrt(rho * rhon);
-    const double betal = rho / (rho + rtd), betar = 1.0 - betal;
-    const double utd = betal * ul + betar * ur, vtd = betal * vl + betar * vr, wtd = betal * wl + betar * wr;
-    const double htd = betal * hl + betar * hr;
-    const double atd2 = (g - 1.0) * (htd - 0.5 * (utd * utd + vtd * vtd + wtd * wtd));
-    const double atd = sqrt(atd2);
-    const double qtd = utd * n[0] + vtd * n[1] + wtd * n[2];
-    if (qtd >= 0.0) {
-        const double dw1 = 0.5 * ((pright - pleft) / atd2 - (qr - ql) * rtd / atd);
-        const double sp1 = qtd - atd;
-        const double sp1m = sp1 < 0.0 ? sp1 : 0.0;
-        const double hd1m = ((g + 1.0) / 4.0 * atd / rtd) * dw1;
-        const double e1 = -fabs(sp1) - hd1m;
-        const double eta1 = e1 > 0.0 ? e1 : 0.0;
-        const double udw1 = dw1 * (sp1m - 0.5 * eta1);
-        const double rql = rho * ql;
-        f[0] = rql + udw1;
-        f[1] = rql * ul + pleft * n[0] + udw1 * (utd - atd * n[0]);
-        f[2] = rql * vl + pleft * n[1] + udw1 * (vtd - atd * n[1]);
-        f[3] = rql * wl + pleft * n[2] + udw1 * (wtd - atd * n[2]);
-        f[4] = rql * hl + udw1 * (htd - qtd * atd);
+    const double rL = L[0], mxL = L[1], myL = L[2], mzL = L[3], EL = L[4];
+    const double rR = R[0], mxR = R[1], myR = R[2], mzR = R[3], ER = R[4];
+    const double uL = mxL / rL, vL = myL / rL, wL = mzL / rL;
+    const double pL = (g - 1.0) * (EL - 0.5 / rL * (mxL * mxL + myL * myL + mzL * mzL));
+    const double uR = mxR / rR, vR = myR / rR, wR = mzR / rR;
+    const double pR = (g - 1.0) * (ER - 0.5 / rR * (mxR * mxR + myR * myR + mzR * mzR));
+    const double unL = n[0] * uL + n[1] * vL + n[2] * wL;
+    const double unR = n[0] * uR + n[1] * vR + n[2] * wR;
+    const double HL = 0.5 * (uL * uL + vL * vL + wL * wL) + g / (g - 1.0) * pL / rL;
+    const double HR = 0.5 * (uR * uR + vR * vR + wR * wR) + g / (g - 1.0) * pR / rR;
+    // Roe averages: weights sqrt(rho) of each side
+    const double srr = sqrt(rL * rR);
+    const double wtL = rL / (rL + srr), wtR = 1.0 - wtL;
+    const double ua = wtL * uL + wtR * uR, va = wtL * vL + wtR * vR, wa = wtL * wL + wtR * wR;
+    const double Ha = wtL * HL + wtR * HR;
+    const double c2 = (g - 1.0) * (Ha - 0.5 * (ua * ua + va * va + wa * wa));
+    const double ca = sqrt(c2);
+    const double una = ua * n[0] + va * n[1] + wa * n[2];
+    if (una >= 0.0) {
+        // left state plus the u - c wave (amplitude a1, speed lam1, entropy fix fix1)
+        const double a1 = 0.5 * ((pR - pL) / c2 - (unR - unL) * srr / ca);
+        const double lam1 = una - ca;
+        const double lam1n = lam1 < 0.0 ? lam1 : 0.0;
+        const double fix1 = ((g + 1.0) / 4.0 * ca / srr) * a1;
+        const double x1 = -fabs(lam1) - fix1;
+        const double eps1 = x1 > 0.0 ? x1 : 0.0;
+        const double s1 = a1 * (lam1n - 0.5 * eps1);
+        const double mn = rL * unL;
+        f[0] = mn + s1;
+        f[1] = mn * uL + pL * n[0] + s1 * (ua - ca * n[0]);
+        f[2] = mn * vL + pL * n[1] + s1 * (va - ca * n[1]);
+        f[3] = mn * wL + pL * n[2] + s1 * (wa - ca * n[2]);
+        f[4] = mn * HL + s1 * (Ha - una * ca);
     } else {
-        const double dw4 = 0.5 * ((pright - pleft) / atd2 + (qr - ql) * rtd / atd);
-        const double sp4 = qtd + atd;
-        const double sp4p = sp4 > 0.0 ? sp4 : 0.0;
-        const double hd4 = ((g + 1.0) / 4.0 * atd / rtd) * dw4;
-        const double e4 = -fabs(sp4) + hd4;
-        const double eta4 = e4 > 0.0 ? e4 : 0.0;
-        const double udw4 = dw4 * (sp4p + 0.5 * eta4);
-        const double rqr = rhon * qr;
-        f[0] = rqr - udw4;
-        f[1] = rqr * ur + pright * n[0] - udw4 * (utd + atd * n[0]);
-        f[2] = rqr * vr + pright * n[1] - udw4 * (vtd + atd * n[1]);
-        f[3] = rqr * wr + pright * n[2] - udw4 * (wtd + atd * n[2]);
-        f[4] = rqr * hr - udw4 * (htd + qtd * atd);
+        // right state minus the u + c wave
+        const double a4 = 0.5 * ((pR - pL) / c2 + (unR - unL) * srr / ca);
+        const double lam4 = una + ca;
+        const double lam4p = lam4 > 0.0 ? lam4 : 0.0;
+        const double fix4 = ((g + 1.0) / 4.0 * ca / srr) * a4;
+        const double x4 = -fabs(lam4) + fix4;
+        const double eps4 = x4 > 0.0 ? x4 : 0.0;
+        const double s4 = a4 * (lam4p + 0.5 * eps4);
+        const double mn = rR * unR;
+        f[0] = mn - s4;
+        f[1] = mn * uR + pR * n[0] - s4 * (ua + ca * n[0]);
+        f[2] = mn * vR + pR * n[1] - s4 * (va + ca * n[1]);
+        f[3] = mn * wR + pR * n[2] - s4 * (wa + ca * n[2]);
+        f[4] = mn * HR - s4 * (Ha + una * ca);
     }
 }
 
@@ -329,7 +333,7 @@ HX_HD void lxf_flux(const Phys &ph, const double QL[NV], const double QR[NV], co
 // and are applied to the node data and to the metric rows (metric_scale) instead: scaling by a
 // power of two commutes with rounding, so 0.5 (a + b) and a/2 + b/2 round to the same double
 // and every flux keeps the reference's bits, with 9-18 fewer multiplications per pair.
-constexpr int node_data_size(int avg) { return avg == 0 ? 8 : avg == 1 ? 9 : 6; }
+constexpr int node_data_size(int avg) { return avg == 0 || avg == 4 ? 8 : avg == 1 ? 9 : 6; }
 constexpr double metric_scale(int avg) { return avg == 0 ? 0.25 : 0.5; }
 
 template <int AVG>
@@ -341,6 +345,9 @@ HX_HD void node_data(const Phys &ph, const double q[NV], double *a)
     if (AVG == 0) {          // rho u, rho v, rho w, rho e, u, v, w, p
         a[0] = q[1]; a[1] = q[2]; a[2] = q[3]; a[3] = q[4];
         a[4] = u; a[5] = v; a[6] = w; a[7] = p;
+    } else if (AVG == 4) {   // rho u, rho v, rho w, u, v, w, p, h = cp p (no kinetic energy)
+        a[0] = q[1]; a[1] = q[2]; a[2] = q[3];
+        a[3] = u; a[4] = v; a[5] = w; a[6] = p; a[7] = ph.cp * p;
     } else if (AVG == 1) {   // q/4, u, v, w, p/4
         a[0] = 0.25 * q[0]; a[1] = 0.25 * q[1]; a[2] = 0.25 * q[2]; a[3] = 0.25 * q[3];
         a[4] = 0.25 * q[4]; a[5] = u; a[6] = v; a[7] = w; a[8] = 0.25 * p;
@@ -350,8 +357,9 @@ HX_HD void node_data(const Phys &ph, const double q[NV], double *a)
     }
 }
 
-// Kennedy-Gruber and Pirozzoli are symmetric in (a, b) in IEEE arithmetic; Standard and Ducros
-// are not, and the reference evaluates every pair with the lower node index as a.
+// Kennedy-Gruber, Pirozzoli and Morinishi are symmetric in (a, b) in IEEE arithmetic (every
+// term is a sum or product of one a-term and the matching b-term); Standard and Ducros are not,
+// and the reference evaluates every pair with the lower node index as a.
 constexpr bool two_point_symmetric(int avg) { return avg >= 2; }
 
 // a, b: node_data of the two nodes; ja, jb: their metric rows times metric_scale
@@ -361,7 +369,31 @@ HX_HD void two_point(const double *a, const double *ja, const double *b, const d
     double f[NV], g[NV], h[NV], J[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) J[d] = ja[d] + jb[d];
-    if (AVG == 0) {
+    if (AVG == 4) {
+        // Morinishi_TwoPointFlux (RiemannSolvers_NS.f90:2147-2229): direction d's flux from
+        // each side's rho u_d (m), with the reference's factors 1/2 and 1/4 in place
+        const double su = a[3] + b[3], sv = a[4] + b[4], sw = a[5] + b[5];
+        const double uL = a[3], vL = a[4], wL = a[5], uR = b[3], vR = b[4], wR = b[5];
+        const double pa = 0.5 * (a[6] + b[6]);
+        double *o3[3] = {f, g, h};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double mL = a[d], mR = b[d];
+            double *o = o3[d];
+            const double s = 0.25 * (mL + mR);
+            o[0] = 0.5 * (mL + mR);
+            o[1] = s * su;
+            o[2] = s * sv;
+            o[3] = s * sw;
+            o[1 + d] = o[1 + d] + pa;
+            o[4] = 0.5 * (a[3 + d] * a[7] + b[3 + d] * b[7]) + 0.25 * (mL * uL + mR * uR) * su
+                 + 0.25 * (mL * vL + mR * vR) * sv
+                 + 0.25 * (mL * wL + mR * wR) * sw
+                 - 0.25 * (mL * (uL * uL) + mR * (uR * uR))
+                 - 0.25 * (mL * (vL * vL) + mR * (vR * vR))
+                 - 0.25 * (mL * (wL * wL) + mR * (wR * wR));
+        }
+    } else if (AVG == 0) {
         const double uL = a[4], vL = a[5], wL = a[6], pL = a[7];
         const double uR = b[4], vR = b[5], wR = b[6], pR = b[7];
         f[0] = (a[0] + b[0]);

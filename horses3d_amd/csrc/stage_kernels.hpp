@@ -88,7 +88,7 @@ struct DtLimits {
 };
 
 // Averaging selectors of the templated split-form kernel (RiemannSolvers_NS.f90:229-285)
-enum { AVG_STANDARD = 0, AVG_DUCROS = 1, AVG_KG = 2, AVG_PIROZZOLI = 3, FORM_STANDARD = -1 };
+enum { AVG_STANDARD = 0, AVG_DUCROS = 1, AVG_KG = 2, AVG_PIROZZOLI = 3, AVG_MORINISHI = 4, FORM_STANDARD = -1 };
 
 // Element face node x = (6e + slot) N1^2 + a + N1 b (a, b the slot's face coordinates); the
 // face quantities the element kernels read are stored in this order, so an element reads its
@@ -334,6 +334,7 @@ k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const dou
 struct DtArgs {
     const unsigned long long *in;   // [2]: convective, viscous limit bit patterns of this step's dt
     double *time;                   // t += dt by one thread (or null)
+    int *nan;                       // set to 1 when an updated state value is NaN (or null)
     DtLimits lim;                   // consumed by the stage's gradient kernel
     DtReduce red;                   // consumed by the stage's face kernel
 };
@@ -652,6 +653,10 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
                 q[v] = q[v] + cdt * gv;
                 stg(Q + v * S, gw, q[v]);
             }
+            // checkForNan (ExplicitMethods.f90:1856-1905) of the updated state: a sticky flag,
+            // written only where a NaN appears
+            if (dta.nan && (isnan(q[0]) || isnan(q[1]) || isnan(q[2]) || isnan(q[3]) || isnan(q[4])))
+                atomicOr(dta.nan, 1);
         }
     }
 }

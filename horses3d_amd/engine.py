@@ -44,14 +44,27 @@ class ControlRun(C.Structure):
 
 
 class HX3DError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+class HX3DDiverged(HX3DError):
+    """HX3D_ENAN: a NaN appeared in the state (the reference's checkForNan stop)."""
+
+
+HX3D_ENAN = 5
+
+
+def _error(msg, code):
+    return (HX3DDiverged if code == HX3D_ENAN else HX3DError)(msg, code)
 
 
 _lib = None
 
 EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
            "hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device",
-           "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_max_dt", "hx3d_advance", "hx3d_advance_dt", "hx3d_residuals",
+           "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_set_ctd_after_steps", "hx3d_get_qdot", "hx3d_max_dt", "hx3d_advance", "hx3d_advance_dt", "hx3d_residuals",
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_query", "hx3d_get_time", "hx3d_engine_device_bytes",
            "hx3d_device_memory", "hx3d_set_timing", "hx3d_set_step_graph",
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
@@ -81,6 +94,8 @@ def load_library(path=LIB_PATH):
         getattr(L, fn).argtypes = [P, P]
     L.hx3d_compute_qdot.argtypes = [P, D, P]
     L.hx3d_rk3_step.argtypes = [P, D, D]
+    L.hx3d_set_ctd_after_steps.argtypes = [P, I]
+    L.hx3d_get_qdot.argtypes = [P, P]
     L.hx3d_max_dt.argtypes = [P, D, D, C.POINTER(D)]
     L.hx3d_advance.argtypes = [P, I, D, D, C.POINTER(D)]
     L.hx3d_advance_dt.argtypes = [P, I, D, C.POINTER(D)]
@@ -309,7 +324,7 @@ class Engine:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise HX3DError("%s: %s (code %d)" % (what, load_library().hx3d_last_error(self.h).decode(), rc))
+            raise _error("%s: %s (code %d)" % (what, load_library().hx3d_last_error(self.h).decode(), rc), rc)
 
     def close(self):
         if getattr(self, "h", None):
@@ -336,6 +351,17 @@ class Engine:
 
     def rk3_step(self, t, dt):
         self._chk(load_library().hx3d_rk3_step(self.h, t, dt), "hx3d_rk3_step")
+
+    def set_ctd_after_steps(self, on):
+        """CTD_AFTER_STEPS: recompute the time derivative at the state each step leaves (on), or
+        keep the last stage's (off, the reference's default)."""
+        self._chk(load_library().hx3d_set_ctd_after_steps(self.h, 1 if on else 0), "hx3d_set_ctd_after_steps")
+
+    def get_qdot(self):
+        """The current time derivative (after qdot() or a step), (nodes, 5)."""
+        out = np.empty((self.nodes, 5))
+        self._chk(load_library().hx3d_get_qdot(self.h, _p(out)), "hx3d_get_qdot")
+        return out
 
     def max_dt(self, cfl, dcfl):
         dt = C.c_double()

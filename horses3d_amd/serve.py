@@ -11,6 +11,11 @@ freed for the next arrival.  A case's result does not depend on what runs beside
 engine computes only from its own arrays, in its own stream order, so the state is the one
 the case reaches alone, bit for bit.
 
+A case that fails is retired alone and the others keep running: a case whose steps diverge (a NaN
+appears in its state; the engine's RK update flags it, as the reference's checkForNan does after
+every step, ExplicitMethods.f90:786) ends with its state at that point and `Result.error` set,
+and so does a case whose engine cannot be set up.
+
 `serve` returns per-case results and the aggregate figures (DOF-updates/s, p50/p99 latency
 from arrival to completion); `one_at_a_time` runs the same requests in arrival order, each
 alone on the plain engine, for comparison."""
@@ -36,16 +41,23 @@ class Request:
 
 
 class Result:
-    def __init__(self, req, Q, t, admitted, finished):
+    """The end of one request: final state Q and time t (the state where it stopped for a failed
+    case, or None if no engine was built), admission and completion times, and error (None, or
+    the message of the failure that retired the case)."""
+
+    def __init__(self, req, Q, t, admitted, finished, error=None):
         self.request, self.Q, self.t = req, Q, t
         self.admitted, self.finished = admitted, finished
         self.latency = finished - req.arrival
+        self.error = error
 
 
 def _stats(results, wall):
-    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
-    work = sum(r.request.nodes * r.request.steps for r in results)
-    return dict(cases=len(results), wall_s=wall, dof_updates=work, dof_updates_per_s=work / wall if wall > 0 else 0.0,
+    ok = [r for r in results if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    work = sum(r.request.nodes * r.request.steps for r in ok)
+    return dict(cases=len(ok), failed=len(results) - len(ok), wall_s=wall, dof_updates=work,
+                dof_updates_per_s=work / wall if wall > 0 else 0.0,
                 p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)))
 
 
@@ -83,9 +95,11 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
         return eng
 
     def finish(eng):
-        t = eng.time()
-        Q = eng.get_state()
-        eng.close()
+        try:
+            t = eng.time()
+            Q = eng.get_state()
+        finally:
+            eng.close()
         return t, Q
 
     pool = ThreadPoolExecutor(max_workers=max(1, workers)) if workers else None
@@ -106,6 +120,10 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
                 preparing[i] = (submit(prepare, r), need)
             for i in [k for k, (f, _) in preparing.items() if f.done()]:
                 f, _ = preparing.pop(i)
+                if f.exception() is not None:
+                    # the case cannot be set up: retired alone with the error
+                    results[i] = Result(requests[i], None, None, clock() - t0, clock() - t0, error=str(f.exception()))
+                    continue
                 active.append([i, f.result(), 0, clock() - t0])
             most = max(most, len(active))
             busy = bool(preparing or retiring)
@@ -114,13 +132,20 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
             running, idle = 0, []
             for slot in list(active):
                 i, eng, queued, admitted = slot
-                if not eng.done():
+                try:
+                    done = eng.done()
+                except E.HX3DDiverged as exc:
+                    # a NaN appeared in this case's state: it stops there, the others go on
+                    active.remove(slot)
+                    retiring[i] = (submit(finish, eng), admitted, str(exc))
+                    continue
+                if not done:
                     running += 1
                 elif queued < requests[i].steps:
                     idle.append(slot)
                 else:
                     active.remove(slot)
-                    retiring[i] = (submit(finish, eng), admitted)
+                    retiring[i] = (submit(finish, eng), admitted, None)
             idle.sort(key=lambda sl: (requests[sl[0]].steps - sl[2]) * requests[sl[0]].nodes)
             for slot in idle[:max(0, max_running - running)]:
                 i, eng, queued, _ = slot
@@ -129,10 +154,10 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
                 eng.advance_async(n, r.cfl, r.dcfl)
                 slot[2] = queued + n
             busy = busy or running > 0 or bool(idle)
-            for i in [k for k, (f, _) in retiring.items() if f.done()]:
-                f, admitted = retiring.pop(i)
+            for i in [k for k, v in retiring.items() if v[0].done()]:
+                f, admitted, err = retiring.pop(i)
                 t, Q = f.result()
-                results[i] = Result(requests[i], Q, t, admitted, clock() - t0)
+                results[i] = Result(requests[i], Q, t, admitted, clock() - t0, error=err)
             if not busy and pending and not active:
                 # nothing running: sleep until the next arrival
                 wait = requests[pending[0]].arrival - (clock() - t0)
@@ -160,23 +185,29 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
 
 
 class _Done:
-    """A finished result in the form of a future (set-up and retirement on the calling thread)."""
+    """A finished result or error in the form of a future (set-up and retirement on the calling
+    thread)."""
 
-    def __init__(self, v):
-        self.v = v
+    def __init__(self, v, exc=None):
+        self.v, self.exc = v, exc
 
     def done(self):
         return True
 
     def result(self):
+        if self.exc is not None:
+            raise self.exc
         return self.v
 
     def exception(self):
-        return None
+        return self.exc
 
 
 def _run_now(fn, *a):
-    return _Done(fn(*a))
+    try:
+        return _Done(fn(*a))
+    except Exception as exc:   # kept for the caller, as a pool's future keeps it
+        return _Done(None, exc)
 
 
 def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=time.sleep):
@@ -192,11 +223,21 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
         if wait > 0:
             sleep(wait)
         admitted = clock() - t0
-        eng = make(r)
-        eng.set_state(r.Q0)
-        t = eng.advance(r.steps, r.cfl, r.dcfl, t=0.0)
-        Q = eng.get_state()
-        eng.close()
-        results[i] = Result(r, Q, t, admitted, clock() - t0)
+        try:
+            eng = make(r)
+        except E.HX3DError as exc:
+            results[i] = Result(r, None, None, admitted, clock() - t0, error=str(exc))
+            continue
+        err = None
+        try:
+            eng.set_state(r.Q0)
+            try:
+                t = eng.advance(r.steps, r.cfl, r.dcfl, t=0.0)
+            except E.HX3DDiverged as exc:
+                t, err = eng.time(), str(exc)
+            Q = eng.get_state()
+        finally:
+            eng.close()
+        results[i] = Result(r, Q, t, admitted, clock() - t0, error=err)
     wall = clock() - t0
     return results, _stats(results, wall)

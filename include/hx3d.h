@@ -107,12 +107,28 @@ int hx3d_get_state_device(hx3d_engine *eng, double *Q_dev);
 /* Drop-in for ComputeTimeDerivative: QDot(Q, t) of the resident state.  QDot_host may be
  * NULL (the derivative then stays on the device for hx3d_residuals). */
 int hx3d_compute_qdot(hx3d_engine *eng, double t, double *QDot_host);
-/* Drop-in for TakeRK3Step: one Williamson RK3 step of size dt on the resident state. */
+/* Drop-in for TakeRK3Step: one Williamson RK3 step of size dt on the resident state.  Like the
+ * reference, it leaves the last stage's time derivative as the current QDot (hx3d_get_qdot,
+ * hx3d_residuals), or, with hx3d_set_ctd_after_steps(eng, 1), the derivative of the new state
+ * (the "compute time derivative after timestep" key, CTD_AFTER_STEPS, ExplicitMethods.f90:784).
+ * Queued on the engine stream; the RK update sets a sticky divergence flag when a NaN appears
+ * (checkForNan, ExplicitMethods.f90:786), reported by hx3d_query, hx3d_advance and
+ * hx3d_check_nan. */
 int hx3d_rk3_step(hx3d_engine *eng, double t, double dt);
+/* on = 1: after every step (hx3d_rk3_step, hx3d_advance, hx3d_advance_dt) the time derivative is
+ * recomputed at the new state (CTD_AFTER_STEPS, TimeIntegrator.f90 "compute time derivative
+ * after timestep"); on = 0 (the reference's default): the last stage's derivative is kept. */
+int hx3d_set_ctd_after_steps(hx3d_engine *eng, int on);
+/* The current time derivative (storage % QDot after ComputeTimeDerivative or a step) in the
+ * packed host layout; computed from the resident state if none is current. */
+int hx3d_get_qdot(hx3d_engine *eng, double *QDot_host);
 /* MaxTimeStep(cfl, dcfl) of the resident state. */
 int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt);
 /* The step loop: nsteps RK3 steps, each with dt = MaxTimeStep(cfl, dcfl) of the state at
- * the start of the step (computed on the device; no host round trip).  t_inout advances. */
+ * the start of the step (computed on the device; no host round trip).  t_inout advances.
+ * With t_inout != NULL the call waits for the steps and returns HX3D_ENAN when a NaN appeared
+ * in the state (the reference stops there, ExplicitMethods.f90:786 and :1856-1905); with
+ * t_inout = NULL it only queues them (see hx3d_query). */
 int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *t_inout);
 /* The step loop with a fixed step: nsteps RK3 steps of size dt (the .control file's "dt" key
  * instead of "cfl", TimeIntegrator.f90:196-198); t_inout advances by dt per step. */
@@ -122,7 +138,9 @@ int hx3d_advance_dt(hx3d_engine *eng, int nsteps, double dt, double *t_inout);
  * (the default): the same launches issued one by one, which measured 2-5% faster on MI355X
  * from 4^3 x N=3 to 16^3 x N=4. */
 int hx3d_set_step_graph(hx3d_engine *eng, int on);
-/* ComputeMaxResiduals: max |QDot| per variable of the last computed derivative. */
+/* ComputeMaxResiduals (DGSEMClass.f90:770-860, called after each step at
+ * TimeIntegrator.f90:852): max |QDot| per variable of the current time derivative (after a step,
+ * the last stage's, or the new state's with CTD_AFTER_STEPS). */
 int hx3d_residuals(hx3d_engine *eng, double res[5]);
 /* Kinetic-energy monitor of the resident state. */
 int hx3d_kinetic_energy(hx3d_engine *eng, double *ke);
@@ -140,7 +158,8 @@ int hx3d_check_nan(hx3d_engine *eng, int *has_nan);
 int hx3d_synchronize(hx3d_engine *eng);
 /* Without blocking: *done = 1 when the engine's queued work is done, else 0.  With
  * t_inout = NULL, hx3d_advance only queues the steps, so a host can keep several engines
- * (each with its own stream) busy at once and poll them. */
+ * (each with its own stream) busy at once and poll them.  Returns HX3D_ENAN (with *done = 1)
+ * when the finished steps diverged (a NaN appeared in the state since it was last set). */
 int hx3d_query(hx3d_engine *eng, int *done);
 /* The engine's time (advanced on the device by hx3d_advance); waits for queued work. */
 int hx3d_get_time(hx3d_engine *eng, double *t);

@@ -695,6 +695,33 @@ static void two_point_flux(const ox_case *c, const double *QL, const double *QR,
         }
         for (int q = 0; q < 5; ++q) fS[q] = f[q] * Ja[0] + g[q] * Ja[1] + h[q] * Ja[2];
         return; }
+    case OX_AVG_MORINISHI: {
+        /* Morinishi_TwoPointFlux (RiemannSolvers_NS.f90:2147-2229): enthalpy without the
+         * kinetic energy, h = cp p */
+        double cp = c->gamma * (1.0 / (1.4 - 1.0));
+        double hL = cp * pL, hR = cp * pR;
+        const double *m[3] = {&QL[1], &QL[2], &QL[3]};
+        for (int d = 0; d < 3; ++d) Ja[d] = 0.5 * (JaL[d] + JaR[d]);
+        double *out[3] = {f, g, h};
+        double vel[3][2] = {{uL, uR}, {vL, vR}, {wL, wR}};
+        for (int d = 0; d < 3; ++d) {
+            /* direction d: the mass flux rho u_d of each side */
+            double mL = *m[d], mR = QR[1 + d];
+            double *o = out[d];
+            o[0] = 0.5 * (mL + mR);
+            o[1] = 0.25 * (mL + mR) * (uL + uR);
+            o[2] = 0.25 * (mL + mR) * (vL + vR);
+            o[3] = 0.25 * (mL + mR) * (wL + wR);
+            o[1 + d] = o[1 + d] + 0.5 * (pL + pR);
+            o[4] = 0.5 * (vel[d][0] * hL + vel[d][1] * hR) + 0.25 * (mL * uL + mR * uR) * (uL + uR)
+                 + 0.25 * (mL * vL + mR * vR) * (vL + vR)
+                 + 0.25 * (mL * wL + mR * wR) * (wL + wR)
+                 - 0.25 * (mL * POW2(uL) + mR * POW2(uR))
+                 - 0.25 * (mL * POW2(vL) + mR * POW2(vR))
+                 - 0.25 * (mL * POW2(wL) + mR * POW2(wR));
+        }
+        for (int q = 0; q < 5; ++q) fS[q] = f[q] * Ja[0] + g[q] * Ja[1] + h[q] * Ja[2];
+        return; }
     default: /* StandardDG_TwoPointFlux */
         for (int d = 0; d < 3; ++d) Ja[d] = (JaL[d] + JaR[d]);
         f[0] = (QL[1] + QR[1]);

@@ -15,7 +15,9 @@ Recipe
   they are not executed.
 * Compiler: ROCm's ``amdflang`` (flang-22); gfortran/ifort are absent from the image.
   Flags follow the reference's RELEASE set (Solver/Makefile.in:42-43) minus the
-  gfortran-only options: ``-cpp -O2 -fopenmp -DNAVIERSTOKES -D_has_Quad``.
+  gfortran-only options: ``-cpp -O2 -fopenmp -DNAVIERSTOKES -D_has_Quad``.  This -O2
+  build (``bin/horses3d.ns``) wrote the golden vectors.  ``--opt=-O3 --tag O3`` builds the
+  timing variant ``bin/horses3d_O3.ns`` (objects under ``O3/``) that the CPU baseline runs.
 * The per-case user plugin (the reference's ProblemFile API, libs/problemfile/
   ProblemFile.f90:22-237) is our own ``oracle/ref/ProblemFile.f90``; it sets the
   Taylor-Green initial condition and dumps state, time derivative and geometry.
@@ -85,10 +87,13 @@ def find_source(libdir, name):
     raise FileNotFoundError(name + ".f90 under " + libdir)
 
 
+BUILD = OUT   # objects, modules and archives of the variant being built
+
+
 def fflags(ref, opt):
     return ["-cpp", opt, "-fopenmp", "-DNAVIERSTOKES", "-D_has_Quad",
             "-I", os.path.join(ref, "Solver", "src", "libs", "foundation"),
-            "-module-dir", os.path.join(OUT, "mod"), "-I", os.path.join(OUT, "mod")]
+            "-module-dir", os.path.join(BUILD, "mod"), "-I", os.path.join(BUILD, "mod")]
 
 
 def compile_one(ref, src, obj, opt):
@@ -103,10 +108,15 @@ def main():
     ap.add_argument("--opt", default="-O2")
     ap.add_argument("--force", action="store_true")
     ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("--tag", default="", help="variant name: objects under oracle/_ref/<tag>/, "
+                                             "executable oracle/_ref/bin/horses3d_<tag>.ns")
     args = ap.parse_args()
+    global BUILD
+    BUILD = os.path.join(OUT, args.tag) if args.tag else OUT
     libs_root = os.path.join(args.ref, "Solver", "src", "libs")
-    for d in ("obj", "mod", "lib", "bin"):
-        os.makedirs(os.path.join(OUT, d), exist_ok=True)
+    for d in ("obj", "mod", "lib"):
+        os.makedirs(os.path.join(BUILD, d), exist_ok=True)
+    os.makedirs(os.path.join(OUT, "bin"), exist_ok=True)
     t0 = time.time()
     archives = {}
     for lib, var in LIBS:
@@ -116,14 +126,14 @@ def main():
             # interfaces the solver libraries use).  Compile it for that module only; its
             # default routines are not linked (our own plugin supplies them).
             src = os.path.join(libdir, "ProblemFile.f90")
-            obj = os.path.join(OUT, "obj", "problemfile_interfaces.o")
+            obj = os.path.join(BUILD, "obj", "problemfile_interfaces.o")
             rc, log = compile_one(args.ref, src, obj, args.opt)
             if rc != 0:
                 sys.stderr.write(log)
                 raise SystemExit("compile failed: %s" % src)
             continue
         names = makefile_objects(libdir, var)
-        objdir = os.path.join(OUT, "obj", lib.replace("/", "_"))
+        objdir = os.path.join(BUILD, "obj", lib.replace("/", "_"))
         os.makedirs(objdir, exist_ok=True)
         objs, pending = [], []
         for n in names:
@@ -145,7 +155,7 @@ def main():
                     sys.stderr.write(log)
                 raise SystemExit("compile failed in %s" % lib)
             pending = [so for so, _ in failed]
-        a = os.path.join(OUT, "lib", "lib%s.a" % lib.replace("/", "_"))
+        a = os.path.join(BUILD, "lib", "lib%s.a" % lib.replace("/", "_"))
         if os.path.exists(a):
             os.remove(a)
         subprocess.check_call(["ar", "crs", a] + objs)
@@ -154,7 +164,7 @@ def main():
 
     # Solver objects + our ProblemFile plugin
     solver = os.path.join(args.ref, "Solver", "src", "NavierStokesSolver")
-    objdir = os.path.join(OUT, "obj", "solver")
+    objdir = os.path.join(BUILD, "obj", "solver")
     os.makedirs(objdir, exist_ok=True)
     extra = []
     for src in (os.path.join(HERE, "ProblemFile.f90"),
@@ -166,7 +176,7 @@ def main():
             sys.stderr.write(log)
             raise SystemExit("compile failed: %s" % src)
         extra.append(obj)
-    exe = os.path.join(OUT, "bin", "horses3d.ns")
+    exe = os.path.join(OUT, "bin", "horses3d%s.ns" % ("_" + args.tag if args.tag else ""))
     link = ([FC, "-fopenmp", "-o", exe] + extra
             + ["-Wl,--start-group"] + [archives[l] for l in LINK_ORDER] + ["-Wl,--end-group"]
             + ["-Wl,--unresolved-symbols=ignore-in-object-files", "-ldl"])

@@ -12,6 +12,22 @@ for p in (REPO, os.path.join(REPO, "oracle"), os.path.join(REPO, "tests", "golde
         sys.path.insert(0, p)
 
 GOLDEN = sorted(glob.glob(os.path.join(REPO, "tests", "golden", "*.npz")))
+# reference runs without CTD_AFTER_STEPS: the final QDot is the last RK stage's
+STAGE3 = sorted(glob.glob(os.path.join(REPO, "tests", "golden", "stage3", "*.npz")))
+
+RK_A = (0.0, -5.0 / 9.0, -153.0 / 128.0)
+RK_C = (1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0)
+
+
+def rk3_by_stages(o, Q, G, dt):
+    """One Williamson RK3 step written out stage by stage from the oracle's time derivative
+    (TakeRK3Step, ExplicitMethods.f90:697-775), Q and G in place; returns the last stage's
+    QDot."""
+    for a, c in zip(RK_A, RK_C):
+        qd = o.qdot(Q)
+        G[:] = a * G + qd
+        Q[:] = Q + c * dt * G
+    return qd
 
 
 def pytest_configure(config):

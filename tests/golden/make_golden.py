@@ -64,9 +64,32 @@ CASES = {
         n=2, N=6, orders=(2, 6, 44), amp=0.1, seed=4, steps=2, dumps=[0], noise=0.05, noise_seed=441,
         ctl=dict(mach=0.3, re=800.0, riemann="lax-friedrichs", inviscid="split-form",
                  averaging="Pirozzoli")),
+    # Morinishi averaging: the split-form two-point flux and, in the standard form, the LxF face
+    # average (AveragedStates)
+    "x_noise_n2_p4_curved_split_morinishi_lxf": dict(
+        n=2, N=4, amp=0.1, seed=12, steps=2, dumps=[0], noise=0.05, noise_seed=121,
+        ctl=dict(mach=0.3, re=900.0, riemann="lax-friedrichs", inviscid="split-form",
+                 averaging="Morinishi")),
+    "x_noise_n2_p3_curved_standard_lxf_morinishi": dict(
+        n=2, N=3, amp=0.1, seed=13, steps=2, dumps=[0], noise=0.05, noise_seed=131,
+        ctl=dict(mach=0.25, re=1200.0, riemann="lax-friedrichs", inviscid="standard",
+                 averaging="Morinishi")),
     "x_noise_n3_p2_standard_roe": dict(
         n=3, N=2, amp=0.1, seed=10, steps=2, dumps=[0], noise=0.05, noise_seed=101,
         ctl=dict(mach=0.8, re=400.0)),
+}
+
+
+# Runs without "compute time derivative after timestep" (the reference's default): the QDot the
+# final dump holds is the last RK stage's, the derivative ComputeMaxResiduals reads after a step
+# (TimeIntegrator.f90:852).  Written to stage3/ (their final QDot is not QDot(Q_final)).
+STAGE3_CASES = {
+    "c1_tgv_n4_p3_roe_standard_10steps": dict(n=4, N=3, amp=0.0, seed=0, steps=10, dumps=[0],
+                                              ctl=dict(mach=0.1, re=1600.0, ctd_after=False)),
+    "c2s_tgv_n3_p4_curved_split_pirozzoli_lxf_5steps": dict(
+        n=3, N=4, amp=0.1, seed=2, steps=5, dumps=[0],
+        ctl=dict(mach=0.1, re=1600.0, riemann="lax-friedrichs", inviscid="split-form",
+                 averaging="Pirozzoli", ctd_after=False)),
 }
 
 
@@ -76,7 +99,7 @@ def phases(seed):
     return np.random.default_rng(seed).uniform(0.0, 2.0 * np.pi, 9)
 
 
-def run_case(name, spec, workdir):
+def run_case(name, spec, workdir, outdir=HERE):
     d = os.path.join(workdir, name)
     os.makedirs(d, exist_ok=True)
     rc.write_specmesh(os.path.join(d, "box.mesh"), spec["n"], spec["amp"], phases(spec["seed"]),
@@ -131,17 +154,21 @@ def run_case(name, spec, workdir):
     meta.update(n=spec["n"], order=spec["N"], amp=spec["amp"], phase_seed=spec["seed"],
                 noise=spec.get("noise", 0.0), orders=spec.get("orders"))
     out["meta"] = np.array(repr(meta))
-    path = os.path.join(HERE, name + ".npz")
+    path = os.path.join(outdir, name + ".npz")
     np.savez_compressed(path, **out)
     return path
 
 
 def main(argv):
-    names = argv or list(CASES)
+    names = argv or list(CASES) + list(STAGE3_CASES)
     work = tempfile.mkdtemp(prefix="hx3d_golden_")
     try:
         for nm in names:
-            p = run_case(nm, CASES[nm], work)
+            if nm in STAGE3_CASES:
+                os.makedirs(os.path.join(HERE, "stage3"), exist_ok=True)
+                p = run_case(nm, STAGE3_CASES[nm], work, os.path.join(HERE, "stage3"))
+            else:
+                p = run_case(nm, CASES[nm], work)
             print("%-48s %8.1f KB" % (os.path.basename(p), os.path.getsize(p) / 1024.0), flush=True)
     finally:
         shutil.rmtree(work, ignore_errors=True)

@@ -69,13 +69,13 @@ def write_control(path, mesh, order, steps, mach, re, riemann="roe", inviscid="s
         fh.write("\n".join(out) + "\n")
 
 
-def run_reference(case_dir, control="case.control", env=None, threads=1, timeout=3600):
+def run_reference(case_dir, control="case.control", env=None, threads=1, timeout=3600, exe=None):
     os.makedirs(os.path.join(case_dir, "RESULTS"), exist_ok=True)
     os.makedirs(os.path.join(case_dir, "MESH"), exist_ok=True)
     e = dict(os.environ)
     e.update({"OMP_NUM_THREADS": str(threads), "OMP_SCHEDULE": "static"})
     e.update(env or {})
-    r = subprocess.run([REF_EXE, control], cwd=case_dir, env=e, capture_output=True,
+    r = subprocess.run([exe or REF_EXE, control], cwd=case_dir, env=e, capture_output=True,
                        text=True, timeout=timeout)
     with open(os.path.join(case_dir, "ref_stdout.log"), "w") as fh:
         fh.write(r.stdout + r.stderr)

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import load_golden, physics_kwargs, GOLDEN
+from conftest import load_golden, physics_kwargs, rk3_by_stages, GOLDEN, STAGE3
 from horses3d_amd import mesh
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +59,7 @@ def test_box4_n3_100_steps_kinetic_energy():
     """4^3 box, N=3, Roe/BR1/standard, CFL 0.4: 100 device-side dt steps against the reference."""
     case, meta = load_golden([p for p in GOLDEN if "c1_tgv" in p][0])
     eng = engine_for(case, physics_kwargs(meta))
+    eng.set_ctd_after_steps(True)   # the fixture's run: "compute time derivative after timestep"
     eng.set_state(case["Q_step0"])
     t = eng.advance(10, 0.4, 0.4, t=0.0)
     assert po.normwise_error(eng.get_state(), case["Q_step10"]).max() < 1e-12
@@ -498,3 +499,98 @@ def test_volume_monitors_match_reference(name, tmp_path):
         print(name, kind, v, ref[2], flush=True)
         assert abs(v - ref[2]) <= 1e-12 * abs(ref[2]), (kind, v, ref[2])
     eng.close()
+
+
+@pytest.mark.parametrize("path", STAGE3, ids=lambda p: __import__("os").path.basename(p)[:-4])
+def test_step_leaves_last_stage_qdot(path):
+    """After a step the current QDot is the last RK stage's, as the reference's storage holds it
+    without CTD_AFTER_STEPS (the derivative ComputeMaxResiduals reads, TimeIntegrator.f90:852);
+    with CTD_AFTER_STEPS it is the new state's."""
+    case, meta = load_golden(path)
+    kw = physics_kwargs(meta)
+    eng, ora = engine_for(case, kw), oracle_for(case, kw)
+    eng.set_state(case["Q_step0"])
+    eng.advance(int(case["steps"]), 0.4, 0.4, t=0.0)
+    assert po.normwise_error(eng.get_state(), case["Q_final"]).max() < 1e-11
+    qd = eng.get_qdot()
+    err = po.normwise_error(qd, case["QDot_final"])
+    print("advance: last-stage QDot vs reference", err, flush=True)
+    assert err.max() < 1e-11, err
+    assert np.allclose(eng.residuals(), np.abs(case["QDot_final"]).max(axis=0), rtol=1e-11, atol=0)
+    # one hx3d_rk3_step against the oracle's step written out stage by stage
+    Q0 = case["Q_step0"]
+    eng.set_state(Q0)
+    dt = eng.max_dt(0.4, 0.4)
+    eng.rk3_step(0.0, dt)
+    Qo, G = Q0.copy(), np.zeros_like(Q0)
+    qd_o = rk3_by_stages(ora, Qo, G, dt)
+    assert po.normwise_error(eng.get_qdot(), qd_o).max() <= TOL
+    assert np.allclose(eng.residuals(), np.abs(qd_o).max(axis=0), rtol=TOL, atol=0)
+    # CTD_AFTER_STEPS: the derivative of the state the step leaves
+    eng.set_ctd_after_steps(True)
+    eng.set_state(Q0)
+    eng.rk3_step(0.0, dt)
+    assert po.normwise_error(eng.get_qdot(), ora.qdot(eng.get_state())).max() <= TOL
+    eng.close()
+
+
+def test_divergence_is_reported():
+    """A NaN in the state (here planted in the initial state) is flagged by the RK update
+    (checkForNan, ExplicitMethods.f90:786): the waiting step loop returns HX3D_ENAN, the queued one
+    reports it when polled, and a new state clears it.  NaN is ordinary data: every kernel runs
+    to its end."""
+    from horses3d_amd.engine import Engine, HX3DDiverged
+    case = mesh.straight_box(3, 3)
+    Q0 = mesh.taylor_green(case, 0.1)
+    bad = Q0.copy()
+    bad[17, 4] = np.nan
+    eng = Engine(case, mach=0.1, reynolds=1600.0)
+    eng.set_state(Q0)
+    eng.advance(2, 0.4, 0.4, t=0.0)
+    assert not eng.has_nan()
+    eng.set_state(bad)
+    with pytest.raises(HX3DDiverged):
+        eng.advance(3, 0.4, 0.4, t=0.0)
+    assert eng.has_nan()
+    eng.set_state(bad)
+    eng.advance_async(2, 0.4, 0.4)
+    eng.synchronize()
+    with pytest.raises(HX3DDiverged):
+        eng.done()
+    eng.set_state(Q0)
+    eng.advance(2, 0.4, 0.4, t=0.0)
+    assert eng.done() and not eng.has_nan()
+    for mixed in (False, True):
+        c = mesh.periodic_box_mixed(3, mesh.random_orders(27, 2, 4, 5)) if mixed else case
+        q = mesh.taylor_green(c, 0.1)
+        q[5, 1] = np.nan
+        e = Engine(c, mach=0.1, reynolds=1600.0)
+        e.set_state(q)
+        with pytest.raises(HX3DDiverged):
+            e.advance_dt(2, 1e-3, t=0.0)
+        e.close()
+    eng.close()
+
+
+def test_serving_retires_a_diverging_case_alone():
+    """A served case whose initial state holds a NaN is retired with its error; every other case
+    completes with the state it reaches alone, bit for bit."""
+    from horses3d_amd import serve
+    rng = np.random.default_rng(11)
+    reqs = []
+    for k in range(5):
+        n, N = int(rng.integers(2, 4)), int(rng.integers(2, 5))
+        case = mesh.straight_box(n, N)
+        mach = float(rng.uniform(0.1, 0.5))
+        Q0 = mesh.taylor_green(case, mach, phase=tuple(rng.uniform(0.0, 2.0 * np.pi, 3)))
+        if k == 2:
+            Q0[3, 0] = np.nan
+        reqs.append(serve.Request(case, dict(mach=mach, reynolds=800.0, riemann="roe"), Q0,
+                                  steps=int(rng.integers(10, 30)), arrival=0.002 * k))
+    res, st = serve.serve(reqs, chunk=4, max_active=8, max_running=3)
+    alone, _ = serve.one_at_a_time(reqs)
+    assert st["failed"] == 1 and res[2].error and "divergence" in res[2].error
+    assert alone[2].error and "divergence" in alone[2].error
+    for k, (r, a) in enumerate(zip(res, alone)):
+        if k != 2:
+            assert r.error is None and r.t == a.t and np.array_equal(r.Q, a.Q), k

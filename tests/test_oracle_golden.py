@@ -6,11 +6,13 @@ ComputeTimeDerivative, SpatialDiscretization.f90:227-320), MaxTimeStep
 (DGSEMClass.f90:870-1035), the RK3 step (ExplicitMethods.f90:667-788) and the kinetic-energy
 monitor (VolumeIntegrals.f90:179-195).
 """
+import os
+
 import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import load_golden, physics_kwargs, GOLDEN
+from conftest import load_golden, physics_kwargs, rk3_by_stages, GOLDEN, STAGE3
 
 
 def make_oracle(case, meta):
@@ -59,3 +61,28 @@ def test_rk3_trajectory_matches_reference():
     ke_ref = case["ke_monitor"][-1, 2]
     assert abs(ke / ke_ref - 1.0) < 1e-10
     assert abs(t / case["t_final"] - 1.0) < 1e-13
+
+
+
+
+@pytest.mark.parametrize("path", STAGE3, ids=lambda p: os.path.basename(p)[:-4])
+def test_last_stage_qdot_is_the_reference_residual_source(path):
+    """Without CTD_AFTER_STEPS the reference's storage holds the last RK stage's QDot after a
+    step; that is what ComputeMaxResiduals and the monitors read (TimeIntegrator.f90:852-865).
+    The oracle's step, written out stage by stage, reproduces the final state and that QDot."""
+    case, meta = load_golden(path)
+    assert meta["ctd_after"] is False
+    o = make_oracle(case, meta)
+    Q = case["Q_step0"].copy()
+    G = np.zeros_like(Q)
+    for s in range(int(case["steps"])):
+        dt = o.max_dt(Q, 0.4, 0.4)
+        assert dt == pytest.approx(case["dt_log"][s, 2], rel=1e-12), s
+        Qs, Gs = Q.copy(), G.copy()
+        o.rk3_step(Qs, Gs, 0.0, dt)
+        qd = rk3_by_stages(o, Q, G, dt)
+        assert np.array_equal(Q, Qs) and np.array_equal(G, Gs)
+    assert po.normwise_error(Q, case["Q_final"]).max() < 1e-12
+    assert po.normwise_error(qd, case["QDot_final"]).max() < 1e-11
+    # a plausible mistake: the derivative of the final state instead of the last stage's
+    assert po.normwise_error(o.qdot(Q), case["QDot_final"]).max() > 1e-6

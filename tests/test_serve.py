@@ -146,9 +146,9 @@ def test_threaded_setup_and_retirement():
         assert r.admitted >= q.arrival
 
 
-def test_failed_setup_releases_engines():
-    """A case whose set-up fails ends the loop with its error, and the engines of the other
-    cases are closed."""
+def test_failed_setup_retires_that_case_alone():
+    """A case whose set-up fails is retired alone with its error; the other cases complete with
+    the results they have alone, and every engine is closed."""
     reqs = requests(6, seed=5)
     for q in reqs:
         q.arrival = 0.0
@@ -163,11 +163,40 @@ def test_failed_setup_releases_engines():
         return e
 
     FakeEngine.live = 0
-    try:
-        serve.serve(reqs, engine_factory=factory, clock=clock, sleep=clock.sleep, poll_s=0.0, chunk=5,
-                    max_active=16, device_memory=lambda: 1 << 40, workers=0)
-    except RuntimeError as e:
-        assert "no memory" in str(e)
-    else:
-        raise AssertionError("the set-up error was not passed on")
+    res, st = serve.serve(reqs, engine_factory=factory, clock=clock, sleep=clock.sleep, poll_s=0.0, chunk=5,
+                          max_active=16, device_memory=lambda: 1 << 40, workers=0)
     assert made and FakeEngine.live == 0
+    assert res[3].error and "no memory" in res[3].error and res[3].Q is None
+    assert st["cases"] == 5 and st["failed"] == 1
+    for k, (r, q) in enumerate(zip(res, reqs)):
+        if k != 3:
+            assert r.error is None and np.array_equal(r.Q, q.Q0 + q.steps * 0.5)
+
+
+class DivergingEngine(FakeEngine):
+    """Diverges (raises the engine's NaN error on polling) once it has taken 7 steps."""
+
+    def done(self):
+        d = super().done()
+        if d and self.steps >= 7:
+            from horses3d_amd.engine import HX3DDiverged
+            raise HX3DDiverged("numerical divergence", 5)
+        return d
+
+
+def test_diverging_case_retired_alone():
+    """A case whose steps diverge stops there with its error; the others are unaffected."""
+    reqs = requests(8, seed=9)
+    for q in reqs:
+        q.steps = max(q.steps, 12)
+    clock = Clock()
+    FakeEngine.live = 0
+    factory = lambda r: (DivergingEngine if r is reqs[2] else FakeEngine)(r, clock)
+    res, st = serve.serve(reqs, engine_factory=factory, clock=clock, sleep=clock.sleep, poll_s=0.0, chunk=5,
+                          max_active=16, device_memory=lambda: 1 << 40, workers=0)
+    assert FakeEngine.live == 0
+    assert res[2].error and "divergence" in res[2].error and res[2].Q is not None
+    assert st["failed"] == 1 and st["cases"] == len(reqs) - 1
+    for k, (r, q) in enumerate(zip(res, reqs)):
+        if k != 2:
+            assert r.error is None and np.array_equal(r.Q, q.Q0 + q.steps * 0.5)

@@ -46,6 +46,31 @@ def test_derivative_matrices(N):
     assert np.allclose(mesh.deriv_matrix(x), D, atol=1e-13)
 
 
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_chebyshev_gauss_lobatto_operators(N):
+    """The Chebyshev-Gauss-Lobatto framework of NodalStorage (NodalStorageClass.f90:282-296):
+    nodes -cos(j pi / N), a derivative matrix exact on polynomials of degree <= N, and an
+    interpolation to the GL nodes exact on the same polynomials."""
+    op = po.nodal_cgl(N)
+    x, D, T = op["xcgl"], op["DCGL"], op["Tcheb2gauss"]
+    assert np.allclose(x, -np.cos(np.arange(N + 1) * np.pi / N), atol=1e-15, rtol=0)
+    assert x[0] == -1.0 and abs(x[-1] - 1.0) < 1e-15
+    xg = po.nodal_gl(N)["x"]
+    for p in range(N + 1):
+        dref = p * x ** (p - 1) if p else 0 * x
+        assert np.allclose(D @ x ** p, dref, atol=1e-12), p
+        assert np.allclose(T @ x ** p, xg ** p, atol=1e-13), p
+    # one degree more is not reproduced where the two node sets differ (N >= 3; for N <= 2 the
+    # CGL and GL nodes coincide and T is the identity)
+    if N >= 3:
+        assert not np.allclose(T @ x ** (N + 1), xg ** (N + 1), atol=1e-6)
+    else:
+        assert np.allclose(T, np.eye(N + 1), atol=1e-15)
+    # rows of an interpolation sum to one, rows of a derivative to zero
+    assert np.allclose(T.sum(axis=1), 1.0, atol=1e-14)
+    assert np.allclose(D.sum(axis=1), 0.0, atol=1e-12)
+
+
 @pytest.mark.parametrize("N,M", [(2, 5), (3, 4), (4, 7), (5, 3), (7, 2), (6, 6)])
 def test_tset_interpolation_and_projection(N, M):
     T = po.tset(N, M)
