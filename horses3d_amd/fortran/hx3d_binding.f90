@@ -108,6 +108,16 @@ module hx3d_binding
          type(c_ptr), value    :: eng
          real(c_double), value :: t, dt
       end function
+      integer(c_int) function hx3d_set_ctd_after_steps(eng, on) bind(C, name="hx3d_set_ctd_after_steps")
+         import :: c_int, c_ptr
+         type(c_ptr), value    :: eng
+         integer(c_int), value :: on
+      end function
+      integer(c_int) function hx3d_get_qdot(eng, QDot) bind(C, name="hx3d_get_qdot")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double)     :: QDot(*)
+      end function
       integer(c_int) function hx3d_max_dt(eng, cfl, dcfl, dt) bind(C, name="hx3d_max_dt")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value          :: eng

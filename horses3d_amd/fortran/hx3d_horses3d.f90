@@ -26,6 +26,7 @@ module hx3d_horses3d
    use ParticlesClass, only: Particles_t
 #endif
    use DGSEMClass, only: ComputeTimeDerivative_f
+   use ExplicitMethods, only: CTD_AFTER_STEPS
    use hx3d_binding
    implicit none
    private
@@ -162,7 +163,11 @@ contains
 
 !
 !  TimeStep_FCN: one Williamson RK3 step of the elements' Q (ExplicitMethods.f90:667-788).
-!  The engine keeps its own RK register; ComputeTimeDerivative is not called.
+!  The engine keeps its own RK register; ComputeTimeDerivative is not called.  On return the
+!  elements hold the new Q and, as after the reference's step, the QDot its integrator reads
+!  next (ComputeMaxResiduals, the monitors and the steady-state tolerance,
+!  TimeIntegrator.f90:852-874): the last stage's, or with CTD_AFTER_STEPS (:784) the derivative
+!  of the new state.  A NaN in the new state stops the run (checkForNan, :786).
 !
    subroutine hx3d_TakeRK3Step(mesh, particles, t, deltaT, ComputeTimeDerivative, dt_vec, dts, &
                                global_dt, iter, dtAdaptation)
@@ -183,16 +188,22 @@ contains
       integer(c_int) :: bad
       call pack_state(mesh)
       call check(hx3d_set_state(eng, Qp))
+      call check(hx3d_set_ctd_after_steps(eng, merge(1_c_int, 0_c_int, CTD_AFTER_STEPS)))
       call check(hx3d_rk3_step(eng, real(t, c_double), real(deltaT, c_double)))
       call check(hx3d_get_state(eng, Qp))
       do e = 1, size(mesh % elements)
          mesh % elements(e) % storage % Q = reshape(Qp(5*off(e)+1:5*off(e+1)), &
                                                     shape(mesh % elements(e) % storage % Q))
       end do
+      call check(hx3d_get_qdot(eng, Qp))
+      do e = 1, size(mesh % elements)
+         mesh % elements(e) % storage % QDot = reshape(Qp(5*off(e)+1:5*off(e+1)), &
+                                                       shape(mesh % elements(e) % storage % QDot))
+      end do
       ! checkForNan (ExplicitMethods.f90:786, :1856-1905)
       call check(hx3d_check_nan(eng, bad))
       if (bad /= 0) then
-         write(*, '(A)') "NaN detected in the solution"
+         write(*, '(A)') "Numerical divergence obtained in solver."
          error stop 99
       end if
    end subroutine hx3d_TakeRK3Step
