@@ -98,7 +98,7 @@ k_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doubl
         for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
         hx::dt_limits(ph, q, ja, Jac[g], dcsi, dcsi, dcsi, cfl, dcfl, tc, tv);
     }
-    hxk::block_min_pair(tc, tv, part, threadIdx.x);
+    hxk::block_min_pair(tc, tv, part, threadIdx.x, blockIdx.x);
 }
 
 // The minima of n (convective, viscous) pairs as the bit patterns of a dt slot.
@@ -123,10 +123,10 @@ k_dt_reduce(const double *__restrict__ part, int n, unsigned long long *__restri
     }
 }
 
-// a host-given dt as both limits of a dt slot
-__global__ void k_dt_set(unsigned long long *bits, double dt)
+// a host-given dt as both limits of the n / 2 pairs of a dt slot
+__global__ void k_dt_set(unsigned long long *bits, int n, double dt)
 {
-    if (threadIdx.x < 2) bits[threadIdx.x] = (unsigned long long)__double_as_longlong(dt);
+    for (int x = threadIdx.x; x < n; x += blockDim.x) bits[x] = (unsigned long long)__double_as_longlong(dt);
 }
 
 // dt = min(convective, viscous) (DGSEMClass.f90:1024-1030); with advance != 0 also
@@ -322,7 +322,7 @@ __global__ void k_fill(double *__restrict__ X, long n, double val)
 }
 __global__ void k_fill_int(int *__restrict__ X, int n, int val)
 {
-    if ((int)threadIdx.x < n) X[threadIdx.x] = val;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) X[x] = val;
 }
 
 // ------------------------------------------------------------------------------------
@@ -405,6 +405,16 @@ struct hx3d_engine {
     // kernels when a NaN appears.  hFlag: pinned host copy of dFlag[1], refreshed in stream
     // order after each queued step loop (read by hx3d_query once the stream is idle).
     int *hFlag = nullptr;
+    int device = 0;   // the HIP device the engine was created on
+    // several independent cases in one engine (hx3d_engine_create_multi): per-case tables; with
+    // one case ncases = 1, mc is all null and dTime = dScal + 1
+    int ncases = 1, nblocks_multi = 0;
+    hxk::Multi mc{};
+    int4 *dBlocks = nullptr;
+    hxk::CaseConst *dCC = nullptr;
+    int *dActive = nullptr, *dFcase = nullptr, *dPfirst = nullptr;
+    double *dTimeM = nullptr;
+    double *dTime = nullptr;   // [ncases] the time of each case
     long dev_bytes = 0;   // device memory the engine holds
     // hx3d_advance's step as a HIP graph, captured for one (cfl, dcfl) and replayed per step
     // (off by default: replays measured 2-5% slower than the same launches issued directly)
@@ -440,24 +450,28 @@ inline bool offsets_fit(long S, long ES, long FS)
     return S < lim && ES < lim && FS < lim;
 }
 
-constexpr int epb_for(int N) { return N == 1 ? 16 : N == 2 ? 8 : N == 3 ? 4 : N == 4 ? 2 : 1; }
+#ifndef HX3D_EPB4
+#define HX3D_EPB4 2
+#endif
+constexpr int epb_for(int N) { return N == 1 ? 16 : N == 2 ? 8 : N == 3 ? 4 : N == 4 ? HX3D_EPB4 : 1; }
 
-// blocks of the uniform-order gradient kernel
-int grad_grid(const hx3d_engine *eng)
+// blocks of the uniform-order element kernels (a multiple of 8, see xcd_block): EPB elements per
+// block, or the block table of several cases
+int elem_grid(const hx3d_engine *eng, int epb)
 {
-    const int epb = epb_for(eng->N);
-    return ((((eng->K + epb - 1) / epb) + 7) / 8) * 8;
+    if (eng->mc.blocks) return grid_for(eng->nblocks_multi);
+    return grid_for((eng->K + epb - 1) / epb);
 }
+int grad_grid(const hx3d_engine *eng) { return elem_grid(eng, epb_for(eng->N)); }
 
 template <int N>
 void launch_grad(hx3d_engine *eng, const hxk::DtLimits &dtl)
 {
     constexpr int EPB = epb_for(N), ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
-    const int nb = grid_for((eng->K + EPB - 1) / EPB);
-    hipLaunchKernelGGL((hxk::k_grad<N, EPB>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
+    hipLaunchKernelGGL((hxk::k_grad<N, EPB>), dim3(elem_grid(eng, EPB)), dim3(TPB), 0, eng->stream, eng->dQ, eng->dJa,
                        eng->dJ, eng->dNbr, eng->dEgeo, eng->dVgrad, eng->K, eng->S, eng->ES,
-                       eng->ops, eng->ph, dtl);
+                       eng->ops, eng->ph, dtl, eng->mc);
 }
 
 template <int N>
@@ -466,7 +480,7 @@ void launch_face(hx3d_engine *eng, const hxk::DtReduce &dtr)
     const int nb = grid_for((int)((eng->FS + 255) / 256));
     hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dVgrad,
                        eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
-                       1.0 / eng->ops.w[0], eng->ph, dtr);
+                       1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
 }
 
 template <int N, int FORM, int EPB>
@@ -474,10 +488,9 @@ void launch_volume_e(hx3d_engine *eng, double a_k, double c_k, int update, const
 {
     constexpr int ne = (N + 1) * (N + 1) * (N + 1);
     constexpr int TPB = ((EPB * ne + 63) / 64) * 64;
-    const int nb = grid_for((eng->K + EPB - 1) / EPB);
-    hipLaunchKernelGGL((hxk::k_volume<N, EPB, FORM>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ,
+    hipLaunchKernelGGL((hxk::k_volume<N, EPB, FORM>), dim3(elem_grid(eng, EPB)), dim3(TPB), 0, eng->stream, eng->dQ,
                        eng->dVgrad, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dG,
-                       qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->ES, eng->ops, eng->ph);
+                       qdot, a_k, c_k, update, dta, eng->K, eng->S, eng->ES, eng->ops, eng->ph, eng->mc);
 }
 
 template <int N, int FORM>
@@ -538,10 +551,13 @@ void stage(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &d
 // hx3d_max_dt's, 4 holds a host-given dt
 constexpr int DT_SLOTS = 5;
 
+// dt slot s of every case: ncases (convective, viscous) pairs
+unsigned long long *dt_slot(const hx3d_engine *eng, int s) { return eng->dBits + 2L * s * eng->ncases; }
+
 DtArgs host_dt(hx3d_engine *eng, double dt)
 {
-    unsigned long long *b = eng->dBits + 2 * 4;
-    hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(64), 0, eng->stream, b, dt);
+    unsigned long long *b = dt_slot(eng, 4);
+    hipLaunchKernelGGL(k_dt_set, dim3(1), dim3(256), 0, eng->stream, b, 2 * eng->ncases, dt);
     DtArgs a{};
     a.in = b;
     return a;
@@ -553,7 +569,7 @@ void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long 
 
 void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 {
-    unsigned long long *b = eng->dBits + 2 * slot;
+    unsigned long long *b = dt_slot(eng, slot);
     launch_maxdt_bits(eng, cfl, dcfl, b);
     hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, eng->stream, b, eng->dScal, 0);
 }
@@ -660,17 +676,18 @@ void launch_maxdt_bits(hx3d_engine *eng, double cfl, double dcfl, unsigned long 
 // the divergence flag's pinned host copy, and both device flags cleared
 int init_flags(hx3d_engine *eng)
 {
-    if (hipHostMalloc((void **)&eng->hFlag, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (!eng->dTime) eng->dTime = eng->dScal + 1;
+    if (hipHostMalloc((void **)&eng->hFlag, sizeof(int) * eng->ncases, hipHostMallocDefault) != hipSuccess)
         return eng->fail(HX3D_ENOMEM, "hipHostMalloc failed");
-    *eng->hFlag = 0;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, eng->stream, eng->dFlag, 2, 0);
+    for (int c = 0; c < eng->ncases; ++c) eng->hFlag[c] = 0;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, eng->stream, eng->dFlag, 1 + eng->ncases, 0);
     return HX3D_OK;
 }
 
 // hFlag <- dFlag[1] in stream order (read by the host once the stream is idle)
 void queue_flag_readback(hx3d_engine *eng)
 {
-    (void)hipMemcpyAsync(eng->hFlag, eng->dFlag + 1, sizeof(int), hipMemcpyDeviceToHost, eng->stream);
+    (void)hipMemcpyAsync(eng->hFlag, eng->dFlag + 1, sizeof(int) * eng->ncases, hipMemcpyDeviceToHost, eng->stream);
 }
 
 // After a queued step loop: the divergence flag's read-back, and the time derivative the
@@ -926,6 +943,8 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     set_physics(eng, d);
     eng->N3h.assign(d->N, d->N + 3L * d->K);
     eng->mach = d->mach;
+    // descriptor checks come first (they need no device); the device is recorded from here on
+    if (hipGetDevice(&eng->device) != hipSuccess) return bail(eng->fail(HX3D_EDEVICE, "no HIP device"));
     if (!uniform) return bail(create_mixed(eng, d));
     eng->K = d->K; eng->N = N; eng->n1 = N + 1; eng->nf = eng->n1 * eng->n1; eng->ne = eng->nf * eng->n1;
     eng->F = d->F;
@@ -1085,6 +1104,147 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     return HX3D_OK;
 }
 
+// ---- several independent cases of one order in one engine ------------------------------
+// The union of the cases' meshes (element, face and node numbers offset case by case; no face
+// joins two cases) is built as one case; then the per-case tables: element blocks that never
+// straddle cases (so a block's case, constants and dt are uniform), each face's case, the
+// per-case constants, dt slots, times, divergence flags and active mask.
+int hx3d_engine_create_multi(const hx3d_case_desc *descs, int ncases, hx3d_engine **out)
+{
+    if (!descs || ncases < 1 || !out) return HX3D_EINVAL;
+    *out = nullptr;
+    const hx3d_case_desc &d0 = descs[0];
+    auto reject = [&](int code, const std::string &m) {
+        hx3d_engine *e = new hx3d_engine();
+        e->fail(code, m);
+        *out = e;
+        return code;
+    };
+    long K = 0, F = 0, nodes = 0, fnodes = 0;
+    const bool fg = d0.fNormal && d0.fT1 && d0.fT2 && d0.fJac;
+    std::vector<long> Kc(ncases), Fc(ncases), nc(ncases);
+    for (int c = 0; c < ncases; ++c) {
+        const hx3d_case_desc &d = descs[c];
+        if (d.K <= 0 || !d.N || !d.faceID || !d.faceSide || !d.fElem || !d.fESide || !d.fRot || !d.Ja || !d.J)
+            return reject(HX3D_EINVAL, "case " + std::to_string(c) + ": missing arrays or K <= 0");
+        for (long e = 0; e < 3L * d.K; ++e)
+            if (d.N[e] != d0.N[0]) return reject(HX3D_EUNSUPPORTED, "several cases in one engine need one order N everywhere");
+        if (d.riemann != d0.riemann || d.inviscid != d0.inviscid || d.averaging != d0.averaging || d.les != d0.les ||
+            d.euler != d0.euler || d.lambda_stab != d0.lambda_stab)
+            return reject(HX3D_EUNSUPPORTED, "several cases in one engine need the same discretisation (Riemann solver, "
+                                             "volume form, averaging, LES model, lambda)");
+        if (d.euler) return reject(HX3D_EUNSUPPORTED, "several cases in one engine: Navier-Stokes cases only");
+        if (fg != (d.fNormal && d.fT1 && d.fT2 && d.fJac))
+            return reject(HX3D_EINVAL, "give the face geometry for every case or for none");
+        if (d.mach <= 0 || d.reynolds <= 0) return reject(HX3D_EINVAL, "case " + std::to_string(c) + ": mach and reynolds must be positive");
+        const int n1 = d0.N[0] + 1;
+        Kc[c] = d.K; Fc[c] = d.F; nc[c] = (long)d.K * n1 * n1 * n1;
+        K += d.K; F += d.F; nodes += nc[c]; fnodes += (long)d.F * n1 * n1;
+    }
+    if (K >= (1L << 31) / 8) return reject(HX3D_EUNSUPPORTED, "too many elements");
+    const int n1 = d0.N[0] + 1, nf = n1 * n1;
+    std::vector<int> N(3 * K), faceID(6 * K), faceSide(6 * K), fElem(2 * F), fESide(2 * F), fRot(F), fcase(F);
+    std::vector<double> Ja(9 * nodes), J(nodes), fN, fT1, fT2, fJ;
+    if (fg) { fN.resize(3 * fnodes); fT1.resize(3 * fnodes); fT2.resize(3 * fnodes); fJ.resize(fnodes); }
+    long k0 = 0, f0 = 0, g0 = 0, q0 = 0;
+    for (int c = 0; c < ncases; ++c) {
+        const hx3d_case_desc &d = descs[c];
+        for (long x = 0; x < 3 * Kc[c]; ++x) N[3 * k0 + x] = d.N[x];
+        for (long x = 0; x < 6 * Kc[c]; ++x) { faceID[6 * k0 + x] = d.faceID[x] + (int)f0; faceSide[6 * k0 + x] = d.faceSide[x]; }
+        for (long x = 0; x < 2 * Fc[c]; ++x) { fElem[2 * f0 + x] = d.fElem[x] + (int)k0; fESide[2 * f0 + x] = d.fESide[x]; }
+        for (long x = 0; x < Fc[c]; ++x) { fRot[f0 + x] = d.fRot[x]; fcase[f0 + x] = c; }
+        std::memcpy(&Ja[9 * g0], d.Ja, sizeof(double) * 9 * nc[c]);
+        std::memcpy(&J[g0], d.J, sizeof(double) * nc[c]);
+        if (fg) {
+            const long qn = Fc[c] * nf;
+            std::memcpy(&fN[3 * q0], d.fNormal, sizeof(double) * 3 * qn);
+            std::memcpy(&fT1[3 * q0], d.fT1, sizeof(double) * 3 * qn);
+            std::memcpy(&fT2[3 * q0], d.fT2, sizeof(double) * 3 * qn);
+            std::memcpy(&fJ[q0], d.fJac, sizeof(double) * qn);
+            q0 += qn;
+        }
+        k0 += Kc[c]; f0 += Fc[c]; g0 += nc[c];
+    }
+    hx3d_case_desc u = d0;
+    u.K = (int)K; u.F = (int)F; u.N = N.data(); u.faceID = faceID.data(); u.faceSide = faceSide.data();
+    u.fElem = fElem.data(); u.fESide = fESide.data(); u.fRot = fRot.data(); u.Ja = Ja.data(); u.J = J.data();
+    u.fNormal = fg ? fN.data() : nullptr; u.fT1 = fg ? fT1.data() : nullptr;
+    u.fT2 = fg ? fT2.data() : nullptr; u.fJac = fg ? fJ.data() : nullptr;
+    hx3d_engine *eng = nullptr;
+    int rc = hx3d_engine_create(&u, &eng);
+    *out = eng;
+    if (rc) return rc;
+    if (eng->mixed) return eng->fail(HX3D_EUNSUPPORTED, "several cases in one engine need one order N everywhere");
+    // block table: EPB elements per block within each case
+    const int epb = epb_for(eng->N);
+    std::vector<int4> blocks;
+    std::vector<int> pfirst(ncases + 1, 0);
+    k0 = 0;
+    for (int c = 0; c < ncases; ++c) {
+        pfirst[c] = (int)blocks.size();
+        for (long e = 0; e < Kc[c]; e += epb)
+            blocks.push_back(make_int4((int)(k0 + e), (int)std::min<long>(epb, Kc[c] - e), c, e == 0 ? 1 : 0));
+        k0 += Kc[c];
+    }
+    pfirst[ncases] = (int)blocks.size();
+    eng->nblocks_multi = (int)blocks.size();
+    while ((long)blocks.size() < grid_for(eng->nblocks_multi)) blocks.push_back(make_int4(0, 0, 0, 0));
+    std::vector<hxk::CaseConst> cc(ncases);
+    for (int c = 0; c < ncases; ++c) {
+        const hx3d_case_desc &d = descs[c];
+        const double M2 = d.mach * d.mach;   // as set_physics forms them
+        cc[c].gammaM2 = eng->ph.gamma * M2;
+        cc[c].mu = 1.0 / d.reynolds;
+        cc[c].mu_to_kappa = 1.0 / (eng->ph.gm1 * M2 * d.prandtl);
+        cc[c].les_c = d.les_c;
+    }
+    if ((long)eng->F * nf / 256 + 1 < ncases)
+        return eng->fail(HX3D_EUNSUPPORTED, "more cases than face blocks to reduce their dt");
+    // per-case replacements of the one-case dt slots, flags and time
+    (void)hipStreamSynchronize(eng->stream);
+    (void)hipFree(eng->dBits);
+    (void)hipFree(eng->dFlag);
+    (void)hipHostFree(eng->hFlag);
+    eng->dBits = nullptr; eng->dFlag = nullptr; eng->hFlag = nullptr;
+    eng->ncases = ncases;
+    const std::vector<int> ones(ncases, 1);
+    if ((rc = dalloc(eng, &eng->dBits, (size_t)2 * DT_SLOTS * ncases)) || (rc = dalloc(eng, &eng->dFlag, (size_t)1 + ncases)) ||
+        (rc = dalloc(eng, &eng->dBlocks, blocks.size())) || (rc = dalloc(eng, &eng->dCC, cc.size())) ||
+        (rc = dalloc(eng, &eng->dActive, (size_t)ncases)) || (rc = dalloc(eng, &eng->dFcase, fcase.size())) ||
+        (rc = dalloc(eng, &eng->dPfirst, pfirst.size())) || (rc = dalloc(eng, &eng->dTimeM, (size_t)ncases)))
+        return rc;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(eng->dBlocks, blocks.data(), blocks.size() * sizeof(int4)) || !up(eng->dCC, cc.data(), cc.size() * sizeof(hxk::CaseConst)) ||
+        !up(eng->dActive, ones.data(), ones.size() * 4) || !up(eng->dFcase, fcase.data(), fcase.size() * 4) ||
+        !up(eng->dPfirst, pfirst.data(), pfirst.size() * 4))
+        return eng->fail(HX3D_EDEVICE, "upload failed");
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, eng->stream, eng->dTimeM, (long)ncases, 0.0);
+    eng->dTime = eng->dTimeM;
+    eng->mc = hxk::Multi{eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, ncases};
+    if ((rc = init_flags(eng))) return rc;
+    if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "initialisation failed");
+    return HX3D_OK;
+}
+
+int hx3d_multi_set_active(hx3d_engine *eng, int c, int on)
+{
+    if (!eng || !eng->mc.blocks || c < 0 || c >= eng->ncases) return HX3D_EINVAL;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, eng->stream, eng->dActive + c, 1, on ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return HX3D_OK;
+}
+
+int hx3d_multi_status(hx3d_engine *eng, double *times, int *diverged)
+{
+    if (!eng || !eng->mc.blocks) return HX3D_EINVAL;
+    if (times) HIPCHK(hipMemcpyAsync(times, eng->dTime, sizeof(double) * eng->ncases, hipMemcpyDeviceToHost, eng->stream));
+    if (diverged) HIPCHK(hipMemcpyAsync(diverged, eng->dFlag + 1, sizeof(int) * eng->ncases, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
 int hx3d_engine_destroy(hx3d_engine *eng)
 {
     if (!eng) return HX3D_OK;
@@ -1093,7 +1253,8 @@ int hx3d_engine_destroy(hx3d_engine *eng)
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff, eng->dElist,
-                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart};
+                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart,
+                    eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, eng->dTimeM};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : eng->evpool) (void)hipEventDestroy(e);
@@ -1117,7 +1278,7 @@ int hx3d_set_state_device(hx3d_engine *eng, const double *Qd)
     // a new state restarts the RK register (the reference allocates G_NS zeroed)
     const long nb2 = (NV * eng->S + 255) / 256;
     hipLaunchKernelGGL(k_fill, dim3(nb2), dim3(256), 0, eng->stream, eng->dG, (long)NV * eng->S, 0.0);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, eng->stream, eng->dFlag + 1, 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, eng->stream, eng->dFlag + 1, eng->ncases, 0);
     queue_flag_readback(eng);
     eng->qdot_valid = false;
     HIPCHK(hipGetLastError());
@@ -1130,6 +1291,18 @@ int hx3d_get_state_device(hx3d_engine *eng, double *Qd)
     const long nb = (eng->nodes + 255) / 256;
     hipLaunchKernelGGL(k_pack_out, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, Qd, eng->nodes, eng->S);
     HIPCHK(hipGetLastError());
+    return HX3D_OK;
+}
+
+int hx3d_get_state_nodes(hx3d_engine *eng, long first, long count, double *Qh)
+{
+    if (!eng || !Qh || first < 0 || count < 0 || first + count > eng->nodes) return HX3D_EINVAL;
+    if (count == 0) return HX3D_OK;
+    const long nb = (count + 255) / 256;
+    // pack the range: node first + g of the SoA state to Qh-ordered staging
+    hipLaunchKernelGGL(k_pack_out, dim3(nb), dim3(256), 0, eng->stream, eng->dQ + first, eng->dStage, count, eng->S);
+    HIPCHK(hipMemcpyAsync(Qh, eng->dStage, sizeof(double) * NV * count, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
     return HX3D_OK;
 }
 
@@ -1243,13 +1416,14 @@ static int finish_steps(hx3d_engine *eng, double *t_inout)
 int hx3d_advance_dt(hx3d_engine *eng, int nsteps, double dt, double *t_inout)
 {
     if (!eng || nsteps < 0 || !(dt > 0.0)) return HX3D_EINVAL;
+    if (eng->mc.blocks && t_inout) return eng->fail(HX3D_EINVAL, "several cases: pass no time, read them with hx3d_multi_status");
     if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
     DtArgs a = host_dt(eng, dt);
     a.nan = eng->dFlag + 1;
     for (int s = 0; s < nsteps; ++s)
         for (int k = 0; k < 3; ++k) {
             DtArgs b = a;
-            if (k == 2) b.time = eng->dScal + 1;   // t += dt once per step, on the device
+            if (k == 2) b.time = eng->dTime;   // t += dt once per step, on the device
             stage(eng, RK_A[k], RK_C[k], 1, b, k == 2 && s == nsteps - 1 ? eng->dQDot : nullptr);
         }
     HIPCHK(hipGetLastError());
@@ -1260,6 +1434,7 @@ int hx3d_advance_dt(hx3d_engine *eng, int nsteps, double dt, double *t_inout)
 int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt)
 {
     if (!eng || !dt) return HX3D_EINVAL;
+    if (eng->mc.blocks) return eng->fail(HX3D_EUNSUPPORTED, "several cases: each case's dt is formed on the device by hx3d_advance");
     launch_maxdt(eng, cfl, dcfl, 3);
     HIPCHK(hipMemcpyAsync(dt, eng->dScal, sizeof(double), hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
@@ -1269,6 +1444,7 @@ int hx3d_max_dt(hx3d_engine *eng, double cfl, double dcfl, double *dt)
 int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *t_inout)
 {
     if (!eng || nsteps < 0) return HX3D_EINVAL;
+    if (eng->mc.blocks && t_inout) return eng->fail(HX3D_EINVAL, "several cases: pass no time, read them with hx3d_multi_status");
     if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
     const size_t nev = eng->timing ? (size_t)12 * nsteps : 0;
     while (eng->evpool.size() < nev) {
@@ -1282,7 +1458,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
     // the slot before any volume kernel of the step reads dt.  Every step issues the same
     // launches with the same arguments: ev, when given, holds 12 events for the step.
     auto issue_step = [eng, cfl, dcfl](hipEvent_t *ev, double *qdot) {
-        unsigned long long *slot = eng->dBits;
+        unsigned long long *slot = dt_slot(eng, 0);
         int nparts;
         if (eng->ph.euler) {
             const long nb = (eng->nodes + 255) / 256;
@@ -1304,7 +1480,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
                 if (!eng->ph.euler) a.lim = hxk::DtLimits{eng->dDtPart, cfl, dcfl, eng->dcsi};
                 a.red = hxk::DtReduce{eng->dDtPart, nparts, slot};
             }
-            if (k == 2) a.time = eng->dScal + 1;
+            if (k == 2) a.time = eng->dTime;
             a.nan = eng->dFlag + 1;
             stage(eng, RK_A[k], RK_C[k], 1, a, k == 2 ? qdot : nullptr, ev ? ev + 4 * k : nullptr);
         }
@@ -1438,7 +1614,8 @@ int hx3d_query(hx3d_engine *eng, int *done)
     if (e == hipSuccess) *done = 1;
     else if (e == hipErrorNotReady) *done = 0;
     else return eng->fail(HX3D_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
-    if (*done && *eng->hFlag) return eng->fail(HX3D_ENAN, "numerical divergence: a NaN appeared in the state");
+    if (*done && !eng->mc.blocks && *eng->hFlag)
+        return eng->fail(HX3D_ENAN, "numerical divergence: a NaN appeared in the state");
     return HX3D_OK;
 }
 

@@ -361,7 +361,7 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
     if (dtl.part)
         hx::dt_limits(ph, q, ja, jg, ops.dcsi[Nx], ops.dcsi[Ny], ops.dcsi[Nz], dtl.cfl, dtl.dcfl, tc, tv);
     }
-    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, n);
+    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, n, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -693,7 +693,7 @@ km_maxdt(const double *__restrict__ Q, const double *__restrict__ Ja, const doub
         for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
         hx::dt_limits(ph, q, ja, Jac[g], dc, de, dz, cfl, dcfl, tc, tv);
     }
-    hxk::block_min_pair(tc, tv, part, threadIdx.x);
+    hxk::block_min_pair(tc, tv, part, threadIdx.x, blockIdx.x);
 }
 
 }  // namespace hxm

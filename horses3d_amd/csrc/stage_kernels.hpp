@@ -77,7 +77,51 @@ __device__ __forceinline__ double operator_entry(const double *src, int t)
     return src[(x / n1) * 8 + (x % n1)];
 }
 
-__device__ __forceinline__ void block_min_pair(double a, double b, double *part, int t);
+__device__ __forceinline__ void block_min_pair(double a, double b, double *part, int t, int slot);
+
+// Several independent cases of one order in one engine (batching): each case has its own
+// physics constants, dt, time, divergence flag and active mask; element blocks never straddle
+// cases, so a block's case and constants are uniform.  All pointers null: one case.
+struct CaseConst {
+    double gammaM2, mu, mu_to_kappa, les_c;   // the constants that differ between cases
+};
+struct Multi {
+    const int4 *blocks;     // per logical block: first element, element count, case, first block of its case
+    const CaseConst *cc;    // [cases]
+    const int *active;      // [cases]: 0 = finished, its blocks and faces do no work
+    const int *fcase;       // [faces]: the case of each face
+    const int *pfirst;      // [cases + 1]: each case's range of logical blocks (its dt partials)
+    int cases;
+};
+
+__device__ __forceinline__ void case_physics(Phys &ph, const CaseConst &c)
+{
+    ph.gammaM2 = c.gammaM2;
+    ph.mu = c.mu;
+    ph.mu_to_kappa = c.mu_to_kappa;
+    ph.les_c = c.les_c;
+}
+
+// The elements of logical block lb: first element e0, count cnt (0: nothing to do), case cs,
+// and whether the block is its case's first.  One case: EPB elements per block.
+template <int EPB>
+__device__ __forceinline__ void block_elements(const Multi &mc, int lb, int K, int &e0, int &cnt, int &cs, bool &first,
+                                               Phys &ph)
+{
+    if (mc.blocks) {
+        const int4 b = mc.blocks[lb];
+        e0 = b.x;
+        cs = b.z;
+        cnt = mc.active[cs] ? b.y : 0;
+        first = b.w != 0;
+        case_physics(ph, mc.cc[cs]);
+    } else {
+        e0 = lb * EPB;
+        cnt = e0 < K ? (K - e0 < EPB ? K - e0 : EPB) : 0;
+        cs = 0;
+        first = lb == 0;
+    }
+}
 
 // MaxTimeStep of the state a stage starts from, accumulated by the stage's gradient kernel
 // (part == null: not this stage): block minima of the convective and viscous limits to
@@ -114,24 +158,30 @@ __global__ void __launch_bounds__(((EPB * (N + 1) * (N + 1) * (N + 1) + 63) / 64
 __attribute__((amdgpu_waves_per_eu(HX3D_GRAD_WAVES)))
 k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double *__restrict__ Jac,
        const int *__restrict__ nbr, const double *__restrict__ egeo, double *__restrict__ vgrad,
-       int K, long S, long ES, Ops ops, Phys ph, DtLimits dtl)
+       int K, long S, long ES, Ops ops, Phys ph, DtLimits dtl, Multi mc)
 {
     constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
     __shared__ double sQ[EPB][NV][ne];
     __shared__ double sD[n1 * n1];
-    const int nb = (K + EPB - 1) / EPB;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int t = threadIdx.x;
+    int e0, cnt, cs;
+    bool first;
+    block_elements<EPB>(mc, lb, K, e0, cnt, cs, first, ph);
+    if (cnt == 0) {   // uniform: a padding block or a finished case
+        if (dtl.part && t == 0) { dtl.part[2 * lb] = INFINITY; dtl.part[2 * lb + 1] = INFINITY; }
+        return;
+    }
     const int el = t / ne, n = t - el * ne;
-    const int e = lb * EPB + el;
-    const bool active = lb < nb && el < EPB && e < K;
+    const int e = e0 + el;
+    const bool active = el < cnt;
     const unsigned g = (unsigned)e * ne + n;
     const int i = n % n1, j = (n / n1) % n1, k = n / nf;
     // Every load is unconditional (padding threads read a clamped element and discard it; see
     // k_volume).  The node's (at most three) faces: x-face slot 5 / 3, y-face 0 / 1, z-face
     // 2 / 4; faces the node is not on get zero traces and zero geometry, so their lifting term
     // is an exact zero and the accumulation below is branch-free.
-    const int ec = e < K ? e : K - 1;
+    const int ec = active ? e : e0 + cnt - 1;
     const unsigned gc = (unsigned)ec * ne + n;
     const bool onx = i == 0 || i == N, ony = j == 0 || j == N, onz = k == 0 || k == N;
     const unsigned fx = efx<N>(ec, i == 0 ? 5 : 3, j, k), fy = efx<N>(ec, j == 0 ? 0 : 1, i, k),
@@ -236,7 +286,7 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
     for (int c = 0; c < 12; ++c) stg(vgrad + c * S, g, vg[c]);
     if (dtl.part) hx::dt_limits(ph, q, ja, jg, dtl.dcsi, dtl.dcsi, dtl.dcsi, dtl.cfl, dtl.dcfl, tc, tv);
     }
-    if (dtl.part) block_min_pair(tc, tv, dtl.part, t);
+    if (dtl.part) block_min_pair(tc, tv, dtl.part, t, lb);
 }
 
 // ------------------------------------------------------------------------------------
@@ -247,8 +297,8 @@ k_grad(const double *__restrict__ Q, const double *__restrict__ Ja, const double
 // ------------------------------------------------------------------------------------
 __device__ __forceinline__ void reduce_dt_partials(const double *part, int n, unsigned long long *bits, int t);
 
-// dtred: (partials, count, slot) of the previous step's dt limits, reduced by block 0 first
-// (or null)
+// dtred: (partials, count, slot) of the step's dt limits, reduced by block 0 first (or null);
+// with several cases, logical block c < cases reduces case c's partials into slot c
 struct DtReduce {
     const double *part;
     int n;
@@ -263,12 +313,25 @@ __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(HX3D_FACE_WAVES)))
 k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ fgeo,
        const int4 *__restrict__ fnode, const double *__restrict__ fdelta, double *__restrict__ fe,
-       long FS, long S, long ES, double b0, Phys ph, DtReduce dtr)
+       long FS, long S, long ES, double b0, Phys ph, DtReduce dtr, Multi mc)
 {
     constexpr int n1 = N + 1, nf = n1 * n1;
-    if (dtr.bits && blockIdx.x == 0) reduce_dt_partials(dtr.part, dtr.n, dtr.bits, threadIdx.x);
-    const unsigned fq = (unsigned)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    if (dtr.bits) {
+        if (!mc.blocks) {
+            if (blockIdx.x == 0) reduce_dt_partials(dtr.part, dtr.n, dtr.bits, threadIdx.x);
+        } else if (lb < mc.cases) {   // case lb's dt (its blocks' partials)
+            const int p0 = mc.pfirst[lb];
+            reduce_dt_partials(dtr.part + 2 * p0, mc.pfirst[lb + 1] - p0, dtr.bits + 2 * lb, threadIdx.x);
+        }
+    }
+    const unsigned fq = (unsigned)lb * blockDim.x + threadIdx.x;
     if (fq >= FS) return;
+    if (mc.blocks) {
+        const int cs = mc.fcase[fq / nf];
+        if (!mc.active[cs]) return;
+        case_physics(ph, mc.cc[cs]);
+    }
     const int4 gi = ldg(fnode, fq);
     const unsigned gx = gi.x, gy = gi.y, gz = gi.z, gw = gi.w;
     // face frame: fgeo[c*FS + fq], c = n (0..2), t1 (3..5), t2 (6..8), |Ja| (9)
@@ -345,7 +408,7 @@ __device__ __forceinline__ double bits_to_double(unsigned long long b) { return 
 // it (inactive threads with +inf).  The minimum of doubles is exact and independent of order,
 // so the two-level reduction (here, then k_dt_reduce) gives the same bits as a serial one,
 // without contended atomics.
-__device__ __forceinline__ void block_min_pair(double a, double b, double *part, int t)
+__device__ __forceinline__ void block_min_pair(double a, double b, double *part, int t, int slot)
 {
     __shared__ double sm[2][16];
     for (int o = 32; o > 0; o >>= 1) {
@@ -357,8 +420,8 @@ __device__ __forceinline__ void block_min_pair(double a, double b, double *part,
     if (t == 0) {
         const int nw = (blockDim.x + 63) >> 6;
         for (int w = 1; w < nw; ++w) { a = fmin(a, sm[0][w]); b = fmin(b, sm[1][w]); }
-        part[2 * blockIdx.x] = a;
-        part[2 * blockIdx.x + 1] = b;
+        part[2 * slot] = a;
+        part[2 * slot + 1] = b;
     }
 }
 
@@ -407,9 +470,12 @@ __host__ __device__ constexpr int pair_slot(int n1, int lo, int hi)
 // Kennedy-Gruber and Pirozzoli up to N = 5 (no spills in 168 VGPRs up to N = 4; at N = 5 the
 // 10 spilled registers cost less than a third wave gains), else no constraint (at N = 6 the
 // forced third wave measured 14% slower than two waves without spills).
+#ifndef HX3D_VOL_WAVES
+#define HX3D_VOL_WAVES 3
+#endif
 constexpr int volume_waves(int n, int form)
 {
-    return form == -1 ? 3 : (form == 2 || form == 3) && n <= 5 ? 3 : 1;
+    return form == -1 ? 3 : (form == 2 || form == 3) && n <= 5 ? HX3D_VOL_WAVES : 1;
 }
 
 template <int N, int EPB, int FORM>
@@ -418,7 +484,7 @@ __attribute__((amdgpu_waves_per_eu(volume_waves(N, FORM))))
 k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ Ja,
          const double *__restrict__ Jac, const double *__restrict__ edelta, const double *__restrict__ fe,
          double *__restrict__ G, double *__restrict__ qdot_out, double a_k, double c_k, int update,
-         DtArgs dta, int K, long S, long ES, Ops ops, Phys ph)
+         DtArgs dta, int K, long S, long ES, Ops ops, Phys ph, Multi mc)
 {
     constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
     constexpr bool SPLIT = FORM >= 0;
@@ -435,12 +501,15 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     __shared__ double sR[EPB][RSZ];
     __shared__ double sHat[n1 * n1], sSharp[n1 * n1];
     __shared__ int sSlot[n1 * n1];   // pair table offset of (own, l); 0 on the diagonal
-    const int nb = (K + EPB - 1) / EPB;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int t = threadIdx.x;
+    int e0, cnt, cs;
+    bool first;
+    block_elements<EPB>(mc, lb, K, e0, cnt, cs, first, ph);
+    if (cnt == 0) return;   // uniform: a padding block or a finished case
     const int el = t / ne, n = t - el * ne;
-    const int e = lb * EPB + el;
-    const bool active = lb < nb && el < EPB && e < K;
+    const int e = e0 + el;
+    const bool active = el < cnt;
     const int i = n % n1, j = (n / n1) % n1, k = n / nf;
     const bool onx = i == 0 || i == N, ony = j == 0 || j == N, onz = k == 0 || k == N;
     // ---- Loads are unconditional (padding threads read a clamped element and discard it): a
@@ -448,7 +517,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     // of any earlier load.  The face values are issued first and summed once the node's own
     // data has arrived (loads complete in order); G and dt are issued at the last direction
     // and consumed after it.
-    const int ec = e < K ? e : K - 1;
+    const int ec = active ? e : e0 + cnt - 1;
     const unsigned gc = (unsigned)ec * ne + n, gw = (unsigned)e * ne + n;
     const double hop = operator_entry<n1>(ops.hatD, t), sop = operator_entry<n1>(ops.sharpD, t);
     double sx[NV], sy[NV], sz[NV];
@@ -553,8 +622,8 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
         if (d == 2) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) gk[v] = ldg(G + v * S, gc);
-            dt0 = bits_to_double(dta.in[0]);
-            dt1 = bits_to_double(dta.in[1]);
+            dt0 = bits_to_double(dta.in[2 * cs]);
+            dt1 = bits_to_double(dta.in[2 * cs + 1]);
         }
         if (!SPLIT) {
             if (active) {
@@ -643,7 +712,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             hold(dt0);
             hold(dt1);
             const double dt = fmin(dt0, dt1);
-            if (dta.time && lb == 0 && t == 0) dta.time[0] += dt;
+            if (dta.time && first && t == 0) dta.time[cs] += dt;
             const double cdt = c_k * dt;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -656,7 +725,7 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
             // checkForNan (ExplicitMethods.f90:1856-1905) of the updated state: a sticky flag,
             // written only where a NaN appears
             if (dta.nan && (isnan(q[0]) || isnan(q[1]) || isnan(q[2]) || isnan(q[3]) || isnan(q[4])))
-                atomicOr(dta.nan, 1);
+                atomicOr(dta.nan + cs, 1);
         }
     }
 }

@@ -62,8 +62,9 @@ def _error(msg, code):
 
 _lib = None
 
-EXPORTS = ["hx3d_engine_create", "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
-           "hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device",
+EXPORTS = ["hx3d_engine_create", "hx3d_engine_create_multi", "hx3d_multi_set_active", "hx3d_multi_status",
+           "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
+           "hx3d_set_state", "hx3d_get_state", "hx3d_get_state_nodes", "hx3d_set_state_device", "hx3d_get_state_device",
            "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_set_ctd_after_steps", "hx3d_get_qdot", "hx3d_max_dt", "hx3d_advance", "hx3d_advance_dt", "hx3d_residuals",
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_query", "hx3d_get_time", "hx3d_engine_device_bytes",
            "hx3d_device_memory", "hx3d_set_timing", "hx3d_set_step_graph",
@@ -86,12 +87,16 @@ def load_library(path=LIB_PATH):
     P, I, D = C.c_void_p, C.c_int, C.c_double
     L.hx3d_engine_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)]
     L.hx3d_engine_destroy.argtypes = [P]
+    L.hx3d_engine_create_multi.argtypes = [P, I, C.POINTER(C.c_void_p)]
+    L.hx3d_multi_set_active.argtypes = [P, I, I]
+    L.hx3d_multi_status.argtypes = [P, P, P]
     L.hx3d_last_error.argtypes = [P]
     L.hx3d_last_error.restype = C.c_char_p
     L.hx3d_num_nodes.argtypes = [P]
     L.hx3d_num_nodes.restype = C.c_long
     for fn in ("hx3d_set_state", "hx3d_get_state", "hx3d_set_state_device", "hx3d_get_state_device"):
         getattr(L, fn).argtypes = [P, P]
+    L.hx3d_get_state_nodes.argtypes = [P, C.c_long, C.c_long, P]
     L.hx3d_compute_qdot.argtypes = [P, D, P]
     L.hx3d_rk3_step.argtypes = [P, D, D]
     L.hx3d_set_ctd_after_steps.argtypes = [P, I]
@@ -284,43 +289,116 @@ def read_control(path, root=None):
         c.close()
 
 
+def _desc(case, keep, mach, reynolds, prandtl=0.72, riemann="roe", inviscid="standard", averaging="standard",
+          les=None, les_c=None, lambda_stab=1.0, euler=False):
+    """A CaseDesc of a case dict; the arrays it points to are kept in `keep`."""
+    keep.update(
+        N=np.ascontiguousarray(case["N"], dtype=np.int32),
+        faceID=np.ascontiguousarray(case["faceID"], dtype=np.int32),
+        faceSide=np.ascontiguousarray(case["faceSide"], dtype=np.int32),
+        fElem=np.ascontiguousarray(case["fElem"], dtype=np.int32),
+        fESide=np.ascontiguousarray(case["fESide"], dtype=np.int32),
+        fRot=np.ascontiguousarray(case["fRot"], dtype=np.int32),
+        Ja=np.ascontiguousarray(case["Ja"], dtype=np.float64),
+        J=np.ascontiguousarray(case["J"], dtype=np.float64))
+    face_geo = all(x in case for x in ("normal", "t1", "t2", "fjac"))
+    if face_geo:
+        for x in ("normal", "t1", "t2", "fjac"):
+            keep[x] = np.ascontiguousarray(case[x], dtype=np.float64)
+    k = keep
+    fg = (lambda x: _p(k[x])) if face_geo else (lambda x: None)
+    les_id = LES[les]
+    return CaseDesc(int(case["K"]), _p(k["N"]), _p(k["faceID"]), _p(k["faceSide"]), int(case["F"]),
+                    _p(k["fElem"]), _p(k["fESide"]), _p(k["fRot"]), _p(k["Ja"]), _p(k["J"]),
+                    fg("normal"), fg("t1"), fg("t2"), fg("fjac"), mach, reynolds, prandtl, 1 if euler else 0,
+                    RIEMANN[riemann], INVISCID[inviscid], AVERAGING[averaging], lambda_stab, les_id,
+                    LES_DEFAULT_C[les_id] if les_c is None else les_c)
+
+
 class Engine:
     """Device-resident DGSEM case.  State arrays on the host are (nodes, 5) float64."""
 
     def __init__(self, case, mach, reynolds, prandtl=0.72, riemann="roe", inviscid="standard",
                  averaging="standard", les=None, les_c=None, lambda_stab=1.0, euler=False):
         L = load_library()
-        self._keep = dict(
-            N=np.ascontiguousarray(case["N"], dtype=np.int32),
-            faceID=np.ascontiguousarray(case["faceID"], dtype=np.int32),
-            faceSide=np.ascontiguousarray(case["faceSide"], dtype=np.int32),
-            fElem=np.ascontiguousarray(case["fElem"], dtype=np.int32),
-            fESide=np.ascontiguousarray(case["fESide"], dtype=np.int32),
-            fRot=np.ascontiguousarray(case["fRot"], dtype=np.int32),
-            Ja=np.ascontiguousarray(case["Ja"], dtype=np.float64),
-            J=np.ascontiguousarray(case["J"], dtype=np.float64))
-        face_geo = all(x in case for x in ("normal", "t1", "t2", "fjac"))
-        if face_geo:
-            for x in ("normal", "t1", "t2", "fjac"):
-                self._keep[x] = np.ascontiguousarray(case[x], dtype=np.float64)
-        k = self._keep
-        fg = (lambda x: _p(k[x])) if face_geo else (lambda x: None)
-        les_id = LES[les]
-        d = CaseDesc(int(case["K"]), _p(k["N"]), _p(k["faceID"]), _p(k["faceSide"]), int(case["F"]),
-                     _p(k["fElem"]), _p(k["fESide"]), _p(k["fRot"]), _p(k["Ja"]), _p(k["J"]),
-                     fg("normal"), fg("t1"), fg("t2"), fg("fjac"), mach, reynolds, prandtl, 1 if euler else 0, RIEMANN[riemann],
-                     INVISCID[inviscid], AVERAGING[averaging], lambda_stab, les_id,
-                     LES_DEFAULT_C[les_id] if les_c is None else les_c)
+        self._keep = {}
+        d = _desc(case, self._keep, mach, reynolds, prandtl, riemann, inviscid, averaging, les, les_c,
+                  lambda_stab, euler)
         h = C.c_void_p()
         rc = L.hx3d_engine_create(C.byref(d), C.byref(h))
+        self._created(L, h, rc, "hx3d_engine_create")
+
+    def _created(self, L, h, rc, what):
         self.h = h
         if rc != 0:
             msg = L.hx3d_last_error(h).decode() if h else "create failed"
             L.hx3d_engine_destroy(h)
             self.h = None
-            raise HX3DError("hx3d_engine_create: %s (code %d)" % (msg, rc))
+            raise _error("%s: %s (code %d)" % (what, msg, rc), rc)
         self.nodes = int(L.hx3d_num_nodes(self.h))
         self.t = 0.0
+
+
+class MultiEngine(Engine):
+    """Several independent cases of one order in one engine (hx3d_engine_create_multi): each
+    case keeps its own physics constants (Mach, Reynolds and Prandtl numbers, LES constant), its
+    own dt (MaxTimeStep of its own state, every step), time and divergence flag, and can be
+    switched off when it has taken its steps; the discretisation is shared.  Every kernel launch
+    carries all the active cases.  The state arrays are the cases' states one after the other
+    (`slices`)."""
+
+    def __init__(self, cases, physics):
+        L = load_library()
+        if isinstance(physics, dict):
+            physics = [physics] * len(cases)
+        self._keeps = [{} for _ in cases]
+        descs = (CaseDesc * len(cases))(*[_desc(c, k, **ph) for c, k, ph in zip(cases, self._keeps, physics)])
+        self._descs = descs
+        h = C.c_void_p()
+        rc = L.hx3d_engine_create_multi(descs, len(cases), C.byref(h))
+        self._created(L, h, rc, "hx3d_engine_create_multi")
+        self.ncases = len(cases)
+        n = [int(c["off"][-1]) if "off" in c else len(c["J"]) for c in cases]
+        o = np.concatenate([[0], np.cumsum(n)])
+        self.slices = [slice(int(a), int(b)) for a, b in zip(o[:-1], o[1:])]
+
+    def set_states(self, states):
+        Q = np.empty((self.nodes, 5))
+        for s, q in zip(self.slices, states):
+            Q[s] = q
+        self.set_state(Q)
+
+    def get_states(self):
+        Q = self.get_state()
+        return [Q[s].copy() for s in self.slices]
+
+    def get_case_state(self, c):
+        """Case c's state alone (waits for the queued work)."""
+        s = self.slices[c]
+        out = np.empty((s.stop - s.start, 5))
+        self._chk(load_library().hx3d_get_state_nodes(self.h, s.start, s.stop - s.start, _p(out)), "hx3d_get_state_nodes")
+        return out
+
+    def set_active(self, c, on):
+        """Case c takes part in the following steps (on) or is left as it is (off)."""
+        self._chk(load_library().hx3d_multi_set_active(self.h, int(c), 1 if on else 0), "hx3d_multi_set_active")
+
+    def status(self):
+        """(times, diverged flags) of the cases once the queued work is done."""
+        t = np.zeros(self.ncases)
+        f = np.zeros(self.ncases, dtype=np.int32)
+        self._chk(load_library().hx3d_multi_status(self.h, _p(t), _p(f)), "hx3d_multi_status")
+        return t, f.astype(bool)
+
+    def advance(self, nsteps, cfl, dcfl, t=None):
+        """Queues nsteps steps of every active case, each with its own CFL dt; returns at once."""
+        assert t is None, "several cases: read their times with status()"
+        self._chk(load_library().hx3d_advance(self.h, nsteps, cfl, dcfl, None), "hx3d_advance")
+
+    def done(self):
+        d = C.c_int()
+        self._chk(load_library().hx3d_query(self.h, C.byref(d)), "hx3d_query")
+        return bool(d.value)
 
     def _chk(self, rc, what):
         if rc != 0:

@@ -373,7 +373,9 @@ def taylor_green(case, mach, phase=(0.0, 0.0, 0.0), gamma=1.4, noise=0.0, seed=0
         u = u + noise * r.standard_normal(u.shape)
         v = v + noise * r.standard_normal(u.shape)
         w = w + noise * r.standard_normal(u.shape)
-    p = 1.0 / (gamma * mach * mach) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    # 1/(gamma M^2) with gamma M^2 formed as gamma * (M * M), as the reference forms it
+    # (PhysicsStorage_NS.f90:250) for its initial conditions
+    p = 1.0 / (gamma * (mach * mach)) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
     rho = np.ones_like(u)
     Q = np.stack([rho, rho * u, rho * v, rho * w, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)], axis=1)
     return np.ascontiguousarray(Q)

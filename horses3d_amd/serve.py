@@ -241,3 +241,163 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
         results[i] = Result(r, Q, t, admitted, clock() - t0, error=err)
     wall = clock() - t0
     return results, _stats(results, wall)
+
+
+def batch_key(r):
+    """Requests that can share one engine: one order N everywhere and the same discretisation."""
+    N = np.asarray(r.case["N"])
+    n0 = int(N.flat[0])
+    if not (N == n0).all():
+        return None   # mixed orders: served alone
+    ph = r.physics
+    return (n0, ph.get("riemann", "roe"), ph.get("inviscid", "standard"), ph.get("averaging", "standard"),
+            ph.get("les"), ph.get("lambda_stab", 1.0), float(r.cfl), float(r.dcfl))
+
+
+class _Batch:
+    """A running several-case engine: the requests it holds, the steps each has left, and which
+    are still active."""
+
+    def __init__(self, idx, eng, admitted, steps):
+        self.idx, self.eng, self.admitted = idx, eng, admitted
+        self.left = list(steps)
+        self.live = [True] * len(idx)
+
+
+def serve_batched(requests, chunk=50, max_batch=64, max_batches=3, min_batch=8, max_wait_s=0.05,
+                  hbm_reserve=8 << 30, poll_s=2e-4, batch_factory=None, device_memory=None,
+                  clock=time.perf_counter, sleep=time.sleep, workers=2, progress=None):
+    """Continuous batching with shared launches: arrived requests that can share an engine (one
+    order, one discretisation, see batch_key) are gathered into several-case engines
+    (engine.MultiEngine), each case with its own physics numbers, CFL dt, time and step count.
+    A group is formed when min_batch such requests wait, when the oldest has waited max_wait_s,
+    or when the device has nothing else to do, up to max_batch cases and the free HBM (less a
+    reserve); at most max_batches engines run at once (their streams overlap on the device).
+    An engine takes chunks of at most `chunk` steps, cut so that no case overshoots its step
+    count; a case that has taken its steps, or whose state holds a NaN, is switched off and
+    its state read back while the others go on.  Returns (results in request order, stats)."""
+    make = batch_factory or (lambda rs: E.MultiEngine([r.case for r in rs], [r.physics for r in rs]))
+    freemem = device_memory or (lambda: E.device_memory()[0])
+    order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
+    pending = list(order)
+    waiting = {}     # key -> [(request index, arrival)]
+    preparing = []   # (future, idx list, reserved bytes)
+    running = []     # _Batch
+    closing = []     # futures of engines being freed
+    results = [None] * len(requests)
+    per_node = [None]
+    done_count = [0]
+
+    def prepare(rs):
+        eng = make(rs)
+        eng.set_states([r.Q0 for r in rs])
+        b = eng.device_bytes()
+        if b > 0:
+            per_node[0] = max(per_node[0] or 0.0, b / sum(r.nodes for r in rs))
+        return eng
+
+    def read_case(eng, pos):
+        return eng.get_case_state(pos)
+
+    def close(eng):
+        eng.close()
+
+    pool = ThreadPoolExecutor(max_workers=max(1, workers)) if workers else None
+    submit = pool.submit if pool else _run_now
+    t0 = clock()
+    try:
+        while pending or waiting or preparing or running or closing:
+            now = clock() - t0
+            while pending and requests[pending[0]].arrival <= now:
+                i = pending.pop(0)
+                key = batch_key(requests[i])
+                waiting.setdefault(key if key is not None else ("alone", i), []).append(i)
+            # form groups
+            idle_device = not running and not preparing
+            for key in list(waiting):
+                lst = waiting[key]
+                if len(running) + len(preparing) >= max_batches:
+                    break
+                oldest = min(requests[i].arrival for i in lst)
+                if not (len(lst) >= min_batch or now - oldest >= max_wait_s or idle_device):
+                    continue
+                held = sum(b for _, _, b in preparing)
+                room = freemem() - hbm_reserve - held
+                take, need = [], 0.0
+                for i in lst[:max_batch]:
+                    nb_ = (per_node[0] or BYTES_PER_NODE_GUESS) * requests[i].nodes
+                    if take and need + nb_ > room:
+                        break
+                    take.append(i)
+                    need += nb_
+                if not take or (need > room and (running or preparing)):
+                    continue   # wait for memory
+                waiting[key] = lst[len(take):]
+                if not waiting[key]:
+                    del waiting[key]
+                preparing.append((submit(prepare, [requests[i] for i in take]), take, need))
+                idle_device = False
+            for item in [p for p in preparing if p[0].done()]:
+                preparing.remove(item)
+                f, take, _ = item
+                if f.exception() is not None:
+                    for i in take:
+                        results[i] = Result(requests[i], None, None, clock() - t0, clock() - t0, error=str(f.exception()))
+                        done_count[0] += 1
+                    continue
+                running.append(_Batch(take, f.result(), clock() - t0, [requests[i].steps for i in take]))
+            busy = bool(preparing)
+            for b in list(running):
+                if not b.eng.done():
+                    busy = True
+                    continue
+                # the engine is idle: retire what finished (its state read back here, the others
+                # are untouched by the read), then queue the next chunk
+                times, bad = b.eng.status()
+                for pos, i in enumerate(b.idx):
+                    if b.live[pos] and (bad[pos] or b.left[pos] == 0):
+                        b.live[pos] = False
+                        b.eng.set_active(pos, False)
+                        err = "numerical divergence: a NaN appeared in the state" if bad[pos] else None
+                        results[i] = Result(requests[i], read_case(b.eng, pos), float(times[pos]), b.admitted,
+                                            clock() - t0, error=err)
+                        done_count[0] += 1
+                        if progress and done_count[0] % progress == 0:
+                            import sys
+                            print("[serve] %d of %d cases done at %.1f s" % (done_count[0], len(requests), clock() - t0),
+                                  file=sys.stderr, flush=True)
+                live = [p for p in range(len(b.idx)) if b.live[p]]
+                if live:
+                    n = min(chunk, min(b.left[p] for p in live))
+                    b.eng.advance(n, requests[b.idx[live[0]]].cfl, requests[b.idx[live[0]]].dcfl)
+                    for p in live:
+                        b.left[p] -= n
+                    busy = True
+                else:
+                    running.remove(b)
+                    closing.append(submit(close, b.eng))
+            closing = [f for f in closing if not f.done()]
+            if not busy and pending and not running and not waiting:
+                wait = requests[pending[0]].arrival - (clock() - t0)
+                if wait > 0:
+                    sleep(min(wait, 0.05))
+            elif busy or closing:
+                sleep(poll_s)
+            elif waiting:
+                sleep(poll_s)
+    except BaseException:
+        if pool:
+            pool.shutdown(wait=True)
+            pool = None
+        for b in running:
+            b.eng.close()
+        for f, _, _ in preparing:
+            if f.done() and f.exception() is None:
+                f.result().close()
+        raise
+    finally:
+        if pool:
+            pool.shutdown(wait=True)
+    wall = clock() - t0
+    st = _stats([r for r in results if r is not None], wall)
+    return results, st

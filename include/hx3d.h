@@ -94,12 +94,31 @@ typedef struct hx3d_engine hx3d_engine;
  * then be given at the face orders), every face interior or periodic. */
 int hx3d_engine_create(const hx3d_case_desc *desc, hx3d_engine **out);
 int hx3d_engine_destroy(hx3d_engine *eng);
+/* Batching (no single counterpart in the reference: HORSES3D runs one case per process): ncases
+ * independent cases of one order N in one engine, stepped together in every launch.  The cases
+ * share the discretisation (Riemann solver, volume form, averaging, LES model, lambda) and may
+ * differ in mesh size and geometry and in the physics numbers (Mach, Reynolds and Prandtl
+ * numbers, LES constant, ConstructPhysicsStorage_NS, PhysicsStorage_NS.f90:145-439).  Each
+ * case's dt is MaxTimeStep of its own state (DGSEMClass.f90:870-1035) and each has its own time
+ * and divergence flag, so every case evolves exactly as it does alone, bit for bit.  The state
+ * arrays are the cases' packed states one after the other.  hx3d_advance and hx3d_advance_dt
+ * take t_inout = NULL (they queue the steps); hx3d_max_dt is not available. */
+int hx3d_engine_create_multi(const hx3d_case_desc *descs, int ncases, hx3d_engine **out);
+/* case c takes part in the following steps (on = 1) or keeps its state and time (on = 0);
+ * stream-ordered */
+int hx3d_multi_set_active(hx3d_engine *eng, int c, int on);
+/* waits for the queued work; then times[c] and diverged[c] (1: a NaN appeared in case c's state)
+ * for every case (either may be NULL) */
+int hx3d_multi_status(hx3d_engine *eng, double *times, int *diverged);
 const char *hx3d_last_error(const hx3d_engine *eng);
 long hx3d_num_nodes(const hx3d_engine *eng);
 
 /* State transfer in the packed host layout Q[5g+v] (LoadSolutionForRestart / the QNS copy). */
 int hx3d_set_state(hx3d_engine *eng, const double *Q_host);
 int hx3d_get_state(hx3d_engine *eng, double *Q_host);
+/* count nodes of the resident state from node first on, packed (a case's part of a
+ * several-case engine); waits */
+int hx3d_get_state_nodes(hx3d_engine *eng, long first, long count, double *Q_host);
 /* The same on device memory (packed layout, stream-ordered on the engine stream). */
 int hx3d_set_state_device(hx3d_engine *eng, const double *Q_dev);
 int hx3d_get_state_device(hx3d_engine *eng, double *Q_dev);

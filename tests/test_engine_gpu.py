@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import load_golden, physics_kwargs, rk3_by_stages, GOLDEN, STAGE3
+from conftest import load_golden, physics_kwargs, reference_initial_state, rk3_by_stages, GOLDEN, STAGE3
 from horses3d_amd import mesh
 
 pytestmark = pytest.mark.gpu
@@ -217,39 +217,149 @@ def test_box20_p2to7_matches_oracle():
     eng.close()
 
 
-def test_box48_n5_properties():
-    """The largest single-GPU case at its full size: 48^3 elements of order 5 (23.9 M nodes),
-    split form with Pirozzoli averaging, Roe, BR1, WALE, on the straight box (the curvilinear
-    map of the benchmark case changes no size or index).  Size-independent properties:
-      * QDot commutes with a periodic shift of the state by one element along each axis, bit
-        for bit (every element and face of this box has the same geometry);
+def test_box48_n5_curvilinear_properties():
+    """The largest single-GPU case at its full size and on its own mesh: 48^3 curvilinear
+    elements (x' = x + 0.15 prod_k sin(x_k + phi_dk), phases of seed 5) of order 5 (23.9 M
+    nodes), split form with Pirozzoli averaging, Roe, BR1, WALE.  Size-independent properties:
+      * free-stream preservation: a uniform state has QDot = 0 to round-off on this mesh (the
+        discrete metric identities hold for every element and face);
       * sum_e sum_n w J QDot = 0 per variable (periodic telescoping);
-      * two device-dt RK3 steps keep the state finite and lower the kinetic energy."""
+      * two device-dt RK3 steps keep the state finite and lower the kinetic energy, and the time
+        the engine reports is the sum of the dt it took."""
+    n, N = 48, 5
+    case = mesh.periodic_box(n, N, amplitude=0.15, phase_seed=5)
+    Q = mesh.taylor_green(case, 0.1, noise=1e-3, seed=5)
+    print("48^3 x N=5 curvilinear case built", flush=True)
+    eng = engine_for(case, dict(mach=0.1, reynolds=1600.0, riemann="roe", inviscid="split-form",
+                                averaging="pirozzoli", les="wale", les_c=None))
+    # free stream: rho = 1, u = (0.3, -0.2, 0.1), p = 1/(gamma M^2)
+    U = np.zeros_like(Q)
+    U[:, 0], U[:, 1], U[:, 2], U[:, 3] = 1.0, 0.3, -0.2, 0.1
+    U[:, 4] = 1.0 / (1.4 * 0.01) / 0.4 + 0.5 * (0.09 + 0.04 + 0.01)
+    eng.set_state(U)
+    fs = np.abs(eng.qdot()).max(axis=0)
+    print("48^3 x N=5 free-stream max |QDot|", fs, flush=True)
+    assert np.all(fs <= 1e-10 * np.abs(U).max(axis=0)), fs
+    eng.set_state(Q)
+    qd = eng.qdot()
+    assert np.isfinite(qd).all()
+    w = mesh.gl_nodes(N)[1]
+    wJ = case["J"] * np.tile(np.einsum("k,j,i->kji", w, w, w).ravel(), n ** 3)
+    tot, scale = (wJ[:, None] * qd).sum(axis=0), (wJ[:, None] * np.abs(qd)).sum(axis=0)
+    assert np.all(np.abs(tot) <= 1e-12 * scale), tot / scale
+    ke0 = eng.kinetic_energy()
+    dt0 = eng.max_dt(0.4, 0.4)
+    t = eng.advance(1, 0.4, 0.4, t=0.0)
+    assert t == dt0
+    dt1 = eng.max_dt(0.4, 0.4)
+    t = eng.advance(1, 0.4, 0.4)
+    assert t == dt0 + dt1
+    assert not eng.has_nan()
+    ke1 = eng.kinetic_energy()
+    print("48^3 x N=5 kinetic energy", ke0, ke1, flush=True)
+    assert 0.0 < ke1 < ke0
+    eng.close()
+
+
+def test_box48_n5_shift_commutes():
+    """On the straight 48^3 x N=5 box every element and face has the same geometry, so QDot
+    commutes with a periodic shift of the state by one element along each axis, bit for bit:
+    any element- or face-index error at this size breaks it."""
     n, N = 48, 5
     case = mesh.straight_box(n, N)
     Q = mesh.taylor_green(case, 0.1, noise=1e-3, seed=5)
-    print("48^3 x N=5 case built", flush=True)
     eng = engine_for(case, dict(mach=0.1, reynolds=1600.0, riemann="roe", inviscid="split-form",
                                 averaging="pirozzoli", les="wale", les_c=None))
     eng.set_state(Q)
     qd = eng.qdot()
-    assert np.isfinite(qd).all()
     ne = (N + 1) ** 3
     Qe, qde = Q.reshape(n, n, n, ne, 5), qd.reshape(n, n, n, ne, 5)   # [k, j, i, node, var]
     for axis in (2, 1, 0):
         eng.set_state(np.roll(Qe, -1, axis=axis).reshape(-1, 5))
         assert np.array_equal(eng.qdot().reshape(n, n, n, ne, 5), np.roll(qde, -1, axis=axis)), axis
-    w = mesh.gl_nodes(N)[1]
-    wJ = case["J"] * np.tile(np.einsum("k,j,i->kji", w, w, w).ravel(), n ** 3)
-    tot, scale = (wJ[:, None] * qd).sum(axis=0), (wJ[:, None] * np.abs(qd)).sum(axis=0)
-    assert np.all(np.abs(tot) <= 1e-12 * scale), tot / scale
-    eng.set_state(Q)
-    ke0 = eng.kinetic_energy()
-    eng.advance(2, 0.4, 0.4, t=0.0)
-    assert not eng.has_nan()
-    ke1 = eng.kinetic_energy()
-    print("48^3 x N=5 kinetic energy", ke0, ke1, flush=True)
-    assert 0.0 < ke1 < ke0
+    eng.close()
+
+
+def test_box4_n5_curvilinear_kg_roe_wale_matches_oracle():
+    """The largest case's discretisation on a small box the oracle runs in seconds: 4^3
+    curvilinear (amplitude 0.15, seed 5) of order 5, split Kennedy-Gruber, Roe, BR1, WALE.  QDot
+    per stage and three device-dt RK3 steps against the oracle, stage by stage."""
+    case = mesh.periodic_box(4, 5, amplitude=0.15, phase_seed=5)
+    Q0 = mesh.taylor_green(case, 0.1, noise=1e-3, seed=45)
+    kw = dict(mach=0.1, reynolds=1600.0, riemann="roe", inviscid="split-form", averaging="kennedy-gruber",
+              les="wale", les_c=None)
+    eng, ora = engine_for(case, kw), oracle_for(case, kw)
+    eng.set_state(Q0)
+    Q, G = Q0.copy(), np.zeros_like(Q0)
+    for s in range(3):
+        dt = eng.max_dt(0.4, 0.4)
+        assert dt == pytest.approx(ora.max_dt(Q, 0.4, 0.4), rel=1e-13)
+        Qs = Q.copy()
+        assert po.normwise_error(eng.qdot(), ora.qdot(Q)).max() <= TOL
+        eng.rk3_step(0.0, dt)
+        qd = rk3_by_stages(ora, Q, G, dt)
+        assert po.normwise_error(eng.get_qdot(), qd).max() <= TOL, s
+        err = po.normwise_error(eng.get_state() - Qs, Q - Qs)
+        print("4^3 x N=5 step", s, err, flush=True)
+        assert err.max() <= 1e-11, err
+        eng.set_state(Q)   # continue from the oracle's state (the engine restarts its register: a = 0 first)
+        G[:] = 0.0
+    eng.close()
+
+
+def test_benchmark_case_100_steps_matches_reference():
+    """The benchmark case itself against the reference solver's own 100-step run
+    (tests/golden/bench, written by make_bench_trajectory.py): 16^3 curvilinear box from the
+    same SpecMesh file (the front end reproduces the reference's geometry bit for bit), N = 4,
+    split Pirozzoli, LxF, BR1, dt from CFL 0.4 on the device every step.
+      * per stage: QDot of the engine's state against the oracle's, <= 1e-12, at steps 0, 50, 100;
+      * every step's dt against the reference's (the initial state is the reference's own: the
+        Taylor-Green formula plus the fixture's table of last-bit corrections);
+      * the kinetic energy after every step within 1e-10 of the reference's monitor;
+      * the state at 4,345 sampled nodes and per-variable sums of squares after 50 and 100 steps,
+        and the last stage's QDot the reference held then."""
+    import os
+    from horses3d_amd import specmesh
+    from horses3d_amd.engine import read_specmesh
+    z = np.load(os.path.join(os.path.dirname(GOLDEN[0]), "bench", "tgv16_n4_curved_split_pirozzoli_lxf_100steps.npz"),
+                allow_pickle=False)
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "box.mesh")
+        specmesh.write_specmesh(path, 16, 0.1, mesh.box_phases(2), bface_order=4)
+        case = read_specmesh(path, N=4, periodic=dict(specmesh.PERIODIC_PAIRS))
+    case["fNf"] = np.full((case["F"], 2), 4, dtype=np.int32)   # one order: every face of order 4
+    case["foff"] = np.arange(case["F"] + 1, dtype=np.int64) * 25
+    print("16^3 x N=4 case read", flush=True)
+    sel = z["sample"]
+    kw = dict(mach=0.1, reynolds=1600.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli",
+              les=None, les_c=None)
+    eng, ora = engine_for(case, kw), oracle_for(case, kw)
+    Q0 = reference_initial_state(case, z)
+    assert np.array_equal(Q0[sel], z["Q_step0"])
+    eng.set_state(Q0)
+    assert po.normwise_error(eng.qdot()[sel], z["QDot_step0"]).max() <= TOL
+    t, ke_err, dt_err = 0.0, 0.0, 0.0
+    for s in range(100):
+        if s in (0, 50):
+            Q = eng.get_state()
+            assert po.normwise_error(eng.qdot(), ora.qdot(Q)).max() <= TOL, s
+        t_prev = t
+        t = eng.advance(1, 0.4, 0.4, t=t)
+        dt_err = max(dt_err, abs((t - t_prev) / z["dt_log"][s, 2] - 1.0))
+        ke_err = max(ke_err, abs(eng.kinetic_energy() / z["ke_monitor"][s + 1, 2] - 1.0))
+        if s + 1 in (50, 100):
+            Q = eng.get_state()
+            e_q = po.normwise_error(Q[sel], z["Q_step%d" % (s + 1)])
+            e_sq = np.abs((Q * Q).sum(axis=0) / z["Qsq_step%d" % (s + 1)] - 1.0)
+            e_qd = po.normwise_error(eng.get_qdot()[sel], z["QDot_step%d" % (s + 1)])
+            print("step", s + 1, "state", e_q, "sum of squares", e_sq, "last-stage QDot", e_qd, flush=True)
+            assert e_q.max() < 1e-11 and e_sq.max() < 1e-12 and e_qd.max() < 1e-10
+    print("dt", dt_err, "kinetic energy", ke_err, flush=True)
+    assert dt_err < 1e-12 and ke_err < 1e-10
+    assert t == pytest.approx(float(z["t_step100"]), rel=1e-12)
+    Q = eng.get_state()
+    assert po.normwise_error(eng.qdot(), ora.qdot(Q)).max() <= TOL
     eng.close()
 
 
@@ -593,4 +703,80 @@ def test_serving_retires_a_diverging_case_alone():
     assert alone[2].error and "divergence" in alone[2].error
     for k, (r, a) in enumerate(zip(res, alone)):
         if k != 2:
+            assert r.error is None and r.t == a.t and np.array_equal(r.Q, a.Q), k
+
+
+@pytest.mark.parametrize("N,form", [(3, ("standard", "standard", "roe", "smagorinsky")),
+                                    (4, ("split-form", "pirozzoli", "lxf", None))])
+def test_multi_case_engine_matches_cases_alone(N, form):
+    """Heterogeneous batching: cases of different sizes, geometry, Mach and Reynolds numbers in
+    one engine, each with its own CFL dt every step and its own step count (a case switched off
+    keeps its state and time).  Every case ends bit for bit where it ends alone."""
+    from horses3d_amd.engine import Engine, MultiEngine
+    inviscid, averaging, riemann, les = form
+    rng = np.random.default_rng(40 + N)
+    cases, phys, Q0 = [], [], []
+    for k, n in enumerate((3, 2, 4, 1)):
+        c = mesh.periodic_box(n, N, amplitude=0.1 if k % 2 else 0.0, phase_seed=k + 1)
+        mach = float(rng.uniform(0.1, 0.8))
+        re = float(np.exp(rng.uniform(np.log(400.0), np.log(5000.0))))
+        cases.append(c)
+        phys.append(dict(mach=mach, reynolds=re, riemann=riemann, inviscid=inviscid, averaging=averaging, les=les))
+        Q0.append(mesh.taylor_green(c, mach, phase=tuple(rng.uniform(0.0, 2.0 * np.pi, 3))))
+    steps = [6, 3, 6, 4]
+    m = MultiEngine(cases, phys)
+    m.set_states(Q0)
+    m.advance(3, 0.4, 0.4)
+    m.set_active(1, False)
+    m.advance(1, 0.4, 0.4)
+    m.set_active(3, False)
+    m.advance(2, 0.4, 0.4)
+    times, bad = m.status()
+    got = m.get_states()
+    assert not bad.any()
+    for k, (c, ph, q, n) in enumerate(zip(cases, phys, Q0, steps)):
+        e = Engine(c, **ph)
+        e.set_state(q)
+        t = e.advance(n, 0.4, 0.4, t=0.0)
+        alone = e.get_state()
+        e.close()
+        print("case", k, "steps", n, "time", times[k], t, flush=True)
+        assert times[k] == t, k
+        assert np.array_equal(got[k], alone), (k, np.abs(got[k] - alone).max())
+    # a NaN in one case flags that case only
+    bad_q = [q.copy() for q in Q0]
+    bad_q[2][5, 4] = np.nan
+    m.set_states(bad_q)
+    for k in range(4):
+        m.set_active(k, True)
+    m.advance(2, 0.4, 0.4)
+    _, bad = m.status()
+    assert list(bad) == [False, False, True, False]
+    m.close()
+
+
+def test_batched_serving_matches_cases_alone():
+    """The serving loop with shared launches (serve_batched): a stream of cases of two orders,
+    different sizes, Mach and Reynolds numbers and step counts, grouped into several-case
+    engines.  Every case ends bit for bit where it ends alone on the plain engine, at the same
+    time; a case with a NaN in its initial state is retired alone with the divergence error."""
+    from horses3d_amd import serve
+    rng = np.random.default_rng(12)
+    reqs = []
+    for k in range(10):
+        n, N = int(rng.integers(1, 4)), 3 + k % 2
+        case = mesh.periodic_box(n, N, amplitude=0.1 * (k % 3 == 0), phase_seed=k + 1)
+        mach = float(rng.uniform(0.1, 0.8))
+        re = float(np.exp(rng.uniform(np.log(400.0), np.log(5000.0))))
+        Q0 = mesh.taylor_green(case, mach, phase=tuple(rng.uniform(0.0, 2.0 * np.pi, 3)))
+        if k == 6:
+            Q0[2, 3] = np.nan
+        reqs.append(serve.Request(case, dict(mach=mach, reynolds=re, riemann="roe", les="smagorinsky"), Q0,
+                                  steps=int(rng.integers(5, 40)), arrival=0.003 * k))
+    res, st = serve.serve_batched(reqs, chunk=8, min_batch=3, max_wait_s=0.01)
+    alone, _ = serve.one_at_a_time(reqs)
+    print(st, flush=True)
+    assert st["failed"] == 1 and res[6].error and alone[6].error
+    for k, (r, a) in enumerate(zip(res, alone)):
+        if k != 6:
             assert r.error is None and r.t == a.t and np.array_equal(r.Q, a.Q), k

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import load_golden, physics_kwargs, rk3_by_stages, GOLDEN, STAGE3
+from conftest import load_golden, physics_kwargs, reference_initial_state, rk3_by_stages, GOLDEN, STAGE3
 
 
 def make_oracle(case, meta):
@@ -86,3 +86,29 @@ def test_last_stage_qdot_is_the_reference_residual_source(path):
     assert po.normwise_error(qd, case["QDot_final"]).max() < 1e-11
     # a plausible mistake: the derivative of the final state instead of the last stage's
     assert po.normwise_error(o.qdot(Q), case["QDot_final"]).max() > 1e-6
+
+
+def test_benchmark_case_start_matches_reference():
+    """The benchmark case at its full size (16^3 curvilinear, N = 4, 512,000 nodes), read from the
+    same SpecMesh file as the reference's 100-step run: the initial state at the sampled nodes,
+    the oracle's QDot there and its first dt against the reference's."""
+    import tempfile
+    from horses3d_amd import mesh, specmesh
+    from horses3d_amd.engine import read_specmesh
+    z = np.load(os.path.join(os.path.dirname(GOLDEN[0]), "bench", "tgv16_n4_curved_split_pirozzoli_lxf_100steps.npz"),
+                allow_pickle=False)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "box.mesh")
+        specmesh.write_specmesh(path, 16, 0.1, mesh.box_phases(2), bface_order=4)
+        case = read_specmesh(path, N=4, periodic=dict(specmesh.PERIODIC_PAIRS))
+    case["fNf"] = np.full((case["F"], 2), 4, dtype=np.int32)   # one order: every face of order 4
+    case["foff"] = np.arange(case["F"] + 1, dtype=np.int64) * 25
+    assert mesh.ndof(case) == int(z["nodes"])
+    o = po.Oracle(case, po.make_params(0.1, 1600.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli"))
+    sel = z["sample"]
+    assert np.array_equal(case["x"][sel], z["x_sample"])
+    Q0 = reference_initial_state(case, z)
+    assert len(z["ic_fix_index"]) < 2000 and np.array_equal(Q0[sel], z["Q_step0"])
+    assert po.normwise_error(o.qdot(Q0)[sel], z["QDot_step0"]).max() < 1e-13
+    assert o.max_dt(Q0, 0.4, 0.4) == pytest.approx(z["dt_log"][0, 2], rel=1e-13)
+    assert abs(o.kinetic_energy(Q0) / z["ke_monitor"][0, 2] - 1.0) < 1e-13

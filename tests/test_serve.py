@@ -5,6 +5,7 @@ figures are those of the recorded times.  Host only; the GPU form is in test_eng
 import time
 
 import numpy as np
+import pytest
 
 from horses3d_amd import serve
 
@@ -200,3 +201,75 @@ def test_diverging_case_retired_alone():
     for k, (r, q) in enumerate(zip(res, reqs)):
         if k != 2:
             assert r.error is None and np.array_equal(r.Q, q.Q0 + q.steps * 0.5)
+
+
+class FakeMulti:
+    """A stand-in several-case engine: n steps take n * 1 ms of the clock for the whole group;
+    each active case's state counts its own steps."""
+    live = 0
+
+    def __init__(self, rs, clock, diverge=None):
+        self.rs, self.clock, self.busy_until = rs, clock, 0.0
+        self.active = [True] * len(rs)
+        self.steps = [0] * len(rs)
+        self.diverge = diverge or {}
+        FakeMulti.live += 1
+
+    def set_states(self, Q):
+        self.Q = [np.array(q, dtype=float) for q in Q]
+
+    def device_bytes(self):
+        return 1000 * sum(r.nodes for r in self.rs)
+
+    def done(self):
+        return self.clock.t >= self.busy_until
+
+    def advance(self, n, cfl, dcfl):
+        assert self.done()
+        self.busy_until = self.clock.t + 1e-3 * n
+        for k in range(len(self.rs)):
+            if self.active[k]:
+                self.steps[k] += n
+                self.Q[k] = self.Q[k] + n * cfl
+
+    def set_active(self, k, on):
+        self.active[k] = on
+
+    def status(self):
+        bad = np.array([k in self.diverge and self.steps[k] >= self.diverge[k] for k in range(len(self.rs))])
+        return 0.1 * np.array(self.steps, dtype=float), bad
+
+    def get_case_state(self, k):
+        return self.Q[k].copy()
+
+    def close(self):
+        FakeMulti.live -= 1
+
+
+def test_batched_serving_completes_every_case_as_alone():
+    """Several-case engines: every request ends with its own step count and the state it has
+    alone, cases are grouped only by their batch key, and a case that diverges is retired alone."""
+    reqs = requests(20, seed=13)
+    for k, q in enumerate(reqs):
+        q.case = {"off": q.case["off"], "N": np.full((2, 3), 3 + k % 2)}
+    clock = Clock()
+    FakeMulti.live = 0
+    made = []
+
+    def factory(rs):
+        keys = {serve.batch_key(r) for r in rs}
+        assert len(keys) == 1
+        m = FakeMulti(rs, clock, diverge={k: 3 for k, r in enumerate(rs) if r is reqs[7]})
+        made.append(m)
+        return m
+
+    res, st = serve.serve_batched(reqs, batch_factory=factory, clock=clock, sleep=clock.sleep, poll_s=0.0, chunk=7,
+                                  min_batch=3, max_wait_s=0.01, device_memory=lambda: 1 << 40, workers=0)
+    assert FakeMulti.live == 0 and len(made) >= 2
+    assert res[7].error and "divergence" in res[7].error
+    assert st["failed"] == 1 and st["cases"] == len(reqs) - 1
+    for k, (r, q) in enumerate(zip(res, reqs)):
+        assert r.admitted >= q.arrival
+        if k != 7:
+            assert r.error is None and np.array_equal(r.Q, q.Q0 + q.steps * 0.5), k
+            assert r.t == pytest.approx(0.1 * q.steps)
