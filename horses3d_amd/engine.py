@@ -339,67 +339,6 @@ class Engine:
         self.t = 0.0
 
 
-class MultiEngine(Engine):
-    """Several independent cases of one order in one engine (hx3d_engine_create_multi): each
-    case keeps its own physics constants (Mach, Reynolds and Prandtl numbers, LES constant), its
-    own dt (MaxTimeStep of its own state, every step), time and divergence flag, and can be
-    switched off when it has taken its steps; the discretisation is shared.  Every kernel launch
-    carries all the active cases.  The state arrays are the cases' states one after the other
-    (`slices`)."""
-
-    def __init__(self, cases, physics):
-        L = load_library()
-        if isinstance(physics, dict):
-            physics = [physics] * len(cases)
-        self._keeps = [{} for _ in cases]
-        descs = (CaseDesc * len(cases))(*[_desc(c, k, **ph) for c, k, ph in zip(cases, self._keeps, physics)])
-        self._descs = descs
-        h = C.c_void_p()
-        rc = L.hx3d_engine_create_multi(descs, len(cases), C.byref(h))
-        self._created(L, h, rc, "hx3d_engine_create_multi")
-        self.ncases = len(cases)
-        n = [int(c["off"][-1]) if "off" in c else len(c["J"]) for c in cases]
-        o = np.concatenate([[0], np.cumsum(n)])
-        self.slices = [slice(int(a), int(b)) for a, b in zip(o[:-1], o[1:])]
-
-    def set_states(self, states):
-        Q = np.empty((self.nodes, 5))
-        for s, q in zip(self.slices, states):
-            Q[s] = q
-        self.set_state(Q)
-
-    def get_states(self):
-        Q = self.get_state()
-        return [Q[s].copy() for s in self.slices]
-
-    def get_case_state(self, c):
-        """Case c's state alone (waits for the queued work)."""
-        s = self.slices[c]
-        out = np.empty((s.stop - s.start, 5))
-        self._chk(load_library().hx3d_get_state_nodes(self.h, s.start, s.stop - s.start, _p(out)), "hx3d_get_state_nodes")
-        return out
-
-    def set_active(self, c, on):
-        """Case c takes part in the following steps (on) or is left as it is (off)."""
-        self._chk(load_library().hx3d_multi_set_active(self.h, int(c), 1 if on else 0), "hx3d_multi_set_active")
-
-    def status(self):
-        """(times, diverged flags) of the cases once the queued work is done."""
-        t = np.zeros(self.ncases)
-        f = np.zeros(self.ncases, dtype=np.int32)
-        self._chk(load_library().hx3d_multi_status(self.h, _p(t), _p(f)), "hx3d_multi_status")
-        return t, f.astype(bool)
-
-    def advance(self, nsteps, cfl, dcfl, t=None):
-        """Queues nsteps steps of every active case, each with its own CFL dt; returns at once."""
-        assert t is None, "several cases: read their times with status()"
-        self._chk(load_library().hx3d_advance(self.h, nsteps, cfl, dcfl, None), "hx3d_advance")
-
-    def done(self):
-        d = C.c_int()
-        self._chk(load_library().hx3d_query(self.h, C.byref(d)), "hx3d_query")
-        return bool(d.value)
-
     def _chk(self, rc, what):
         if rc != 0:
             raise _error("%s: %s (code %d)" % (what, load_library().hx3d_last_error(self.h).decode(), rc), rc)
@@ -529,3 +468,60 @@ class MultiEngine(Engine):
         out = np.zeros(4)
         self._chk(load_library().hx3d_kernel_times(self.h, _p(out)), "hx3d_kernel_times")
         return out
+
+
+class MultiEngine(Engine):
+    """Several independent cases of one order in one engine (hx3d_engine_create_multi): each
+    case keeps its own physics constants (Mach, Reynolds and Prandtl numbers, LES constant), its
+    own dt (MaxTimeStep of its own state, every step), time and divergence flag, and can be
+    switched off when it has taken its steps; the discretisation is shared.  Every kernel launch
+    carries all the active cases.  The state arrays are the cases' states one after the other
+    (`slices`)."""
+
+    def __init__(self, cases, physics):
+        L = load_library()
+        if isinstance(physics, dict):
+            physics = [physics] * len(cases)
+        self._keeps = [{} for _ in cases]
+        descs = (CaseDesc * len(cases))(*[_desc(c, k, **ph) for c, k, ph in zip(cases, self._keeps, physics)])
+        self._descs = descs
+        h = C.c_void_p()
+        rc = L.hx3d_engine_create_multi(descs, len(cases), C.byref(h))
+        self._created(L, h, rc, "hx3d_engine_create_multi")
+        self.ncases = len(cases)
+        n = [int(c["off"][-1]) if "off" in c else len(c["J"]) for c in cases]
+        o = np.concatenate([[0], np.cumsum(n)])
+        self.slices = [slice(int(a), int(b)) for a, b in zip(o[:-1], o[1:])]
+
+    def set_states(self, states):
+        Q = np.empty((self.nodes, 5))
+        for s, q in zip(self.slices, states):
+            Q[s] = q
+        self.set_state(Q)
+
+    def get_states(self):
+        Q = self.get_state()
+        return [Q[s].copy() for s in self.slices]
+
+    def get_case_state(self, c):
+        """Case c's state alone (waits for the queued work)."""
+        s = self.slices[c]
+        out = np.empty((s.stop - s.start, 5))
+        self._chk(load_library().hx3d_get_state_nodes(self.h, s.start, s.stop - s.start, _p(out)), "hx3d_get_state_nodes")
+        return out
+
+    def set_active(self, c, on):
+        """Case c takes part in the following steps (on) or is left as it is (off)."""
+        self._chk(load_library().hx3d_multi_set_active(self.h, int(c), 1 if on else 0), "hx3d_multi_set_active")
+
+    def status(self):
+        """(times, diverged flags) of the cases once the queued work is done."""
+        t = np.zeros(self.ncases)
+        f = np.zeros(self.ncases, dtype=np.int32)
+        self._chk(load_library().hx3d_multi_status(self.h, _p(t), _p(f)), "hx3d_multi_status")
+        return t, f.astype(bool)
+
+    def advance(self, nsteps, cfl, dcfl, t=None):
+        """Queues nsteps steps of every active case, each with its own CFL dt; returns at once."""
+        assert t is None, "several cases: read their times with status()"
+        self._chk(load_library().hx3d_advance(self.h, nsteps, cfl, dcfl, None), "hx3d_advance")

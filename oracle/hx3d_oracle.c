@@ -600,51 +600,51 @@ static void roe_flux(const ox_case *c, const double *QL, const double *QR, const
                      double *flux)
 {   /* RiemannSolvers_NS.f90:1541-1656 RoeRiemannSolver */
     double g = c->gamma;
-    double rho = QL[0], rhou = QL[1], rhov = QL[2], rhow = QL[3], rhoe = QL[4];
-    double rhon = QR[0], rhoun = QR[1], rhovn = QR[2], rhown = QR[3], rhoen = QR[4];
-    double ul = rhou / rho, vl = rhov / rho, wl = rhow / rho;
-    double pleft = (g - 1.0) * (rhoe - 0.5 / rho * (POW2(rhou) + POW2(rhov) + POW2(rhow)));
-    double ur = rhoun / rhon, vr = rhovn / rhon, wr = rhown / rhon;
-    double pright = (g - 1.0) * (rhoen - 0.5 / rhon * (POW2(rhoun) + POW2(rhovn) + POW2(rhown)));
-    double ql = n[0] * ul + n[1] * vl + n[2] * wl;
-    double qr = n[0] * ur + n[1] * vr + n[2] * wr;
-    double hl = 0.5 * (ul * ul + vl * vl + wl * wl) + g / (g - 1.0) * pleft / rho;
-    double hr = 0.5 * (ur * ur + vr * vr + wr * wr) + g / (g - 1.0) * pright / rhon;
-    double rtd = sqrt(rho * rhon);
-    double betal = rho / (rho + rtd), betar = 1.0 - betal;
-    double utd = betal * ul + betar * ur, vtd = betal * vl + betar * vr, wtd = betal * wl + betar * wr;
-    double htd = betal * hl + betar * hr;
-    double atd2 = (g - 1.0) * (htd - 0.5 * (utd * utd + vtd * vtd + wtd * wtd));
-    double atd = sqrt(atd2);
-    double qtd = utd * n[0] + vtd * n[1] + wtd * n[2];
-    if (qtd >= 0.0) {
-        double dw1 = 0.5 * ((pright - pleft) / atd2 - (qr - ql) * rtd / atd);
-        double sp1 = qtd - atd;
-        double sp1m = sp1 < 0.0 ? sp1 : 0.0;
-        double hd1m = ((g + 1.0) / 4.0 * atd / rtd) * dw1;
-        double e1 = -fabs(sp1) - hd1m;
-        double eta1 = e1 > 0.0 ? e1 : 0.0;
-        double udw1 = dw1 * (sp1m - 0.5 * eta1);
-        double rql = rho * ql;
-        flux[0] = (rql + udw1);
-        flux[1] = (rql * ul + pleft * n[0] + udw1 * (utd - atd * n[0]));
-        flux[2] = (rql * vl + pleft * n[1] + udw1 * (vtd - atd * n[1]));
-        flux[3] = (rql * wl + pleft * n[2] + udw1 * (wtd - atd * n[2]));
-        flux[4] = (rql * hl + udw1 * (htd - qtd * atd));
+    double rL = QL[0], mxL = QL[1], myL = QL[2], mzL = QL[3], EL = QL[4];
+    double rR = QR[0], mxR = QR[1], myR = QR[2], mzR = QR[3], ER = QR[4];
+    double uL = mxL / rL, vL = myL / rL, wL = mzL / rL;
+    double pL = (g - 1.0) * (EL - 0.5 / rL * (POW2(mxL) + POW2(myL) + POW2(mzL)));
+    double uR = mxR / rR, vR = myR / rR, wR = mzR / rR;
+    double pR = (g - 1.0) * (ER - 0.5 / rR * (POW2(mxR) + POW2(myR) + POW2(mzR)));
+    double unL = n[0] * uL + n[1] * vL + n[2] * wL;
+    double unR = n[0] * uR + n[1] * vR + n[2] * wR;
+    double HL = 0.5 * (uL * uL + vL * vL + wL * wL) + g / (g - 1.0) * pL / rL;
+    double HR = 0.5 * (uR * uR + vR * vR + wR * wR) + g / (g - 1.0) * pR / rR;
+    double srr = sqrt(rL * rR);
+    double wtL = rL / (rL + srr), wtR = 1.0 - wtL;
+    double ua = wtL * uL + wtR * uR, va = wtL * vL + wtR * vR, wa = wtL * wL + wtR * wR;
+    double Ha = wtL * HL + wtR * HR;
+    double c2 = (g - 1.0) * (Ha - 0.5 * (ua * ua + va * va + wa * wa));
+    double ca = sqrt(c2);
+    double una = ua * n[0] + va * n[1] + wa * n[2];
+    if (una >= 0.0) {
+        double a1 = 0.5 * ((pR - pL) / c2 - (unR - unL) * srr / ca);
+        double lam1 = una - ca;
+        double lam1n = lam1 < 0.0 ? lam1 : 0.0;
+        double fix1 = ((g + 1.0) / 4.0 * ca / srr) * a1;
+        double x1 = -fabs(lam1) - fix1;
+        double eps1 = x1 > 0.0 ? x1 : 0.0;
+        double s1 = a1 * (lam1n - 0.5 * eps1);
+        double mnL = rL * unL;
+        flux[0] = (mnL + s1);
+        flux[1] = (mnL * uL + pL * n[0] + s1 * (ua - ca * n[0]));
+        flux[2] = (mnL * vL + pL * n[1] + s1 * (va - ca * n[1]));
+        flux[3] = (mnL * wL + pL * n[2] + s1 * (wa - ca * n[2]));
+        flux[4] = (mnL * HL + s1 * (Ha - una * ca));
     } else {
-        double dw4 = 0.5 * ((pright - pleft) / atd2 + (qr - ql) * rtd / atd);
-        double sp4 = qtd + atd;
-        double sp4p = sp4 > 0.0 ? sp4 : 0.0;
-        double hd4 = ((g + 1.0) / 4.0 * atd / rtd) * dw4;
-        double e4 = -fabs(sp4) + hd4;
-        double eta4 = e4 > 0.0 ? e4 : 0.0;
-        double udw4 = dw4 * (sp4p + 0.5 * eta4);
-        double rqr = rhon * qr;
-        flux[0] = (rqr - udw4);
-        flux[1] = (rqr * ur + pright * n[0] - udw4 * (utd + atd * n[0]));
-        flux[2] = (rqr * vr + pright * n[1] - udw4 * (vtd + atd * n[1]));
-        flux[3] = (rqr * wr + pright * n[2] - udw4 * (wtd + atd * n[2]));
-        flux[4] = (rqr * hr - udw4 * (htd + qtd * atd));
+        double a4 = 0.5 * ((pR - pL) / c2 + (unR - unL) * srr / ca);
+        double lam4 = una + ca;
+        double lam4p = lam4 > 0.0 ? lam4 : 0.0;
+        double fix4 = ((g + 1.0) / 4.0 * ca / srr) * a4;
+        double x4 = -fabs(lam4) + fix4;
+        double eps4 = x4 > 0.0 ? x4 : 0.0;
+        double s4 = a4 * (lam4p + 0.5 * eps4);
+        double mnR = rR * unR;
+        flux[0] = (mnR - s4);
+        flux[1] = (mnR * uR + pR * n[0] - s4 * (ua + ca * n[0]));
+        flux[2] = (mnR * vR + pR * n[1] - s4 * (va + ca * n[1]));
+        flux[3] = (mnR * wR + pR * n[2] - s4 * (wa + ca * n[2]));
+        flux[4] = (mnR * HR - s4 * (Ha + una * ca));
     }
 }
 
