@@ -1447,9 +1447,11 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
     if (eng->mc.blocks && t_inout) return eng->fail(HX3D_EINVAL, "several cases: pass no time, read them with hx3d_multi_status");
     if (t_inout) HIPCHK(hipMemcpyAsync(eng->dScal + 1, t_inout, sizeof(double), hipMemcpyHostToDevice, eng->stream));
     const size_t nev = eng->timing ? (size_t)12 * nsteps : 0;
+    // timing events need no system-scope fence: they are read only through
+    // hipEventElapsedTime, and the fence adds its own time to every measured launch
     while (eng->evpool.size() < nev) {
         hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         eng->evpool.push_back(e);
     }
     // dt of step s (slot 0): MaxTimeStep of the state the step starts from.  The first stage's
