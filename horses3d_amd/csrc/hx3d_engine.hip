@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -171,6 +172,54 @@ __global__ void k_finish_max(const double *part, int nparts, double *out)
     if (threadIdx.x < NV) {
         double r = 0.0;
         for (int b = 0; b < nparts; ++b) r = fmax(r, part[(long)b * NV + threadIdx.x]);
+        out[threadIdx.x] = r;
+    }
+}
+
+// The allocation-free residuals (hx3d_residuals_noalloc): max |X| per variable over all nodes
+// by at most NOALLOC_BLOCKS blocks, each striding over the nodes, partial maxima to part[b*NV + v];
+// then one block reduces the nparts partials into out[0..NV-1].  The maximum of doubles is exact
+// and independent of order, so the result has the bits of k_absmax + k_finish_max.
+constexpr int NOALLOC_BLOCKS = 2048;
+
+__global__ void __launch_bounds__(256)
+k_absmax_strided(const double *__restrict__ X, long nodes, long S, double *__restrict__ part)
+{
+    __shared__ double sm[4][NV];
+    double m[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < nodes; g += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) m[v] = fmax(m[v], fabs(X[v * S + g]));
+    }
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        m[v] = wave_max(m[v]);
+        if (ln == 0) sm[w][v] = m[v];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double r = sm[0][threadIdx.x];
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) r = fmax(r, sm[x][threadIdx.x]);
+        part[(long)blockIdx.x * NV + threadIdx.x] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_finish_max_block(const double *__restrict__ part, int nparts, double *out)
+{
+    __shared__ double sm[4][NV];
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double r = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) r = fmax(r, part[(long)b * NV + v]);
+        r = wave_max(r);
+        if (ln == 0) sm[w][v] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double r = sm[0][threadIdx.x];
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) r = fmax(r, sm[x][threadIdx.x]);
         out[threadIdx.x] = r;
     }
 }
@@ -364,6 +413,35 @@ Ops make_ops(int N)
         if (e_ != hipSuccess) return eng->fail(HX3D_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
     } while (0)
 
+}  // namespace
+
+// Scratch of the allocation-free entry point (hx3d_residuals_noalloc): NOALLOC_BLOCKS x NV
+// partial maxima, 80 KB, one copy per GPU the library runs on, held until the process ends.
+// On the host the name `scratch` is only a stand-in, not a device address: host code takes the
+// address from hipGetSymbolAddress, once per device with that device current (noalloc_scratch),
+// and uses that pointer in kernel arguments.  Every call on a GPU shares that GPU's copy, so
+// calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[NOALLOC_BLOCKS * NV];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+double *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// scratch's device address on device dev (the current device), looked up on the device's first
+// call; nullptr with err set when the lookup fails
+double *noalloc_scratch(int dev, hipError_t *err)
+{
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess || !p) return nullptr;
+        noalloc_ptr[dev] = static_cast<double *>(p);
+    }
+    *err = hipSuccess;
+    return noalloc_ptr[dev];
+}
 }  // namespace
 
 struct hx3d_engine {
@@ -1547,6 +1625,32 @@ int hx3d_residuals(hx3d_engine *eng, double res[5])
     hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(64), 0, eng->stream, eng->dPart, nb, eng->dStage);
     HIPCHK(hipMemcpyAsync(res, eng->dStage, sizeof(double) * NV, hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
+    return HX3D_OK;
+}
+
+int hx3d_residuals_noalloc(hx3d_engine *eng, double *res_dev)
+{
+    if (!eng || !res_dev) return HX3D_EINVAL;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != eng->device)
+        return eng->fail(HX3D_EINVAL, "hx3d_residuals_noalloc: make the engine's device current first");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return eng->fail(HX3D_EUNSUPPORTED, "hx3d_residuals_noalloc: device index beyond the scratch table");
+    hipError_t e = hipSuccess;
+    double *part = noalloc_scratch(dev, &e);
+    if (!part) return eng->fail(HX3D_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    if (!eng->qdot_valid) {   // the stage kernels run on the engine's own buffers (no allocation)
+        int rc = hx3d_compute_qdot(eng, 0.0, nullptr);
+        if (rc) return rc;
+    }
+    // at most NOALLOC_BLOCKS blocks stride over the nodes, so the scratch fits every node count
+    const int nb = (int)std::min<long>((eng->nodes + 255) / 256, NOALLOC_BLOCKS);
+    if (nb < 1 || nb > NOALLOC_BLOCKS)
+        return eng->fail(HX3D_EINVAL, "hx3d_residuals_noalloc: partials beyond the scratch");
+    hipLaunchKernelGGL(k_absmax_strided, dim3(nb), dim3(256), 0, eng->stream, eng->dQDot, eng->nodes, eng->S, part);
+    hipLaunchKernelGGL(k_finish_max_block, dim3(1), dim3(256), 0, eng->stream, part, nb, res_dev);
+    HIPCHK(hipGetLastError());
     return HX3D_OK;
 }
 

@@ -66,6 +66,7 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_create_multi", "hx3d_multi_set_act
            "hx3d_engine_destroy", "hx3d_last_error", "hx3d_num_nodes",
            "hx3d_set_state", "hx3d_get_state", "hx3d_get_state_nodes", "hx3d_set_state_device", "hx3d_get_state_device",
            "hx3d_compute_qdot", "hx3d_rk3_step", "hx3d_set_ctd_after_steps", "hx3d_get_qdot", "hx3d_max_dt", "hx3d_advance", "hx3d_advance_dt", "hx3d_residuals",
+           "hx3d_residuals_noalloc",
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_query", "hx3d_get_time", "hx3d_engine_device_bytes",
            "hx3d_device_memory", "hx3d_set_timing", "hx3d_set_step_graph",
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
@@ -105,6 +106,7 @@ def load_library(path=LIB_PATH):
     L.hx3d_advance.argtypes = [P, I, D, D, C.POINTER(D)]
     L.hx3d_advance_dt.argtypes = [P, I, D, C.POINTER(D)]
     L.hx3d_residuals.argtypes = [P, P]
+    L.hx3d_residuals_noalloc.argtypes = [P, P]
     L.hx3d_kinetic_energy.argtypes = [P, C.POINTER(D)]
     L.hx3d_check_nan.argtypes = [P, C.POINTER(I)]
     L.hx3d_synchronize.argtypes = [P]

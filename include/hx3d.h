@@ -161,6 +161,16 @@ int hx3d_set_step_graph(hx3d_engine *eng, int on);
  * TimeIntegrator.f90:852): max |QDot| per variable of the current time derivative (after a step,
  * the last stage's, or the new state's with CTD_AFTER_STEPS). */
 int hx3d_residuals(hx3d_engine *eng, double res[5]);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): hx3d_residuals with
+ * the result left in device memory.  res_dev[0..4] (caller-owned device memory) receives max |QDot|
+ * per variable of the current time derivative, computed first from the resident state if none is
+ * current; queued on the engine stream, no wait, no allocation.  The engine's device must be
+ * current.  Partial maxima go to the library's one static device array per GPU (2048 blocks x 5
+ * doubles, 80 KB, held until the process ends); the kernels take at most 2048 blocks that stride
+ * over the nodes, so every node count an engine holds fits it.  Calls that overlap on one GPU
+ * (any engines, any streams) race for that array. */
+int hx3d_residuals_noalloc(hx3d_engine *eng, double *res_dev);
 /* Kinetic-energy monitor of the resident state. */
 int hx3d_kinetic_energy(hx3d_engine *eng, double *ke);
 /* Volume monitors (VolumeMonitor_Update, VolumeMonitor.f90:278-330, with the integrals of
