@@ -201,35 +201,46 @@ def batch_bench(args):
 
 
 def serve_bench(args):
-    """The serving stream, scaled to a run of seconds: cases of n^3 straight periodic hexes, n
-    uniform on {8..16} (the full stream goes to 24), order N uniform on {3..6}, Mach uniform on
-    [0.1, 0.8], Re log-uniform on [400, 5000], Taylor-Green state with seeded phase offsets,
-    Roe + BR1 + Smagorinsky (Cs 0.2), step counts uniform on [100, 1000], Poisson arrivals."""
+    """The serving stream: a Poisson stream of --serve-cases periodic-box cases of n^3 straight
+    hexes, n uniform on {8..--serve-nmax}, order N uniform on {3..6}, Mach uniform on [0.1, 0.8],
+    Re log-uniform on [400, 5000], Taylor-Green state with seeded phase offsets, Roe + BR1 +
+    Smagorinsky (Cs 0.2), step counts uniform on [100, 1000], dt from CFL 0.4 per case.  Each case
+    is built when it is set up and dropped once on the device.  Served with shared launches
+    (serve_batched: several-case engines grouped by order), then the same stream one case at a
+    time on the plain engine; final states compared by digest."""
     import numpy as np
     from horses3d_amd import mesh, serve
     rng = np.random.default_rng(args.serve_seed)
+
+    def req(n, N, mach, re, phase, steps, arrival):
+        def build():
+            case = mesh.straight_box(n, N)
+            return case, mesh.taylor_green(case, mach, phase=phase)
+        return serve.Request(None, dict(mach=mach, reynolds=re, riemann="roe", les="smagorinsky", les_c=0.2), None,
+                             steps=steps, arrival=arrival, build=build, nodes=n ** 3 * (N + 1) ** 3, order=N)
+
     reqs, t = [], 0.0
-    print("[bench] building %d serving cases" % args.serve_cases, file=sys.stderr, flush=True)
     for k in range(args.serve_cases):
-        n, N = int(rng.integers(8, 17)), int(rng.integers(3, 7))
+        n, N = int(rng.integers(8, args.serve_nmax + 1)), int(rng.integers(3, 7))
         mach = float(rng.uniform(0.1, 0.8))
         re = float(np.exp(rng.uniform(np.log(400.0), np.log(5000.0))))
-        case = mesh.straight_box(n, N)
-        Q0 = mesh.taylor_green(case, mach, phase=tuple(rng.uniform(0.0, 2.0 * np.pi, 3)))
+        phase = tuple(float(x) for x in rng.uniform(0.0, 2.0 * np.pi, 3))
         t += float(rng.exponential(args.serve_gap))
-        reqs.append(serve.Request(case, dict(mach=mach, reynolds=re, riemann="roe", les="smagorinsky"), Q0,
-                                  steps=int(rng.integers(100, 1001)), arrival=t))
-    print("[bench] serving", file=sys.stderr, flush=True)
-    res, st = serve.serve(reqs, chunk=10, max_active=16, max_running=args.serve_running)
-    print("[bench] one at a time", file=sys.stderr, flush=True)
-    alone, st1 = serve.one_at_a_time(reqs)
-    same = all(r.t == a.t and np.array_equal(r.Q, a.Q) for r, a in zip(res, alone))
+        reqs.append(req(n, N, mach, re, phase, int(rng.integers(100, 1001)), t))
+    work = sum(r.nodes * r.steps for r in reqs)
+    print("[bench] serving %d cases, %.3g DOF-updates in all" % (len(reqs), work), file=sys.stderr, flush=True)
+    res, st = serve.serve_batched(reqs, chunk=args.serve_chunk, max_batch=64, max_batches=args.serve_running,
+                                  workers=4, progress=16, keep_states=False)
+    print("[bench] served in %.1f s; one at a time" % st["wall_s"], file=sys.stderr, flush=True)
+    alone, st1 = serve.one_at_a_time(reqs, keep_states=False, progress=16)
+    same = all(r.t == a.t and r.Q == a.Q for r, a in zip(res, alone))
     print(json.dumps({"serve": st, "one_at_a_time": st1, "results_identical": same,
                       "throughput_ratio": st["dof_updates_per_s"] / st1["dof_updates_per_s"],
-                      "max_running": args.serve_running,
-                      "config": {"workload": "Poisson stream of %d periodic-box cases (n^3, n on 8..16; N on 3..6; "
-                                             "Mach 0.1-0.8; Re 400-5000; Roe+BR1+Smagorinsky; 100-1000 steps; "
-                                             "mean arrival gap %g s)" % (len(reqs), args.serve_gap)}}))
+                      "max_batches": args.serve_running,
+                      "config": {"workload": "Poisson stream of %d periodic-box cases (n^3 straight hexes, n on 8..%d; "
+                                             "N on 3..6; Mach 0.1-0.8; Re 400-5000 log-uniform; Roe+BR1+Smagorinsky "
+                                             "Cs 0.2; 100-1000 steps, CFL 0.4 per case; mean arrival gap %g s, seed %d)"
+                                             % (len(reqs), args.serve_nmax, args.serve_gap, args.serve_seed)}}))
 
 
 def main():
@@ -265,10 +276,12 @@ def main():
                     help="only run the serving comparison: a Poisson stream of independent periodic-box "
                          "cases stepped together (continuous batching) against the same stream one case "
                          "at a time on the plain engine; prints one JSON line of both")
-    ap.add_argument("--serve-cases", type=int, default=32)
-    ap.add_argument("--serve-gap", type=float, default=0.05, help="mean arrival gap (s) of the serving stream")
+    ap.add_argument("--serve-cases", type=int, default=256)
+    ap.add_argument("--serve-nmax", type=int, default=24, help="largest box side n of the serving stream")
+    ap.add_argument("--serve-gap", type=float, default=0.2, help="mean arrival gap (s) of the serving stream")
     ap.add_argument("--serve-seed", type=int, default=4)
-    ap.add_argument("--serve-running", type=int, default=2, help="cases fed at a time by the serving loop")
+    ap.add_argument("--serve-chunk", type=int, default=50, help="steps per chunk of a serving engine")
+    ap.add_argument("--serve-running", type=int, default=3, help="several-case engines stepping at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the "

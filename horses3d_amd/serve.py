@@ -31,19 +31,55 @@ BYTES_PER_NODE_GUESS = 1200.0   # device bytes per node before the first engine 
 
 class Request:
     """One simulation: a case dict, engine keyword arguments, initial state, step count and
-    CFL numbers, arriving `arrival` seconds after the loop starts."""
+    CFL numbers, arriving `arrival` seconds after the loop starts.
 
-    def __init__(self, case, physics, Q0, steps, cfl=0.4, dcfl=0.4, arrival=0.0, name=None):
-        self.case, self.physics, self.Q0 = case, physics, Q0
+    A request may instead carry `build`, a function returning (case, Q0), with its node count
+    and its order N (one order everywhere, or None for mixed orders): the arrays are then made
+    when the case is set up and dropped once they are on the device (release), so a long
+    stream never holds every case on the host at once."""
+
+    def __init__(self, case, physics, Q0, steps, cfl=0.4, dcfl=0.4, arrival=0.0, name=None,
+                 build=None, nodes=None, order=None):
+        self._case, self.physics, self._Q0, self.build = case, physics, Q0, build
         self.steps, self.cfl, self.dcfl, self.arrival = int(steps), cfl, dcfl, float(arrival)
         self.name = name
-        self.nodes = int(case["off"][-1]) if "off" in case else len(Q0)
+        if case is not None:
+            self.nodes = int(case["off"][-1]) if "off" in case else len(Q0)
+            N = np.asarray(case.get("N", [-1]))
+            self.order = int(N.flat[0]) if N.flat[0] >= 0 and (N == N.flat[0]).all() else None
+        else:
+            self.nodes, self.order = int(nodes), order
+
+    def _make(self):
+        if self._case is None:
+            self._case, self._Q0 = self.build()
+
+    @property
+    def case(self):
+        self._make()
+        return self._case
+
+    @property
+    def Q0(self):
+        self._make()
+        return self._Q0
+
+    def release(self):
+        """Drops built arrays (a request given its arrays keeps them)."""
+        if self.build is not None:
+            self._case, self._Q0 = None, None
+
+
+def _digest(Q):
+    import hashlib
+    return None if Q is None else hashlib.sha1(np.ascontiguousarray(Q).tobytes()).hexdigest()
 
 
 class Result:
     """The end of one request: final state Q and time t (the state where it stopped for a failed
     case, or None if no engine was built), admission and completion times, and error (None, or
-    the message of the failure that retired the case)."""
+    the message of the failure that retired the case).  With keep_states=False the loops keep
+    only the SHA-1 digest of each final state's bytes (Q is then that hex string)."""
 
     def __init__(self, req, Q, t, admitted, finished, error=None):
         self.request, self.Q, self.t = req, Q, t
@@ -62,7 +98,8 @@ def _stats(results, wall):
 
 
 def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30, poll_s=2e-4,
-          engine_factory=None, device_memory=None, clock=time.perf_counter, sleep=time.sleep, workers=2):
+          engine_factory=None, device_memory=None, clock=time.perf_counter, sleep=time.sleep, workers=2,
+          keep_states=True):
     """Runs the requests with continuous batching; returns (results in request order, stats).
 
     Up to max_active cases are resident; of these, up to max_running have steps queued at a
@@ -86,8 +123,11 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
     most = 0
 
     def prepare(r):
-        eng = make(r)
-        eng.set_state(r.Q0)
+        try:
+            eng = make(r)
+            eng.set_state(r.Q0)
+        finally:
+            r.release()
         eng.t = 0.0
         b = eng.device_bytes()
         if b > 0:
@@ -100,7 +140,7 @@ def serve(requests, chunk=20, max_active=16, max_running=2, hbm_reserve=8 << 30,
             Q = eng.get_state()
         finally:
             eng.close()
-        return t, Q
+        return t, (Q if keep_states else _digest(Q))
 
     pool = ThreadPoolExecutor(max_workers=max(1, workers)) if workers else None
     submit = pool.submit if pool else _run_now
@@ -210,7 +250,8 @@ def _run_now(fn, *a):
         return _Done(None, exc)
 
 
-def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=time.sleep):
+def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=time.sleep, keep_states=True,
+                  progress=None):
     """The same requests in arrival order, each alone on the plain engine (create, step to the
     end, read back, free) and waiting for its arrival; returns (results, stats)."""
     make = engine_factory or (lambda r: E.Engine(r.case, **r.physics))
@@ -226,11 +267,15 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
         try:
             eng = make(r)
         except E.HX3DError as exc:
+            r.release()
             results[i] = Result(r, None, None, admitted, clock() - t0, error=str(exc))
             continue
         err = None
         try:
-            eng.set_state(r.Q0)
+            try:
+                eng.set_state(r.Q0)
+            finally:
+                r.release()
             try:
                 t = eng.advance(r.steps, r.cfl, r.dcfl, t=0.0)
             except E.HX3DDiverged as exc:
@@ -238,19 +283,22 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
             Q = eng.get_state()
         finally:
             eng.close()
-        results[i] = Result(r, Q, t, admitted, clock() - t0, error=err)
+        results[i] = Result(r, Q if keep_states else _digest(Q), t, admitted, clock() - t0, error=err)
+        done = sum(x is not None for x in results)
+        if progress and done % progress == 0:
+            import sys
+            print("[serve] one at a time: %d of %d cases done at %.1f s" % (done, len(requests), clock() - t0),
+                  file=sys.stderr, flush=True)
     wall = clock() - t0
     return results, _stats(results, wall)
 
 
 def batch_key(r):
     """Requests that can share one engine: one order N everywhere and the same discretisation."""
-    N = np.asarray(r.case["N"])
-    n0 = int(N.flat[0])
-    if not (N == n0).all():
+    if r.order is None:
         return None   # mixed orders: served alone
     ph = r.physics
-    return (n0, ph.get("riemann", "roe"), ph.get("inviscid", "standard"), ph.get("averaging", "standard"),
+    return (r.order, ph.get("riemann", "roe"), ph.get("inviscid", "standard"), ph.get("averaging", "standard"),
             ph.get("les"), ph.get("lambda_stab", 1.0), float(r.cfl), float(r.dcfl))
 
 
@@ -266,7 +314,7 @@ class _Batch:
 
 def serve_batched(requests, chunk=50, max_batch=64, max_batches=3, min_batch=8, max_wait_s=0.05,
                   hbm_reserve=8 << 30, poll_s=2e-4, batch_factory=None, device_memory=None,
-                  clock=time.perf_counter, sleep=time.sleep, workers=2, progress=None):
+                  clock=time.perf_counter, sleep=time.sleep, workers=2, progress=None, keep_states=True):
     """Continuous batching with shared launches: arrived requests that can share an engine (one
     order, one discretisation, see batch_key) are gathered into several-case engines
     (engine.MultiEngine), each case with its own physics numbers, CFL dt, time and step count.
@@ -289,15 +337,20 @@ def serve_batched(requests, chunk=50, max_batch=64, max_batches=3, min_batch=8, 
     done_count = [0]
 
     def prepare(rs):
-        eng = make(rs)
-        eng.set_states([r.Q0 for r in rs])
+        try:
+            eng = make(rs)
+            eng.set_states([r.Q0 for r in rs])
+        finally:
+            for r in rs:
+                r.release()
         b = eng.device_bytes()
         if b > 0:
             per_node[0] = max(per_node[0] or 0.0, b / sum(r.nodes for r in rs))
         return eng
 
     def read_case(eng, pos):
-        return eng.get_case_state(pos)
+        Q = eng.get_case_state(pos)
+        return Q if keep_states else _digest(Q)
 
     def close(eng):
         eng.close()

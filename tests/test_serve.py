@@ -249,9 +249,8 @@ class FakeMulti:
 def test_batched_serving_completes_every_case_as_alone():
     """Several-case engines: every request ends with its own step count and the state it has
     alone, cases are grouped only by their batch key, and a case that diverges is retired alone."""
-    reqs = requests(20, seed=13)
-    for k, q in enumerate(reqs):
-        q.case = {"off": q.case["off"], "N": np.full((2, 3), 3 + k % 2)}
+    reqs = [serve.Request({"off": q.case["off"], "N": np.full((2, 3), 3 + k % 2)}, q.physics, q.Q0, q.steps,
+                          cfl=q.cfl, arrival=q.arrival) for k, q in enumerate(requests(20, seed=13))]
     clock = Clock()
     FakeMulti.live = 0
     made = []
@@ -273,3 +272,34 @@ def test_batched_serving_completes_every_case_as_alone():
         if k != 7:
             assert r.error is None and np.array_equal(r.Q, q.Q0 + q.steps * 0.5), k
             assert r.t == pytest.approx(0.1 * q.steps)
+
+
+def test_built_requests_are_made_at_set_up_and_dropped_after_upload():
+    """A request given a builder makes its arrays when its case is set up and drops them once
+    they are on the device, in every loop; the results are those of requests given the arrays."""
+    base = requests(8, seed=5)
+    made = []
+
+    def lazy(q):
+        def build():
+            made.append(q.nodes)
+            return {"off": q.case["off"]}, q.Q0.copy()
+        return serve.Request(None, {}, None, q.steps, cfl=q.cfl, arrival=q.arrival, build=build, nodes=q.nodes)
+
+    ref, _ = run(base, device_memory=lambda: 1 << 40)
+    for loop in ("serve", "one_at_a_time"):
+        made.clear()
+        reqs = [lazy(q) for q in base]
+        clock = Clock()
+        f = (lambda r: FakeEngine(r, clock))
+        if loop == "serve":
+            res, _ = serve.serve(reqs, engine_factory=f, clock=clock, sleep=clock.sleep, poll_s=0.0, workers=0,
+                                 device_memory=lambda: 1 << 40)
+        else:
+            res, _ = serve.one_at_a_time(reqs, engine_factory=f, clock=clock, sleep=clock.sleep)
+        assert sorted(made) == sorted(q.nodes for q in base), loop
+        assert all(q._case is None and q._Q0 is None for q in reqs), loop
+        for a, b in zip(res, ref):
+            assert np.array_equal(a.Q, b.Q) and a.t == b.t, loop
+    digests, _ = run([lazy(q) for q in base], keep_states=False, device_memory=lambda: 1 << 40)
+    assert [d.Q for d in digests] == [serve._digest(b.Q) for b in ref]
