@@ -462,17 +462,19 @@ struct hx3d_engine {
     // mixed orders (p-anisotropic elements, mortar faces)
     bool mixed = false;
     long FSm = 0;
-    int *dN3 = nullptr, *dEfoff = nullptr, *dElist = nullptr, *dEofn = nullptr;
+    int *dN3 = nullptr, *dEfoff = nullptr, *dEofn = nullptr;
     long *dOff = nullptr;
     hxm::FaceRec *dFaces = nullptr;
     hxm::OpsAll *dOpsAll = nullptr;
     double *dU = nullptr, *dLift = nullptr;
-    int bucket_first[9] = {0}, bucket_count[9] = {0};   // elements needing w waves: w = 1..8
+    // mixed orders: the element tiles of the persistent element kernels (packing policy
+    // tile_policy, see hx3d_autotune) and each kernel's grid (blocks resident at once)
+    hxm::Tile *dTiles = nullptr;
+    int ntiles = 0, tile_policy = 0, grid_mgrad = 0, grid_mvol = 0;
+    std::vector<int> N3h_mixed;   // [3K] element orders (host copy for re-tiling)
     // side streams for the element-bucket launches (the buckets touch disjoint elements, so
     // they run side by side), forked from and joined back to the engine stream with events
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
-    bool capturing = false;   // a step graph is being captured: buckets stay on the engine stream
+    bool capturing = false;   // a step graph is being captured
     std::vector<int> N3h;   // element orders (Nx, Ny, Nz), host copy for solution files
     double mach = 0;
     bool qdot_valid = false, timing = false;
@@ -655,50 +657,19 @@ void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 // ------------------------------------------------------------------------------------
 // mixed orders
 // ------------------------------------------------------------------------------------
-// The bucket launches of one element kernel: forked to the side streams, bucket b on stream
-// b mod 4 (0: the engine stream), joined back before the next kernel of the stage.
-bool buckets_spread(const hx3d_engine *eng) { return eng->side[0] && !eng->capturing; }
-
-void buckets_fork(hx3d_engine *eng)
-{
-    if (!buckets_spread(eng)) return;
-    (void)hipEventRecord(eng->fork_ev, eng->stream);
-    for (int k = 0; k < 3; ++k) (void)hipStreamWaitEvent(eng->side[k], eng->fork_ev, 0);
-}
-
-hipStream_t bucket_stream(const hx3d_engine *eng, int b)
-{
-    return buckets_spread(eng) && b % 4 ? eng->side[b % 4 - 1] : eng->stream;
-}
-
-void buckets_join(hx3d_engine *eng)
-{
-    if (!buckets_spread(eng)) return;
-    for (int k = 0; k < 3; ++k) {
-        (void)hipEventRecord(eng->join_ev[k], eng->side[k]);
-        (void)hipStreamWaitEvent(eng->stream, eng->join_ev[k], 0);
-    }
-}
+// The persistent element kernels: one launch sweeps every element, grid_m* blocks (as many as
+// are resident at once) taking the tiles in turn.
+template <int FORM>
+size_t mvolume_lds() { return sizeof(double) * (2 * hxm::MAXN1 * 64 + (size_t)hxm::mvolume_per_node<FORM>() * hxm::TILE); }
+size_t mgrad_lds() { return sizeof(double) * (hxm::MAXN1 * 64 + (size_t)NV * hxm::TILE); }
 
 template <int FORM>
 void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
-    constexpr bool SPLIT = FORM >= 0;
-    constexpr int ND = hx::node_data_size(SPLIT ? FORM : 0);
-    const int per_node = 3 * (SPLIT ? 4 : 5) + (SPLIT ? ND + 3 : 0);
-    DtArgs a = dta;   // the time is advanced by the first launch only
-    buckets_fork(eng);
-    int b = 0;
-    for (int w = 1; w <= 8; ++w) {
-        if (!eng->bucket_count[w]) continue;
-        const size_t lds = sizeof(double) * (per_node * 64 * w + 6 * 64);
-        hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->bucket_count[w]), dim3(64 * w), lds, bucket_stream(eng, b++),
-                           eng->dElist + eng->bucket_first[w], eng->dQ, eng->dU, eng->dN3, eng->dOff,
-                           eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG,
-                           qdot, a_k, c_k, update, a, eng->S, eng->ES, eng->ph);
-        a.time = nullptr;
-    }
-    buckets_join(eng);
+    hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->grid_mvol), dim3(hxm::TILE), mvolume_lds<FORM>(), eng->stream,
+                       eng->dTiles, eng->ntiles, eng->dQ, eng->dU, eng->dN3, eng->dOff, eng->dEfoff, eng->dJa, eng->dJ,
+                       eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG, qdot, a_k, c_k, update, dta, eng->S, eng->ES,
+                       eng->ph);
 }
 
 void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
@@ -710,17 +681,9 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         hipLaunchKernelGGL(hxm::km_face_grad, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
                            eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
                            eng->S, eng->ES, eng->FSm);
-        buckets_fork(eng);
-        int b = 0;
-        for (int w = 1; w <= 8; ++w) {
-            if (!eng->bucket_count[w]) continue;
-            hipLaunchKernelGGL(hxm::km_grad, dim3(eng->bucket_count[w]), dim3(64 * w), sizeof(double) * (NV * 64 * w + 3 * 64),
-                               bucket_stream(eng, b++), eng->dElist + eng->bucket_first[w], eng->dQ, eng->dN3,
-                               eng->dOff, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S,
-                               eng->ES, eng->ph, lim);
-            if (lim.part) lim.part += 2 * (long)eng->bucket_count[w];   // each bucket's own partials
-        }
-        buckets_join(eng);
+        hipLaunchKernelGGL(hxm::km_grad, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream, eng->dTiles,
+                           eng->ntiles, eng->dQ, eng->dN3, eng->dOff, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
+                           eng->dOpsAll, eng->dU, eng->S, eng->ES, eng->ph, lim);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
     hipLaunchKernelGGL(hxm::km_face, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
@@ -799,6 +762,108 @@ void set_physics(hx3d_engine *eng, const hx3d_case_desc *d)
 // Mixed orders: element and face tables (Face_LinkWithElements, FaceClass.f90:187-278), the
 // operators of every order, element buckets, device arrays.  The face frames at the face
 // orders must come with the descriptor (MappedGeometryFace of the mortar faces).
+// Element tiles of the persistent element kernels under packing policy `policy`:
+//   0  largest elements first, each into the first tile with room (first-fit decreasing):
+//      the fewest tiles and idle threads;
+//   1  elements of one order triple (Nx, Ny, Nz) together, in turn: tiles of like elements, so
+//      the contractions' loop bounds agree across a wave;
+//   2  one element per tile.
+// Every policy gives the same results (each element is computed from its own data alone; the
+// dt partials are minima).  Also sets each element kernel's grid: the blocks resident at once.
+int set_tiles(hx3d_engine *eng, int policy)
+{
+    const std::vector<int> &N3 = eng->N3h_mixed;
+    const int K = eng->K;
+    std::vector<int> ne(K);
+    for (int e = 0; e < K; ++e) ne[e] = (N3[3 * e] + 1) * (N3[3 * e + 1] + 1) * (N3[3 * e + 2] + 1);
+    std::vector<int> order(K);
+    for (int e = 0; e < K; ++e) order[e] = e;
+    std::vector<hxm::Tile> tiles;
+    auto open_tile = [&]() {
+        hxm::Tile t;
+        std::memset(&t, 0, sizeof(t));
+        tiles.push_back(t);
+        return (int)tiles.size() - 1;
+    };
+    auto add = [&](int ti, int e) {
+        hxm::Tile &t = tiles[ti];
+        t.elem[t.count] = e;
+        t.off[t.count + 1] = t.off[t.count] + ne[e];
+        ++t.count;
+    };
+    auto fits = [&](int ti, int e) {
+        const hxm::Tile &t = tiles[ti];
+        return t.count < hxm::TILE_MAXE && t.off[t.count] + ne[e] <= hxm::TILE;
+    };
+    if (policy == 0) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ne[a] > ne[b]; });
+        const int min_ne = K ? ne[order[K - 1]] : 0;
+        int first_open = 0;   // tiles before it hold no element of any size left
+        for (int e : order) {
+            int ti = -1;
+            for (int x = first_open; x < (int)tiles.size(); ++x)
+                if (fits(x, e)) { ti = x; break; }
+            if (ti < 0) ti = open_tile();
+            add(ti, e);
+            while (first_open < (int)tiles.size() &&
+                   (tiles[first_open].count == hxm::TILE_MAXE || tiles[first_open].off[tiles[first_open].count] + min_ne > hxm::TILE))
+                ++first_open;
+        }
+    } else if (policy == 1) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (ne[a] != ne[b]) return ne[a] > ne[b];
+            for (int c = 0; c < 3; ++c)
+                if (N3[3 * a + c] != N3[3 * b + c]) return N3[3 * a + c] < N3[3 * b + c];
+            return false;
+        });
+        int ti = -1;
+        for (int x = 0; x < K; ++x) {
+            const int e = order[x];
+            const bool same = ti >= 0 && std::equal(&N3[3 * e], &N3[3 * e + 3], &N3[3 * tiles[ti].elem[0]]);
+            if (ti < 0 || !same || !fits(ti, e)) ti = open_tile();
+            add(ti, e);
+        }
+    } else if (policy == 2) {
+        for (int e = 0; e < K; ++e) add(open_tile(), e);
+    } else {
+        return eng->fail(HX3D_EINVAL, "unknown tile packing policy");
+    }
+    for (const hxm::Tile &t : tiles)
+        if (t.count < 1 || t.off[t.count] > hxm::TILE) return eng->fail(HX3D_EUNSUPPORTED, "an element has more nodes than a tile");
+    if (eng->dTiles) {
+        if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "synchronize failed");
+        (void)hipFree(eng->dTiles);
+        eng->dTiles = nullptr;
+    }
+    int rc;
+    if ((rc = dalloc(eng, &eng->dTiles, tiles.size()))) return rc;
+    if (hipMemcpy(eng->dTiles, tiles.data(), tiles.size() * sizeof(hxm::Tile), hipMemcpyHostToDevice) != hipSuccess)
+        return eng->fail(HX3D_EDEVICE, "tile upload failed");
+    eng->ntiles = (int)tiles.size();
+    eng->tile_policy = policy;
+    // grids: the blocks resident at once on the device (at least one), no more than the tiles
+    int dev = 0, cus = 0, bg = 0, bv = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return eng->fail(HX3D_EDEVICE, "no HIP device");
+    hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bg, reinterpret_cast<const void *>(&hxm::km_grad),
+                                                                 hxm::TILE, mgrad_lds());
+    const bool split = eng->inviscid == HX3D_INVISCID_SPLIT;
+    const int avg = eng->ph.averaging;
+    auto occ = [&](const void *k, size_t lds) {
+        if (oe == hipSuccess) oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k, hxm::TILE, lds);
+    };
+    if (!split) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::FORM_STANDARD>), mvolume_lds<hxk::FORM_STANDARD>());
+    else if (avg == HX3D_AVG_DUCROS) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_DUCROS>), mvolume_lds<hxk::AVG_DUCROS>());
+    else if (avg == HX3D_AVG_KENNEDYGRUBER) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_KG>), mvolume_lds<hxk::AVG_KG>());
+    else if (avg == HX3D_AVG_PIROZZOLI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_PIROZZOLI>), mvolume_lds<hxk::AVG_PIROZZOLI>());
+    else if (avg == HX3D_AVG_MORINISHI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_MORINISHI>), mvolume_lds<hxk::AVG_MORINISHI>());
+    else occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_STANDARD>), mvolume_lds<hxk::AVG_STANDARD>());
+    if (oe != hipSuccess) return eng->fail(HX3D_EDEVICE, std::string("occupancy query: ") + hipGetErrorString(oe));
+    eng->grid_mgrad = std::max(1, std::min(eng->ntiles, std::max(1, bg) * cus));
+    eng->grid_mvol = std::max(1, std::min(eng->ntiles, std::max(1, bv) * cus));
+    return HX3D_OK;
+}
+
 int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
 {
     eng->mixed = true;
@@ -903,27 +968,19 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
             }
         fdelta[f] = std::sqrt(surf / ((r.Nf[0] + 1) * (r.Nf[1] + 1)));
     }
-    // element buckets by the waves one element needs
-    std::vector<int> elist;
-    for (int w = 1; w <= 8; ++w) {
-        eng->bucket_first[w] = (int)elist.size();
-        for (int e = 0; e < K; ++e)
-            if ((int)((off[e + 1] - off[e] + 63) / 64) == w) elist.push_back(e);
-        eng->bucket_count[w] = (int)elist.size() - eng->bucket_first[w];
-    }
     {
         // the volume kernel keeps three directions of fluxes (+ two-point node data and a metric
-        // row for the split form) of one element in LDS
+        // row for the split form) of a tile's elements in LDS
         const bool split = d->inviscid == HX3D_INVISCID_SPLIT;
-        const int per_node = 3 * (split ? 4 : 5) + (split ? hx::node_data_size(d->averaging) + 3 : 0);
+        const long per_node = 3 * (split ? 4 : 5) + (split ? hx::node_data_size(d->averaging) + 3 : 0);
         int dev = 0, maxlds = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
             return eng->fail(HX3D_EDEVICE, "no HIP device");
-        for (int w = 1; w <= 8; ++w)
-            if (eng->bucket_count[w] && (long)sizeof(double) * per_node * 64 * w > maxlds)
-                return eng->fail(HX3D_EUNSUPPORTED, "an element of this size needs more LDS than a block may use");
+        if ((long)sizeof(double) * (per_node * hxm::TILE + 2 * hxm::MAXN1 * 64) > maxlds)
+            return eng->fail(HX3D_EUNSUPPORTED, "a tile of elements needs more LDS than a block may use");
     }
+    eng->N3h_mixed = N3;
     int rc;
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
         (rc = dalloc(eng, &eng->dU, (size_t)15 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
@@ -937,17 +994,11 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         (rc = dalloc(eng, &eng->dStage, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dBits, 2 * DT_SLOTS)) ||
         (rc = dalloc(eng, &eng->dFlag, 2)) || (rc = dalloc(eng, &eng->dN3, N3.size())) ||
         (rc = dalloc(eng, &eng->dOff, off.size())) || (rc = dalloc(eng, &eng->dEfoff, efoff.size())) ||
-        (rc = dalloc(eng, &eng->dElist, elist.size())) || (rc = dalloc(eng, &eng->dEofn, eofn.size())) ||
+        (rc = dalloc(eng, &eng->dEofn, eofn.size())) ||
         (rc = dalloc(eng, &eng->dFaces, faces.size())) || (rc = dalloc(eng, &eng->dOpsAll, 1)))
         return rc;
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "hipStreamCreate failed");
-    for (int k = 0; k < 3; ++k)
-        if (hipStreamCreateWithFlags(&eng->side[k], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&eng->join_ev[k], hipEventDisableTiming) != hipSuccess)
-            return eng->fail(HX3D_EDEVICE, "hipStreamCreate failed");
-    if (hipEventCreateWithFlags(&eng->fork_ev, hipEventDisableTiming) != hipSuccess)
-        return eng->fail(HX3D_EDEVICE, "hipEventCreate failed");
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
@@ -956,7 +1007,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
         !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dN3, N3.data(), N3.size() * 4) ||
         !up(eng->dOff, off.data(), off.size() * 8) || !up(eng->dEfoff, efoff.data(), efoff.size() * 4) ||
-        !up(eng->dElist, elist.data(), elist.size() * 4) || !up(eng->dEofn, eofn.data(), eofn.size() * 4) ||
+        !up(eng->dEofn, eofn.data(), eofn.size() * 4) ||
         !up(eng->dFaces, faces.data(), faces.size() * sizeof(hxm::FaceRec)) ||
         !up(eng->dOpsAll, &oa, sizeof(oa)))
         return eng->fail(HX3D_EDEVICE, "upload failed");
@@ -966,6 +1017,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
     hipLaunchKernelGGL(k_fill, dim3((15 * S + 255) / 256), dim3(256), 0, eng->stream, eng->dU, 15L * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, eng->stream, eng->dScal, 4L, 0.0);
     if (int rc2 = init_flags(eng)) return rc2;
+    if (int rc3 = set_tiles(eng, 0)) return rc3;
     if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "initialisation failed");
     return HX3D_OK;
 }
@@ -1330,18 +1382,13 @@ int hx3d_engine_destroy(hx3d_engine *eng)
     if (eng->stream) (void)hipStreamSynchronize(eng->stream);
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
-                    eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff, eng->dElist,
-                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart,
+                    eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff,
+                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart, eng->dTiles,
                     eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, eng->dTimeM};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : eng->evpool) (void)hipEventDestroy(e);
     if (eng->step_graph) (void)hipGraphExecDestroy(eng->step_graph);
-    for (int k = 0; k < 3; ++k) {
-        if (eng->side[k]) (void)hipStreamDestroy(eng->side[k]);
-        if (eng->join_ev[k]) (void)hipEventDestroy(eng->join_ev[k]);
-    }
-    if (eng->fork_ev) (void)hipEventDestroy(eng->fork_ev);
     if (eng->hFlag) (void)hipHostFree(eng->hFlag);
     if (eng->stream) (void)hipStreamDestroy(eng->stream);
     delete eng;
@@ -1551,7 +1598,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
                                    eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dDtPart);
             nparts = (int)nb;
         } else {
-            nparts = eng->mixed ? eng->K : grad_grid(eng);
+            nparts = eng->mixed ? eng->grid_mgrad : grad_grid(eng);
         }
         for (int k = 0; k < 3; ++k) {
             DtArgs a{};
@@ -1745,6 +1792,71 @@ int hx3d_set_step_graph(hx3d_engine *eng, int on)
 {
     if (!eng) return HX3D_EINVAL;
     eng->use_graph = on != 0;
+    return HX3D_OK;
+}
+
+int hx3d_autotune(hx3d_engine *eng, int reps, int *choice, double *ms)
+{
+    if (!eng || reps < 1) return HX3D_EINVAL;
+    if (!eng->mixed) {   // one tiling: nothing to choose
+        if (choice) *choice = -1;
+        return HX3D_OK;
+    }
+    constexpr int NPOL = 3;
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableSystemFence));
+    if (hipEventCreateWithFlags(&ev[1], hipEventDisableSystemFence) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return eng->fail(HX3D_EDEVICE, "hipEventCreate failed");
+    }
+    int best = -1, rc = HX3D_OK;
+    float best_ms = 0.f;
+    for (int pol = 0; pol < NPOL && rc == HX3D_OK; ++pol) {
+        if ((rc = set_tiles(eng, pol))) break;
+        // one untimed evaluation, then reps timed ones of the resident state's time derivative
+        for (int r = 0; r <= reps; ++r) {
+            if (r == 1) (void)hipEventRecord(ev[0], eng->stream);
+            stage(eng, 0.0, 0.0, 0, host_dt(eng, 0.0), eng->dQDot);
+        }
+        (void)hipEventRecord(ev[1], eng->stream);
+        float t = 0.f;
+        if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess) {
+            rc = eng->fail(HX3D_EDEVICE, "autotune timing failed");
+            break;
+        }
+        if (ms) ms[pol] = t / reps;
+        if (best < 0 || t < best_ms) { best = pol; best_ms = t; }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
+    if ((rc = set_tiles(eng, best))) return rc;
+    if (eng->step_graph) {   // a captured step holds the old tiles
+        HIPCHK(hipGraphExecDestroy(eng->step_graph));
+        eng->step_graph = nullptr;
+    }
+    eng->qdot_valid = true;   // dQDot holds the resident state's derivative
+    if (choice) *choice = best;
+    return HX3D_OK;
+}
+
+int hx3d_set_tiling(hx3d_engine *eng, int policy)
+{
+    if (!eng || !eng->mixed) return HX3D_EINVAL;
+    int rc = set_tiles(eng, policy);
+    if (rc) return rc;
+    if (eng->step_graph) {
+        HIPCHK(hipGraphExecDestroy(eng->step_graph));
+        eng->step_graph = nullptr;
+    }
+    return HX3D_OK;
+}
+
+int hx3d_get_tiling(const hx3d_engine *eng, int *policy, int *ntiles)
+{
+    if (!eng || !eng->mixed) return HX3D_EINVAL;
+    if (policy) *policy = eng->tile_policy;
+    if (ntiles) *ntiles = eng->ntiles;
     return HX3D_OK;
 }
 

@@ -7,13 +7,14 @@
 // One stage = km_face_grad -> km_grad -> km_face -> km_volume:
 //   km_face_grad  one wave per face: both Q traces to the face order, BR1 unStar there,
 //                 projected back onto each side's element face (lift[15][ES]);
-//   km_grad       one block per element: local gradient + lifting -> grad Q (15 values);
+//   km_grad       element tiles: local gradient + lifting -> grad Q (15 values);
 //   km_face       one wave per face: Q and grad Q traces to the face order, Riemann + BR1
 //                 fluxes there, projected back (right side negated) and scaled by the
 //                 receiving element's surface weight -> fe[5][ES];
-//   km_volume     one block per element: volume integral, surface integral, 1/J, RK update and
+//   km_volume     element tiles: volume integral, surface integral, 1/J, RK update and
 //                 the next step's dt limits.
-// Element kernels run once per bucket of elements needing the same number of waves.
+// The element kernels are persistent: one launch per stage sweeps every element, each block
+// taking tiles (elements packed into at most 512 nodes, one thread per node) in turn.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -46,6 +47,25 @@ struct FaceRec {
     int Nf[2], rot, proj[2];    // proj: 0 none, 1 first index, 2 second, 3 both
     int foff;                   // first face node (face order) of this face
 };
+
+// A tile of elements for the persistent element kernels: up to TILE_MAXE elements whose nodes,
+// one thread each, fill at most TILE threads; thread t works on node t - off[s] of elem[s],
+// off[s] <= t < off[s + 1] (threads from off[count] on are idle).  Which elements share a tile
+// (the packing) is chosen on the host (hx3d_autotune); every packing gives the same results.
+constexpr int TILE = 512, TILE_MAXE = 30;
+struct Tile {
+    int count;
+    int elem[TILE_MAXE];
+    int off[TILE_MAXE + 1];
+};
+
+// slot of thread t in tile T (uniform tile, so the offsets come through scalar loads)
+__device__ __forceinline__ int tile_slot(const Tile &T, int t)
+{
+    int s = 0;
+    for (int x = 1; x < T.count; ++x) s = t >= T.off[x] ? x : s;
+    return s;
+}
 
 // element face slot 0..5 = EFRONT, EBACK, EBOTTOM, ERIGHT, ETOP, ELEFT: the fixed axis, its end,
 // and the axes of the slot's face coordinates (a, b)
@@ -184,6 +204,83 @@ __device__ __forceinline__ void project_k(const double (*m)[64], const FaceRec &
     }
 }
 
+// The two-direction case (projection type 3) of adapt_k / project_k, factorised: along the
+// first index into tmp (lanes (a, l) of the half-interpolated grid), then along the second.
+// Each pass is the reference's interpolation along one index; the pair does (N0 + N1 + 2) K
+// multiply-adds per node instead of (N0 + 1)(N1 + 1) K, in a different rounding order (within
+// the 1e-12 parity bar).  Every lane of the wave calls it (it passes two block barriers);
+// lanes without a node get 0.  src: K values at src[k*64 + m + (S0+1) l] of a grid of
+// (S0+1) x (S1+1) nodes; out: the node (i, j) of the (D0+1) x (D1+1) grid; T0 (D0+1) x (S0+1),
+// T1 (D1+1) x (S1+1); tmp: K x 64 doubles of LDS.
+template <int K>
+__device__ __forceinline__ void interp2_k(const double *T0, const double *T1, int S0, int S1, int D0, int D1,
+                                          const double *src, double *tmp, int lane, int i, int j, bool on,
+                                          double *out)
+{
+    if (lane < (D0 + 1) * (S1 + 1)) {
+        const int a = lane % (D0 + 1), l = lane / (D0 + 1);
+        double acc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = 0.0;
+        for (int m = 0; m <= S0; ++m) {
+            const double w = T0[a * (S0 + 1) + m];
+            const int q = m + (S0 + 1) * l;
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k] += w * src[k * 64 + q];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) tmp[k * 64 + lane] = acc[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = 0.0;
+    if (on) {
+        for (int l = 0; l <= S1; ++l) {
+            const double w = T1[j * (S1 + 1) + l];
+            const int q = i + (D0 + 1) * l;
+#pragma unroll
+            for (int k = 0; k < K; ++k) out[k] += w * tmp[k * 64 + q];
+        }
+    }
+    __syncthreads();   // tmp read: the next call may overwrite it
+}
+
+// Face_AdaptSolutionToFace / Face_ProjectFluxToElements at node (i, j) of the destination grid,
+// lanes without a node (on = false) giving 0.  FACT: faces interpolated along both indices take
+// the factorised pair of passes.  That is for the values that only reach the viscous terms (the
+// BR1 traces and lifting of the gradient, the gradients at the face): there the different
+// rounding order stays far inside the parity bar.  The state traces of the Riemann solver and
+// the projected face fluxes keep the reference's order: the surface integral cancels against the
+// volume integral, and factorising those measured 1.7e-12 on the 20^3 p-anisotropic case.
+// Uniform per face: every lane of the wave calls it.
+template <int K, bool FACT>
+__device__ __forceinline__ void adapt_face(const double (*m)[64], const FaceRec &r, int side, const double *tr,
+                                           double *tmp, int lane, int i, int j, bool on, double *out)
+{
+    if (FACT && r.proj[side] == 3) {
+        interp2_k<K>(m[0], m[1], r.Nfs[side][0], r.Nfs[side][1], r.Nf[0], r.Nf[1], tr, tmp, lane, i, j, on, out);
+    } else if (on) {
+        adapt_k<K>(m, r, side, tr, i, j, out);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = 0.0;
+    }
+}
+
+template <int K, bool FACT>
+__device__ __forceinline__ void project_face(const double (*m)[64], const FaceRec &r, int side, const double *fl,
+                                             double *tmp, int lane, int i, int j, bool on, double *out)
+{
+    if (FACT && r.proj[side] == 3) {
+        interp2_k<K>(m[2], m[3], r.Nf[0], r.Nf[1], r.Nfs[side][0], r.Nfs[side][1], fl, tmp, lane, i, j, on, out);
+    } else if (on) {
+        project_k<K>(m, r, side, fl, i, j, out);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = 0.0;
+    }
+}
+
 // Stage the side's element trace of K node arrays src[c*S + g] into tr[c*64 + q] in the face
 // orientation (the right side is rotated, Face_AdaptSolutionToFace).  One wave, lane = q.
 template <int K>
@@ -225,6 +322,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     __shared__ double sT[2][NV * 64];
     __shared__ double sU[NV][64];
     __shared__ double sM[2][4][64];
+    __shared__ double sTmp[NV * 64];   // the half-interpolated grid of a factorised interpolation
     // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
     const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
@@ -233,37 +331,37 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     stage_mortar(*opsp, r, sM, lane);
     gather_trace_k<NV>(r, 0, N3, off, Q, S, sT[0], lane);
     gather_trace_k<NV>(r, 1, N3, off, Q, S, sT[1], lane);
-    __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
-    double uS[NV], nrm[3];
-    if (lane < nq) {
-        const int i = lane % (n0 + 1), j = lane / (n0 + 1);
-        const long fq = r.foff + lane;
-        nrm[0] = fgeo[fq];
-        nrm[1] = fgeo[FSm + fq];
-        nrm[2] = fgeo[2 * FSm + fq];
-        const double jac = fgeo[9 * FSm + fq];
-        double qL[NV], qR[NV];
-        adapt_k<NV>(sM[0], r, 0, sT[0], i, j, qL);
-        adapt_k<NV>(sM[1], r, 1, sT[1], i, j, qR);
+    const bool on = lane < nq;
+    const int i = on ? lane % (n0 + 1) : 0, j = on ? lane / (n0 + 1) : 0;
+    const long fq = r.foff + (on ? lane : 0);
+    double nrm[3];
+    nrm[0] = fgeo[fq];
+    nrm[1] = fgeo[FSm + fq];
+    nrm[2] = fgeo[2 * FSm + fq];
+    const double jac = fgeo[9 * FSm + fq];
+    __syncthreads();
+    double uS[NV], qL[NV], qR[NV];
+    adapt_face<NV, true>(sM[0], r, 0, sT[0], sTmp, lane, i, j, on, qL);
+    adapt_face<NV, true>(sM[1], r, 1, sT[1], sTmp, lane, i, j, on, qR);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
-    }
+    for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
     // unStar(v) n_d, one direction at a time, projected onto both sides: lift[(5d + v)*ES + x]
     for (int d = 0; d < 3; ++d) {
         if (d) __syncthreads();   // the previous direction's projections have read sU
-        if (lane < nq) {
+        if (on) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) sU[v][lane] = uS[v] * nrm[d];
         }
         __syncthreads();
         for (int side = 0; side < 2; ++side) {
             const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
-            if (lane < (N0 + 1) * (N1 + 1)) {
-                const int i = lane % (N0 + 1), j = lane / (N0 + 1);
-                const long x = elem_face_node(r, side, efoff, i, j);
-                double lv[NV];
-                project_k<NV>(sM[side], r, side, sU[0], i, j, lv);
+            const bool son = lane < (N0 + 1) * (N1 + 1);
+            const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
+            double lv[NV];
+            project_face<NV, true>(sM[side], r, side, sU[0], sTmp, lane, si, sj, son, lv);
+            if (son) {
+                const long x = elem_face_node(r, side, efoff, si, sj);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) lift[(d * NV + v) * ES + x] = lv[v];
             }
@@ -276,20 +374,28 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
 // BR1 lifting (BR1_GradientFaceLoop, EllipticBR1.f90:531-569, with the weights b of
 // VectorWeakIntegrals_StdFace): grad Q at every node, U[(5d + v)*S + g].
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(512)
-km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *__restrict__ N3,
+__global__ void __launch_bounds__(TILE)
+km_grad(const Tile *__restrict__ tiles, int ntiles, const double *__restrict__ Q, const int *__restrict__ N3,
         const long *__restrict__ off, const int *__restrict__ efoff, const double *__restrict__ Ja,
         const double *__restrict__ Jac, const double *__restrict__ lift, const OpsAll *__restrict__ opsp,
         double *__restrict__ U, long S, long ES, Phys ph, hxk::DtLimits dtl)
 {
     HIP_DYNAMIC_SHARED(double, smem)
-    const int e = elist[blockIdx.x];
     const OpsAll &ops = *opsp;
+    const int t = threadIdx.x;
+    double *sD = smem;                   // D of every order, [MAXN1][64] (visible after the first barrier)
+    double *sQall = smem + MAXN1 * 64;   // per element [NV][ne], at NV * off[slot]
+    for (int x = t; x < MAXN1 * 64; x += blockDim.x) sD[x] = ops.D[x / 64][x % 64];
+    double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this thread's nodes (stage 1)
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const Tile &T = tiles[tile];
+    const int slot = tile_slot(T, t);
+    const int e = T.elem[slot];
     const int Nx = N3[3 * e], Ny = N3[3 * e + 1], Nz = N3[3 * e + 2];
     const int nx = Nx + 1, ny = Ny + 1, nz = Nz + 1, ne = nx * ny * nz;
-    const int n = threadIdx.x;
-    const bool active = n < ne;
-    const int nc = active ? n : 0;   // loads are unconditional (see k_volume)
+    const bool active = t < T.off[T.count];
+    const int nc = active ? t - T.off[slot] : 0;   // loads are unconditional (see k_volume)
+    const int n = nc;
     const int i = nc % nx, j = (nc / nx) % ny, k = nc / (nx * ny);
     const long g = off[e] + nc;
     double q[NV], ja[9];
@@ -303,17 +409,14 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
     const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
     const long xy = efoff[6 * e + (j == 0 ? 0 : 1)] + i + nx * k;
     const long xz = efoff[6 * e + (k == 0 ? 2 : 4)] + i + nx * j;
-    double *sQ = smem;             // [NV][ne]
-    double *sD = smem + NV * ne;   // D of the three orders, [3][64]
+    double *sQ = sQall + NV * T.off[slot];   // [NV][ne]
     if (active) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) sQ[v * ne + n] = q[v];
     }
-    for (int t = n; t < 3 * 64; t += blockDim.x) sD[t] = ops.D[N3[3 * e + t / 64]][t % 64];
     __syncthreads();
-    double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this node (stage 1)
     if (active) {
-    const double *Dx = sD, *Dy = sD + 64, *Dz = sD + 128;
+    const double *Dx = sD + 64 * Nx, *Dy = sD + 64 * Ny, *Dz = sD + 64 * Nz;
     double uxi[NV], ueta[NV], uzeta[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { uxi[v] = 0.0; ueta[v] = 0.0; uzeta[v] = 0.0; }
@@ -358,10 +461,16 @@ km_grad(const int *__restrict__ elist, const double *__restrict__ Q, const int *
             U[(d * NV + v) * S + g] = (onx || ony || onz) ? gr[d] + fi * iJ : gr[d];
         }
     }
-    if (dtl.part)
-        hx::dt_limits(ph, q, ja, jg, ops.dcsi[Nx], ops.dcsi[Ny], ops.dcsi[Nz], dtl.cfl, dtl.dcfl, tc, tv);
+    if (dtl.part) {
+        double c0, c1;
+        hx::dt_limits(ph, q, ja, jg, ops.dcsi[Nx], ops.dcsi[Ny], ops.dcsi[Nz], dtl.cfl, dtl.dcfl, c0, c1);
+        tc = fmin(tc, c0);
+        tv = fmin(tv, c1);
     }
-    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, n, blockIdx.x);
+    }
+    __syncthreads();   // the tile's sQ has been read: the next tile overwrites it
+    }
+    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, t, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -380,6 +489,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     __shared__ double sQ[2][NV * 64];
     __shared__ double sG[3 * NV * 64];
     __shared__ double sM[2][4][64];
+    __shared__ double sTmp[NV * 64];   // the half-interpolated grid of a factorised interpolation
     // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
     const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
@@ -406,15 +516,19 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     }
     const double jac = fgeo[9 * FSm + fq];
     double qs[2][NV];
-    adapt_k<NV>(sM[0], r, 0, sQ[0], i, j, qs[0]);
-    adapt_k<NV>(sM[1], r, 1, sQ[1], i, j, qs[1]);
+    adapt_face<NV, false>(sM[0], r, 0, sQ[0], sTmp, lane, i, j, on, qs[0]);
+    adapt_face<NV, false>(sM[1], r, 1, sQ[1], sTmp, lane, i, j, on, qs[1]);
     double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (!ph.euler) {
         // BR1_RiemannSolver (EllipticBR1.f90:816-868): each side's viscous flux from its
-        // gradients at the face order, the sides one after the other through sG
-        auto side_flux = [&](int s, double (*Fs)[3]) {
-            double g[3 * NV], vg[12];
-            adapt_k<3 * NV>(sM[s], r, s, sG, i, j, g);
+        // gradients at the face order, the sides one after the other through sG; the
+        // gradients (15 values) are interpolated five at a time (the size of sTmp)
+        auto side_grad = [&](int s, double *g) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(sM[s], r, s, sG + c * NV * 64, sTmp, lane, i, j, on, g + c * NV);
+        };
+        auto side_flux = [&](int s, const double *g, double (*Fs)[3]) {
+            double vg[12];
             hx::viscous_state(ph, qs[s], g, g + NV, g + 2 * NV, vg);
             double mu, kappa;
             hx::laminar_mu_kappa(ph, qs[s], mu, kappa);
@@ -426,13 +540,15 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
             const double ir = 1.0 / qs[s][0];
             hx::viscous_flux(mu, kappa, qs[s][1] * ir, qs[s][2] * ir, qs[s][3] * ir, vg, Fs);
         };
-        double Fs[2][NV][3];
-        if (on) side_flux(0, Fs[0]);
+        double Fs[2][NV][3], g[3 * NV];
+        side_grad(0, g);
+        if (on) side_flux(0, g, Fs[0]);
         __syncthreads();   // side 0's gradients consumed
         gather_trace_k<3 * NV>(r, 1, N3, off, U, S, sG, lane);
         __syncthreads();
+        side_grad(1, g);
         if (on) {
-            side_flux(1, Fs[1]);
+            side_flux(1, g, Fs[1]);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 visc[v] = 0.5 * (Fs[0][v][0] + Fs[1][v][0]) * nrm[0] + 0.5 * (Fs[0][v][1] + Fs[1][v][1]) * nrm[1] +
@@ -451,13 +567,14 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     __syncthreads();
     for (int side = 0; side < 2; ++side) {
         const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
-        if (lane < (N0 + 1) * (N1 + 1)) {
-            const int i = lane % (N0 + 1), j = lane / (N0 + 1);
-            const long x = elem_face_node(r, side, efoff, i, j);
+        const bool son = lane < (N0 + 1) * (N1 + 1);
+        const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
+        double a[NV];
+        project_face<NV, false>(sM[side], r, side, sF, sTmp, lane, si, sj, son, a);
+        if (son) {
+            const long x = elem_face_node(r, side, efoff, si, sj);
             const int e = r.e[side], s = r.slot[side];
             const double b = opsp->b0[N3[3 * e + M_SLOT_AXIS[s]]];
-            double a[NV];
-            project_k<NV>(sM[side], r, side, sF, i, j, a);
 #pragma unroll
             for (int v = 0; v < NV; ++v) fe[v * ES + x] = (side == 0 ? a[v] : -1.0 * a[v]) * b;
         }
@@ -467,11 +584,18 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
 // ------------------------------------------------------------------------------------
 // km_volume: as hxk::k_volume with the element's own orders per direction, the viscous
 // state formed from grad Q (ViscousFlux_STATE), and MaxTimeStep's 1/|x_1 - x_0| per direction.
-// LDS (dynamic): sF[3][NF][ne], sN[ND][ne], sJ[3][ne], then hatD and sharpD [2][3][64].
+// LDS (dynamic): hatD and sharpD of every order [2][MAXN1][64], then per element of the tile,
+// at per_node * off[slot]: sF[3][NF][ne], sN[ND][ne], sJ[3][ne].
 // ------------------------------------------------------------------------------------
 template <int FORM>
-__global__ void __launch_bounds__(512)
-km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *__restrict__ U,
+constexpr int mvolume_per_node()
+{
+    return 3 * (FORM >= 0 ? 4 : 5) + (FORM >= 0 ? hx::node_data_size(FORM) + 3 : 0);
+}
+
+template <int FORM>
+__global__ void __launch_bounds__(TILE)
+km_volume(const Tile *__restrict__ tiles, int ntiles, double *__restrict__ Q, const double *__restrict__ U,
           const int *__restrict__ N3, const long *__restrict__ off, const int *__restrict__ efoff,
           const double *__restrict__ Ja, const double *__restrict__ Jac, const double *__restrict__ edelta,
           const double *__restrict__ fe, const OpsAll *__restrict__ opsp, double *__restrict__ G,
@@ -483,27 +607,30 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
     constexpr int ND = hx::node_data_size(AVG);
     constexpr bool SYM = hx::two_point_symmetric(AVG);
     constexpr int NF = SPLIT ? 4 : 5, V0 = SPLIT ? 1 : 0;
+    constexpr int PER_NODE = mvolume_per_node<FORM>();
     HIP_DYNAMIC_SHARED(double, smem)
-    const int e = elist[blockIdx.x];
     const OpsAll &ops = *opsp;
+    const int t = threadIdx.x;
+    // hatD and sharpD of every order (read at LDS latency in the contractions below; visible
+    // after the first barrier of the direction loop)
+    double *sOp = smem;
+    for (int x = t; x < 2 * MAXN1 * 64; x += blockDim.x)
+        sOp[x] = x < MAXN1 * 64 ? ops.hatD[x / 64][x % 64] : ops.sharpD[(x / 64) - MAXN1][x % 64];
+    double *chunk0 = smem + 2 * MAXN1 * 64;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const Tile &T = tiles[tile];
+    const int slot = tile_slot(T, t);
+    const int e = T.elem[slot];
     const int Nx = N3[3 * e], Ny = N3[3 * e + 1], Nz = N3[3 * e + 2];
     const int nx = Nx + 1, ny = Ny + 1, nz = Nz + 1, ne = nx * ny * nz;
-    const int t = threadIdx.x;
-    const bool active = t < ne;
-    const int n = active ? t : 0;
+    const bool active = t < T.off[T.count];
+    const int n = active ? t - T.off[slot] : 0;
     const int i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
     const long g = off[e] + n;
     const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
-    double *sF = smem;                        // [3][NF][ne]
-    double *sN = sF + 3 * NF * ne;            // [ND][ne]
-    double *sJ = sN + (SPLIT ? ND * ne : 0);  // [3][ne]
-    // hatD and sharpD of the element's three orders, [2][3][64] (read at LDS latency in the
-    // contractions below; made visible by the first barrier of the direction loop)
-    double *sOp = sJ + (SPLIT ? 3 * ne : 0);
-    for (int tt = t; tt < 6 * 64; tt += blockDim.x) {
-        const int dd = (tt / 64) % 3, c = tt % 64;
-        sOp[tt] = tt < 3 * 64 ? ops.hatD[N3[3 * e + dd]][c] : ops.sharpD[N3[3 * e + dd]][c];
-    }
+    double *sF = chunk0 + PER_NODE * T.off[slot];   // [3][NF][ne]
+    double *sN = sF + 3 * NF * ne;                  // [ND][ne]
+    double *sJ = sN + (SPLIT ? ND * ne : 0);        // [3][ne]
     // loads, unconditional (see hxk::k_volume)
     double sx[NV], sy[NV], sz[NV];
     {
@@ -602,7 +729,7 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
         const int Nd = d == 0 ? Nx : (d == 1 ? Ny : Nz);
         const int stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
         const int base = n - own * stride;
-        const double *hatD = sOp + 64 * d, *sharpD = sOp + 3 * 64 + 64 * d;
+        const double *hatD = sOp + 64 * Nd, *sharpD = sOp + MAXN1 * 64 + 64 * Nd;
         if (!SPLIT) {
             for (int l = 0; l <= Nd; ++l) {
                 const double hh = hatD[own * 8 + l];
@@ -658,7 +785,7 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
             hxk::hold(dt0);
             hxk::hold(dt1);
             const double dt = fmin(dt0, dt1);
-            if (dta.time && blockIdx.x == 0 && t == 0) dta.time[0] += dt;
+            if (dta.time && tile == 0 && t == 0) dta.time[0] += dt;   // once per launch
             const double cdt = c_k * dt;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -671,6 +798,8 @@ km_volume(const int *__restrict__ elist, double *__restrict__ Q, const double *_
             if (dta.nan && (isnan(q[0]) || isnan(q[1]) || isnan(q[2]) || isnan(q[3]) || isnan(q[4])))
                 atomicOr(dta.nan, 1);
         }
+    }
+    __syncthreads();   // the tile's LDS has been read: the next tile overwrites it
     }
 }
 

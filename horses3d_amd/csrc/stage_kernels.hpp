@@ -620,8 +620,16 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
         }
         if (SPLIT || d == 0) __syncthreads();
         if (d == 2) {
+            // The first RK stage (a_k = 0) does not read the register: the reference's
+            // 0 G + QDot is QDot up to the sign of an exact zero, which no state value sees
+            // (Q + c dt (+-0) = Q); every later stage reads it.
+            if (a_k != 0.0) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) gk[v] = ldg(G + v * S, gc);
+                for (int v = 0; v < NV; ++v) gk[v] = ldg(G + v * S, gc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) gk[v] = 0.0;
+            }
             dt0 = bits_to_double(dta.in[2 * cs]);
             dt1 = bits_to_double(dta.in[2 * cs + 1]);
         }

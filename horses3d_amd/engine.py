@@ -69,6 +69,7 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_create_multi", "hx3d_multi_set_act
            "hx3d_residuals_noalloc",
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_query", "hx3d_get_time", "hx3d_engine_device_bytes",
            "hx3d_device_memory", "hx3d_set_timing", "hx3d_set_step_graph",
+           "hx3d_autotune", "hx3d_set_tiling", "hx3d_get_tiling",
            "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
            "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
            "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
@@ -117,6 +118,9 @@ def load_library(path=LIB_PATH):
     L.hx3d_device_memory.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     L.hx3d_set_timing.argtypes = [P, I]
     L.hx3d_set_step_graph.argtypes = [P, I]
+    L.hx3d_autotune.argtypes = [P, I, C.POINTER(I), P]
+    L.hx3d_set_tiling.argtypes = [P, I]
+    L.hx3d_get_tiling.argtypes = [P, C.POINTER(I), C.POINTER(I)]
     L.hx3d_kernel_times.argtypes = [P, P]
     L.hx3d_mesh_read_specmesh.argtypes = [C.c_char_p, I, P, C.c_char_p, C.POINTER(C.c_void_p)]
     L.hx3d_mesh_error.argtypes = [P]
@@ -462,6 +466,22 @@ class Engine:
     def set_step_graph(self, on):
         """advance() replays a captured HIP graph per step (on) or issues the launches (default)."""
         self._chk(load_library().hx3d_set_step_graph(self.h, 1 if on else 0), "hx3d_set_step_graph")
+
+    def autotune(self, reps=5):
+        """Mixed orders: times each element tiling on the resident state and keeps the fastest;
+        returns (policy, ms per evaluation of each policy), or (-1, None) for one order."""
+        ch = C.c_int()
+        ms = np.zeros(3)
+        self._chk(load_library().hx3d_autotune(self.h, int(reps), C.byref(ch), _p(ms)), "hx3d_autotune")
+        return ch.value, (ms if ch.value >= 0 else None)
+
+    def set_tiling(self, policy):
+        self._chk(load_library().hx3d_set_tiling(self.h, int(policy)), "hx3d_set_tiling")
+
+    def tiling(self):
+        pol, nt = C.c_int(), C.c_int()
+        self._chk(load_library().hx3d_get_tiling(self.h, C.byref(pol), C.byref(nt)), "hx3d_get_tiling")
+        return pol.value, nt.value
 
     def set_timing(self, on):
         self._chk(load_library().hx3d_set_timing(self.h, 1 if on else 0), "hx3d_set_timing")

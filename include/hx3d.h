@@ -196,6 +196,17 @@ int hx3d_get_time(hx3d_engine *eng, double *t);
 long hx3d_engine_device_bytes(const hx3d_engine *eng);
 /* Free and total memory of the current HIP device (admission by free HBM). */
 int hx3d_device_memory(size_t *free_bytes, size_t *total_bytes);
+/* No counterpart in the reference (its elements run one at a time).  Mixed orders: the element
+ * kernels are persistent, one launch per stage over tiles of elements (one thread per node, at
+ * most 512 nodes a tile).  The tiling (which elements share a block) is a packing policy:
+ * 0 first-fit decreasing by size, 1 elements of one order triple together, 2 one element per
+ * tile.  hx3d_autotune times reps evaluations of the resident state's time derivative under each
+ * policy (ms[p] per evaluation when ms != NULL, 3 entries), keeps the fastest and returns it in
+ * *choice; results do not depend on the policy.  A uniform-order engine has one tiling:
+ * *choice = -1.  hx3d_set_tiling / hx3d_get_tiling set and read the policy (mixed orders). */
+int hx3d_autotune(hx3d_engine *eng, int reps, int *choice, double *ms);
+int hx3d_set_tiling(hx3d_engine *eng, int policy);
+int hx3d_get_tiling(const hx3d_engine *eng, int *policy, int *ntiles);
 /* Average duration (ms) per launch of each stage kernel over the last hx3d_advance call,
  * measured with HIP events on the engine stream when timing was enabled with
  * hx3d_set_timing(eng, 1): out[0..2] = gradient+lifting, face flux, volume+surface+RK;

@@ -182,6 +182,35 @@ def test_mixed_orders_box_matches_oracle(inviscid, averaging, riemann, les):
     eng.close()
 
 
+@pytest.mark.parametrize("inviscid", ["standard", "split-form"])
+def test_mixed_tilings_agree_bit_for_bit(inviscid):
+    """The persistent element kernels of mixed orders give the same bits under every element
+    tiling (first-fit decreasing, order-triple groups, one element per tile): QDot and two
+    device-dt steps; hx3d_autotune keeps one of them and leaves the resident state's QDot."""
+    case = mesh.periodic_box_mixed(5, mesh.random_orders(125, 1, 7, 11))
+    Q0 = mesh.taylor_green(case, 0.2, noise=1e-3, seed=12)
+    kw = dict(mach=0.2, reynolds=1600.0, riemann="lxf" if inviscid == "split-form" else "roe", inviscid=inviscid,
+              averaging="pirozzoli" if inviscid == "split-form" else "standard", les="wale", les_c=None)
+    eng = engine_for(case, kw)
+    out = []
+    for pol in (0, 1, 2):
+        eng.set_tiling(pol)
+        assert eng.tiling()[0] == pol
+        eng.set_state(Q0)
+        qd = eng.qdot()
+        t = eng.advance(2, 0.4, 0.4, t=0.0)
+        out.append((qd, eng.get_state(), t))
+        print("tiling", pol, "tiles", eng.tiling()[1], flush=True)
+    for qd, Q, t in out[1:]:
+        assert np.array_equal(qd, out[0][0]) and np.array_equal(Q, out[0][1]) and t == out[0][2]
+    eng.set_state(Q0)
+    choice, ms = eng.autotune(reps=3)
+    print("autotune", choice, ms, flush=True)
+    assert choice in (0, 1, 2) and eng.tiling()[0] == choice and np.all(ms > 0)
+    assert np.array_equal(eng.get_qdot(), out[0][0])
+    eng.close()
+
+
 def test_box20_p2to7_matches_oracle():
     """The p-anisotropic batching case at its full size: a 20^3 box, orders (Nx, Ny, Nz) on
     {2..7} per element (seed 3; 1.32 M nodes, mortar faces), Taylor-Green state with 1e-3
