@@ -557,8 +557,8 @@ void launch_grad(hx3d_engine *eng, const hxk::DtLimits &dtl)
 template <int N>
 void launch_face(hx3d_engine *eng, const hxk::DtReduce &dtr)
 {
-    const int nb = grid_for((int)((eng->FS + 255) / 256));
-    hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(256), 0, eng->stream, eng->dQ, eng->dVgrad,
+    const int nb = grid_for((int)((eng->FS + hxk::FACE_TPB - 1) / hxk::FACE_TPB));
+    hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ, eng->dVgrad,
                        eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
                        1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
 }
@@ -1200,7 +1200,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     // ---- device -------------------------------------------------------------------
     int rc;
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||
-        (rc = dalloc(eng, &eng->dVgrad, (size_t)12 * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
+        (rc = dalloc(eng, &eng->dVgrad, (size_t)(eng->ph.les ? 13 : 12) * S)) || (rc = dalloc(eng, &eng->dJa, (size_t)9 * S)) ||
         (rc = dalloc(eng, &eng->dWq, (size_t)S)) ||
         (rc = dalloc(eng, &eng->dJ, (size_t)S)) || (rc = dalloc(eng, &eng->dFgeo, (size_t)10 * FS)) ||
         (rc = dalloc(eng, &eng->dFe, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dEgeo, (size_t)4 * ES)) || (rc = dalloc(eng, &eng->dQDot, (size_t)NV * S)) ||
@@ -1328,7 +1328,7 @@ int hx3d_engine_create_multi(const hx3d_case_desc *descs, int ncases, hx3d_engin
         cc[c].mu_to_kappa = 1.0 / (eng->ph.gm1 * M2 * d.prandtl);
         cc[c].les_c = d.les_c;
     }
-    if ((long)eng->F * nf / 256 + 1 < ncases)
+    if ((long)eng->F * nf / hxk::FACE_TPB + 1 < ncases)
         return eng->fail(HX3D_EUNSUPPORTED, "more cases than face blocks to reduce their dt");
     // per-case replacements of the one-case dt slots, flags and time
     (void)hipStreamSynchronize(eng->stream);

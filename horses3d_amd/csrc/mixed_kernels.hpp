@@ -533,7 +533,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
             double mu, kappa;
             hx::laminar_mu_kappa(ph, qs[s], mu, kappa);
             if (ph.les) {
-                const double mt = hx::les_viscosity(ph, fdelta[f], qs[s], vg);
+                const double mt = hx::les_mu(ph, fdelta[f], qs[s][0], hx::les_factor(ph, qs[s], g, g + NV, g + 2 * NV));
                 mu = mu + mt;
                 kappa = kappa + mt * ph.mu_to_kappa;
             }
@@ -664,7 +664,7 @@ km_volume(const Tile *__restrict__ tiles, int ntiles, double *__restrict__ Q, co
             double mu, kappa;
             hx::laminar_mu_kappa(ph, q, mu, kappa);
             if (ph.les) {
-                const double mt = hx::les_viscosity(ph, dl, q, vg);
+                const double mt = hx::les_mu(ph, dl, q[0], hx::les_factor(ph, q, gq[0], gq[1], gq[2]));
                 mu = mu + mt;
                 kappa = kappa + mt * ph.mu_to_kappa;
             }

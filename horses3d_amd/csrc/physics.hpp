@@ -127,14 +127,24 @@ HX_HD void viscous_flux(double mu, double kappa, double u, double v, double w, c
     F[4][2] = F[1][2] * u + F[2][2] * v + F[3][2] * w + kappa * vg[11];
 }
 
-// common/LESModels.f90:256-307 Smagorinsky and :358-~440 WALE eddy viscosity (no wall
-// model), velocity gradients of getVelocityGradients_State
-// (navierstokes/VariableConversion_NS.f90:373-392).
-HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], const double vg[12])
+// common/LESModels.f90:256-307 Smagorinsky and :358-~440 WALE eddy viscosity (no wall model),
+// mu_t = rho (C delta)^2 X, split in two: les_factor gives X, the part that does not depend on
+// the filter width delta (the element's or the face's), from the state and its conservative
+// gradients with the velocity gradients formed as getVelocityGradients_State forms them
+// (VariableConversion_NS.f90:373-392: uDivRho = Q(IRHOU:IRHOW) invRho^2); les_mu applies
+// rho (C delta)^2 in the reference's order.
+HX_HD double les_factor(const Phys &ph, const double q[NV], const double gx[NV], const double gy[NV],
+                        const double gz[NV])
 {
-    // velocity gradients: the viscous-flux form of vg (they agree with
-    // getVelocityGradients_State up to the rounding of u/rho vs Q/rho^2)
-    const double *ux = vg, *uy = vg + 3, *uz = vg + 6;
+    const double invRho = 1.0 / q[0], invRho2 = invRho * invRho;
+    const double uDR[3] = {q[1] * invRho2, q[2] * invRho2, q[3] * invRho2};
+    double ux[3], uy[3], uz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        ux[d] = invRho * gx[1 + d] - uDR[d] * gx[0];
+        uy[d] = invRho * gy[1 + d] - uDR[d] * gy[0];
+        uz[d] = invRho * gz[1 + d] - uDR[d] * gz[0];
+    }
     if (ph.les == 1) {
         // S(:,1) = U_x, S(:,2) = U_y, S(:,3) = U_z; S(1,:) += U_x ...; S = S/2; |S| = sqrt(2 sum S*S)
         double S[3][3];
@@ -147,9 +157,7 @@ HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], con
         for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int a = 0; a < 3; ++a) { const double s = 0.5 * S[a][b]; ss += s * s; }
-        const double normS = sqrt(2.0 * ss);
-        const double LS = ph.les_c * delta;
-        return q[0] * (LS * LS) * normS;
+        return sqrt(2.0 * ss);
     }
     if (ph.les == 2) {
         double G[3][3], S[3][3], G2[3][3], Sd[3][3];
@@ -182,12 +190,16 @@ HX_HD double les_viscosity(const Phys &ph, double delta, const double q[NV], con
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int i = 0; i < 3; ++i) normSd += Sd[i][j] * Sd[i][j];
-        const double LS = ph.les_c * delta;
-        double mu = q[0] * (LS * LS) * (pow(normSd, 1.5) / (pow(normS, 2.5) + pow(normSd, 1.25)));
-        if (normS < 1.0e-8 && normSd < 1.0e-8) mu = 0.0;
-        return mu;
+        if (normS < 1.0e-8 && normSd < 1.0e-8) return 0.0;   // mu = 0
+        return pow(normSd, 1.5) / (pow(normS, 2.5) + pow(normSd, 1.25));
     }
     return 0.0;
+}
+
+HX_HD double les_mu(const Phys &ph, double delta, double rho, double X)
+{
+    const double LS = ph.les_c * delta;
+    return rho * (LS * LS) * X;
 }
 
 // navierstokes/RiemannSolvers_NS.f90:1541-1656 RoeRiemannSolver: the one-sided Roe flux with
