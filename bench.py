@@ -247,13 +247,14 @@ def serve_bench(args):
     work = sum(r.nodes * r.steps for r in reqs)
     print("[bench] serving %d cases, %.3g DOF-updates in all" % (len(reqs), work), file=sys.stderr, flush=True)
     res, st = serve.serve_batched(reqs, chunk=args.serve_chunk, max_batch=64, max_batches=args.serve_running,
-                                  workers=4, progress=16, keep_states=False)
+                                  workers=args.serve_workers, progress=16, keep_states=False)
     print("[bench] served in %.1f s; one at a time" % st["wall_s"], file=sys.stderr, flush=True)
     alone, st1 = serve.one_at_a_time(reqs, keep_states=False, progress=16)
     same = all(r.t == a.t and r.Q == a.Q for r, a in zip(res, alone))
     print(json.dumps({"serve": st, "one_at_a_time": st1, "results_identical": same,
                       "throughput_ratio": st["dof_updates_per_s"] / st1["dof_updates_per_s"],
-                      "max_batches": args.serve_running,
+                      "one_at_a_time_stepping_only_dof_updates_per_s": st1["dof_updates"] / st1["stepping_s"],
+                      "max_batches": args.serve_running, "host_workers": args.serve_workers,
                       "config": {"workload": "Poisson stream of %d periodic-box cases (n^3 straight hexes, n on 8..%d; "
                                              "N on 3..6; Mach 0.1-0.8; Re 400-5000 log-uniform; Roe+BR1+Smagorinsky "
                                              "Cs 0.2; 100-1000 steps, CFL 0.4 per case; mean arrival gap %g s, seed %d)"
@@ -299,6 +300,7 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=4)
     ap.add_argument("--serve-chunk", type=int, default=50, help="steps per chunk of a serving engine")
     ap.add_argument("--serve-running", type=int, default=3, help="several-case engines stepping at once")
+    ap.add_argument("--serve-workers", type=int, default=8, help="host threads that set up and retire cases")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the "
@@ -421,6 +423,23 @@ def main():
                 kernels={k: dict(avg_launch_ms=round(v["ms"], 5), GBs=round(v["GBs"], 1)) for k, v in kern.items()},
                 fp64_vector=fp64)
     eng.close()
+    single_batch = None
+    if args.largest:
+        # the same case through the batched engine as a single-case batch (block table, per-case
+        # constants and dt slot): the same steps from the same state
+        from horses3d_amd.engine import MultiEngine
+        print("[bench] single-case batch", file=sys.stderr, flush=True)
+        me = MultiEngine([case], [kw])
+        me.set_states([Q0])
+        me.advance(args.warmup, 0.4, 0.4)
+        me.synchronize()
+        t0 = time.perf_counter()
+        me.advance(args.steps, 0.4, 0.4)
+        me.synchronize()
+        elb = time.perf_counter() - t0
+        times, bad = me.status()
+        me.close()
+        single_batch = dict(value=nd * args.steps / elb, ms_per_step=elb / args.steps * 1e3, diverged=bool(bad[0]))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -457,6 +476,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if single_batch:
+        line["single_case_batch"] = single_batch
     print(json.dumps(line))
 
 

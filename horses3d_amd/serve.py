@@ -257,6 +257,7 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
     make = engine_factory or (lambda r: E.Engine(r.case, **r.physics))
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     results = [None] * len(requests)
+    stepping = 0.0   # time inside the step loops (the rest is set-up, transfers and waiting)
     t0 = clock()
     for i in order:
         r = requests[i]
@@ -276,10 +277,12 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
                 eng.set_state(r.Q0)
             finally:
                 r.release()
+            ts = clock()
             try:
                 t = eng.advance(r.steps, r.cfl, r.dcfl, t=0.0)
             except E.HX3DDiverged as exc:
                 t, err = eng.time(), str(exc)
+            stepping += clock() - ts
             Q = eng.get_state()
         finally:
             eng.close()
@@ -290,7 +293,9 @@ def one_at_a_time(requests, engine_factory=None, clock=time.perf_counter, sleep=
             print("[serve] one at a time: %d of %d cases done at %.1f s" % (done, len(requests), clock() - t0),
                   file=sys.stderr, flush=True)
     wall = clock() - t0
-    return results, _stats(results, wall)
+    st = _stats(results, wall)
+    st["stepping_s"] = stepping
+    return results, st
 
 
 def batch_key(r):
