@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -469,7 +470,7 @@ struct hx3d_engine {
     double *dU = nullptr, *dLift = nullptr;
     // mixed orders: the element tiles of the persistent element kernels (packing policy
     // tile_policy, see hx3d_autotune) and each kernel's grid (blocks resident at once)
-    hxm::Tile *dTiles = nullptr;
+    hxm::NodeMap *dTmap = nullptr;   // per tile thread: node, element, node indices and orders
     int ntiles = 0, tile_policy = 0, grid_mgrad = 0, grid_mvol = 0;
     std::vector<int> N3h_mixed;   // [3K] element orders (host copy for re-tiling)
     // side streams for the element-bucket launches (the buckets touch disjoint elements, so
@@ -667,7 +668,7 @@ template <int FORM>
 void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
     hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->grid_mvol), dim3(hxm::TILE), mvolume_lds<FORM>(), eng->stream,
-                       eng->dTiles, eng->ntiles, eng->dQ, eng->dU, eng->dN3, eng->dOff, eng->dEfoff, eng->dJa, eng->dJ,
+                       eng->dTmap, eng->ntiles, eng->dQ, eng->dU, eng->dEfoff, eng->dJa, eng->dJ,
                        eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG, qdot, a_k, c_k, update, dta, eng->S, eng->ES,
                        eng->ph);
 }
@@ -681,8 +682,8 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         hipLaunchKernelGGL(hxm::km_face_grad, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
                            eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
                            eng->S, eng->ES, eng->FSm);
-        hipLaunchKernelGGL(hxm::km_grad, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream, eng->dTiles,
-                           eng->ntiles, eng->dQ, eng->dN3, eng->dOff, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
+        hipLaunchKernelGGL(hxm::km_grad, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream, eng->dTmap,
+                           eng->ntiles, eng->dQ, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
                            eng->dOpsAll, eng->dU, eng->S, eng->ES, eng->ph, lim);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
@@ -830,14 +831,36 @@ int set_tiles(hx3d_engine *eng, int policy)
     }
     for (const hxm::Tile &t : tiles)
         if (t.count < 1 || t.off[t.count] > hxm::TILE) return eng->fail(HX3D_EUNSUPPORTED, "an element has more nodes than a tile");
-    if (eng->dTiles) {
+    // the threads' node map (hxm::NodeMap): node offsets as the device's element offsets
+    std::vector<long> eoff(K + 1, 0);
+    for (int e = 0; e < K; ++e) eoff[e + 1] = eoff[e] + ne[e];
+    if (eoff[K] > INT_MAX) return eng->fail(HX3D_EUNSUPPORTED, "mixed orders: more than 2^31 - 1 nodes");
+    std::vector<hxm::NodeMap> map(tiles.size() * (size_t)hxm::TILE);
+    for (size_t ti = 0; ti < tiles.size(); ++ti) {
+        const hxm::Tile &T = tiles[ti];
+        int slot = 0;
+        for (int t = 0; t < hxm::TILE; ++t) {
+            const bool active = t < T.off[T.count];
+            while (active && t >= T.off[slot + 1]) ++slot;
+            const int s = active ? slot : T.count - 1;
+            const int e = T.elem[s], n = active ? t - T.off[s] : 0;
+            const int nx = N3[3 * e] + 1, ny = N3[3 * e + 1] + 1;
+            const int i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
+            hxm::NodeMap &m = map[ti * hxm::TILE + t];
+            m.g = (int)(eoff[e] + n);
+            m.e = e;
+            m.code = i | j << 3 | k << 6 | N3[3 * e] << 9 | N3[3 * e + 1] << 12 | N3[3 * e + 2] << 15 |
+                     (active ? T.off[s] : 0) << 18 | (active ? 1 : 0) << 27;
+        }
+    }
+    if (eng->dTmap) {
         if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "synchronize failed");
-        (void)hipFree(eng->dTiles);
-        eng->dTiles = nullptr;
+        (void)hipFree(eng->dTmap);
+        eng->dTmap = nullptr;
     }
     int rc;
-    if ((rc = dalloc(eng, &eng->dTiles, tiles.size()))) return rc;
-    if (hipMemcpy(eng->dTiles, tiles.data(), tiles.size() * sizeof(hxm::Tile), hipMemcpyHostToDevice) != hipSuccess)
+    if ((rc = dalloc(eng, &eng->dTmap, map.size()))) return rc;
+    if (hipMemcpy(eng->dTmap, map.data(), map.size() * sizeof(hxm::NodeMap), hipMemcpyHostToDevice) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "tile upload failed");
     eng->ntiles = (int)tiles.size();
     eng->tile_policy = policy;
@@ -1383,7 +1406,7 @@ int hx3d_engine_destroy(hx3d_engine *eng)
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff,
-                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart, eng->dTiles,
+                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart, eng->dTmap,
                     eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, eng->dTimeM};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);

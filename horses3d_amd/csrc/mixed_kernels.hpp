@@ -48,10 +48,11 @@ struct FaceRec {
     int foff;                   // first face node (face order) of this face
 };
 
-// A tile of elements for the persistent element kernels: up to TILE_MAXE elements whose nodes,
-// one thread each, fill at most TILE threads; thread t works on node t - off[s] of elem[s],
-// off[s] <= t < off[s + 1] (threads from off[count] on are idle).  Which elements share a tile
-// (the packing) is chosen on the host (hx3d_autotune); every packing gives the same results.
+// A tile of elements for the persistent element kernels (host side): up to TILE_MAXE elements
+// whose nodes, one thread each, fill at most TILE threads; thread t works on node t - off[s] of
+// elem[s], off[s] <= t < off[s + 1] (threads from off[count] on are idle).  Which elements share
+// a tile (the packing) is chosen on the host (hx3d_autotune); every packing gives the same
+// results.  The kernels read the tiles through their node map (NodeMap).
 constexpr int TILE = 512, TILE_MAXE = 30;
 struct Tile {
     int count;
@@ -59,12 +60,19 @@ struct Tile {
     int off[TILE_MAXE + 1];
 };
 
-// slot of thread t in tile T (uniform tile, so the offsets come through scalar loads)
-__device__ __forceinline__ int tile_slot(const Tile &T, int t)
+// What thread t of a tile works on, built on the host with the tiles (one entry per thread of
+// every tile, at [tile * TILE + t]): node g of element e, and code = i | j << 3 | k << 6 |
+// Nx << 9 | Ny << 12 | Nz << 15 | (first thread of the element in the tile) << 18 | active << 27.
+// One load gives a thread everything the tile table, the orders and the node offsets would
+// give through three dependent ones; the element kernels load the next tile's entry while
+// they work on the current one.  Idle threads (active = 0) carry node 0 of the tile's last
+// element, so that their unconditional loads stay in bounds.
+struct NodeMap {
+    int g, e, code;
+};
+__device__ __forceinline__ NodeMap load_map(const NodeMap *__restrict__ map, int tile, int t)
 {
-    int s = 0;
-    for (int x = 1; x < T.count; ++x) s = t >= T.off[x] ? x : s;
-    return s;
+    return map[(long)tile * TILE + t];
 }
 
 // element face slot 0..5 = EFRONT, EBACK, EBOTTOM, ERIGHT, ETOP, ELEFT: the fixed axis, its end,
@@ -317,12 +325,12 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
              const int *__restrict__ efoff, const FaceRec *__restrict__ faces, const double *__restrict__ fgeo,
              const OpsAll *__restrict__ opsp, double *__restrict__ lift, int F, long S, long ES, long FSm)
 {
-    // LDS: both sides' Q traces, one direction's unStar, the mortar matrices (11.5 KB: the
-    // block count per CU is then bound by registers, not LDS)
+    // LDS: both sides' Q traces, then one direction's unStar in their place; the mortar
+    // matrices; the half-interpolated grid of a factorised interpolation (11.5 KB)
     __shared__ double sT[2][NV * 64];
-    __shared__ double sU[NV][64];
     __shared__ double sM[2][4][64];
-    __shared__ double sTmp[NV * 64];   // the half-interpolated grid of a factorised interpolation
+    __shared__ double sTmp[NV * 64];
+    double *sU = sT[0];   // [NV][64], once both traces are at the face order
     // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
     const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
@@ -348,10 +356,10 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
     // unStar(v) n_d, one direction at a time, projected onto both sides: lift[(5d + v)*ES + x]
     for (int d = 0; d < 3; ++d) {
-        if (d) __syncthreads();   // the previous direction's projections have read sU
+        __syncthreads();   // the traces (d = 0), the previous direction's unStar (d > 0) have been read
         if (on) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) sU[v][lane] = uS[v] * nrm[d];
+            for (int v = 0; v < NV; ++v) sU[v * 64 + lane] = uS[v] * nrm[d];
         }
         __syncthreads();
         for (int side = 0; side < 2; ++side) {
@@ -359,7 +367,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
             const bool son = lane < (N0 + 1) * (N1 + 1);
             const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
             double lv[NV];
-            project_face<NV, true>(sM[side], r, side, sU[0], sTmp, lane, si, sj, son, lv);
+            project_face<NV, true>(sM[side], r, side, sU, sTmp, lane, si, sj, son, lv);
             if (son) {
                 const long x = elem_face_node(r, side, efoff, si, sj);
 #pragma unroll
@@ -375,8 +383,8 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
 // VectorWeakIntegrals_StdFace): grad Q at every node, U[(5d + v)*S + g].
 // ------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(TILE)
-km_grad(const Tile *__restrict__ tiles, int ntiles, const double *__restrict__ Q, const int *__restrict__ N3,
-        const long *__restrict__ off, const int *__restrict__ efoff, const double *__restrict__ Ja,
+km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__ Q,
+        const int *__restrict__ efoff, const double *__restrict__ Ja,
         const double *__restrict__ Jac, const double *__restrict__ lift, const OpsAll *__restrict__ opsp,
         double *__restrict__ U, long S, long ES, Phys ph, hxk::DtLimits dtl)
 {
@@ -387,29 +395,29 @@ km_grad(const Tile *__restrict__ tiles, int ntiles, const double *__restrict__ Q
     double *sQall = smem + MAXN1 * 64;   // per element [NV][ne], at NV * off[slot]
     for (int x = t; x < MAXN1 * 64; x += blockDim.x) sD[x] = ops.D[x / 64][x % 64];
     double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this thread's nodes (stage 1)
+    NodeMap nm = load_map(tmap, blockIdx.x < ntiles ? blockIdx.x : 0, t);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const Tile &T = tiles[tile];
-    const int slot = tile_slot(T, t);
-    const int e = T.elem[slot];
-    const int Nx = N3[3 * e], Ny = N3[3 * e + 1], Nz = N3[3 * e + 2];
+    const int e = nm.e, code = nm.code;
+    const long g = nm.g;   // loads are unconditional (see k_volume)
+    const int i = code & 7, j = (code >> 3) & 7, k = (code >> 6) & 7;
+    const int Nx = (code >> 9) & 7, Ny = (code >> 12) & 7, Nz = (code >> 15) & 7;
     const int nx = Nx + 1, ny = Ny + 1, nz = Nz + 1, ne = nx * ny * nz;
-    const bool active = t < T.off[T.count];
-    const int nc = active ? t - T.off[slot] : 0;   // loads are unconditional (see k_volume)
-    const int n = nc;
-    const int i = nc % nx, j = (nc / nx) % ny, k = nc / (nx * ny);
-    const long g = off[e] + nc;
+    const int first = (code >> 18) & 511;
+    const bool active = (code >> 27) & 1;
+    const int n = i + nx * (j + ny * k);
     double q[NV], ja[9];
 #pragma unroll
     for (int v = 0; v < NV; ++v) q[v] = Q[v * S + g];
 #pragma unroll
     for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
     const double jg = Jac[g];
+    if (tile + (int)gridDim.x < ntiles) nm = load_map(tmap, tile + gridDim.x, t);   // the next tile's
     const double iJ = 1.0 / jg;   // the same bits as a stored 1/J
     const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
     const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
     const long xy = efoff[6 * e + (j == 0 ? 0 : 1)] + i + nx * k;
     const long xz = efoff[6 * e + (k == 0 ? 2 : 4)] + i + nx * j;
-    double *sQ = sQall + NV * T.off[slot];   // [NV][ne]
+    double *sQ = sQall + NV * first;   // [NV][ne]
     if (active) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) sQ[v * ne + n] = q[v];
@@ -507,13 +515,9 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     const bool on = lane < nq;
     const int i = on ? lane % (n0 + 1) : 0, j = on ? lane / (n0 + 1) : 0;
     const long fq = r.foff + (on ? lane : 0);
-    double nrm[3], t1[3], t2[3];
+    double nrm[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        nrm[c] = fgeo[c * FSm + fq];
-        t1[c] = fgeo[(3 + c) * FSm + fq];
-        t2[c] = fgeo[(6 + c) * FSm + fq];
-    }
+    for (int c = 0; c < 3; ++c) nrm[c] = fgeo[c * FSm + fq];
     const double jac = fgeo[9 * FSm + fq];
     double qs[2][NV];
     adapt_face<NV, false>(sM[0], r, 0, sQ[0], sTmp, lane, i, j, on, qs[0]);
@@ -559,8 +563,17 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     double *sF = sG;   // face flux [NV][64]
     if (on) {
         double inv[NV];
-        if (ph.riemann == 1) hx::lxf_flux(ph, qs[0], qs[1], nrm, t1, t2, inv);
-        else hx::roe_flux(ph, qs[0], qs[1], nrm, inv);
+        if (ph.riemann == 1) {
+            double t1[3], t2[3];   // the tangents: the Lax-Friedrichs flux alone reads them
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                t1[c] = fgeo[(3 + c) * FSm + fq];
+                t2[c] = fgeo[(6 + c) * FSm + fq];
+            }
+            hx::lxf_flux(ph, qs[0], qs[1], nrm, t1, t2, inv);
+        } else {
+            hx::roe_flux(ph, qs[0], qs[1], nrm, inv);
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) sF[v * 64 + lane] = (inv[v] - visc[v]) * jac;
     }
@@ -595,8 +608,8 @@ constexpr int mvolume_per_node()
 
 template <int FORM>
 __global__ void __launch_bounds__(TILE)
-km_volume(const Tile *__restrict__ tiles, int ntiles, double *__restrict__ Q, const double *__restrict__ U,
-          const int *__restrict__ N3, const long *__restrict__ off, const int *__restrict__ efoff,
+km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, const double *__restrict__ U,
+          const int *__restrict__ efoff,
           const double *__restrict__ Ja, const double *__restrict__ Jac, const double *__restrict__ edelta,
           const double *__restrict__ fe, const OpsAll *__restrict__ opsp, double *__restrict__ G,
           double *__restrict__ qdot_out, double a_k, double c_k, int update, DtArgs dta, long S, long ES,
@@ -617,18 +630,18 @@ km_volume(const Tile *__restrict__ tiles, int ntiles, double *__restrict__ Q, co
     for (int x = t; x < 2 * MAXN1 * 64; x += blockDim.x)
         sOp[x] = x < MAXN1 * 64 ? ops.hatD[x / 64][x % 64] : ops.sharpD[(x / 64) - MAXN1][x % 64];
     double *chunk0 = smem + 2 * MAXN1 * 64;
+    NodeMap nm = load_map(tmap, blockIdx.x < ntiles ? blockIdx.x : 0, t);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const Tile &T = tiles[tile];
-    const int slot = tile_slot(T, t);
-    const int e = T.elem[slot];
-    const int Nx = N3[3 * e], Ny = N3[3 * e + 1], Nz = N3[3 * e + 2];
+    const int e = nm.e, code = nm.code;
+    const long g = nm.g;
+    const int i = code & 7, j = (code >> 3) & 7, k = (code >> 6) & 7;
+    const int Nx = (code >> 9) & 7, Ny = (code >> 12) & 7, Nz = (code >> 15) & 7;
     const int nx = Nx + 1, ny = Ny + 1, nz = Nz + 1, ne = nx * ny * nz;
-    const bool active = t < T.off[T.count];
-    const int n = active ? t - T.off[slot] : 0;
-    const int i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
-    const long g = off[e] + n;
+    const int first = (code >> 18) & 511;
+    const bool active = (code >> 27) & 1;
+    const int n = i + nx * (j + ny * k);
     const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
-    double *sF = chunk0 + PER_NODE * T.off[slot];   // [3][NF][ne]
+    double *sF = chunk0 + PER_NODE * first;   // [3][NF][ne]
     double *sN = sF + 3 * NF * ne;                  // [ND][ne]
     double *sJ = sN + (SPLIT ? ND * ne : 0);        // [3][ne]
     // loads, unconditional (see hxk::k_volume)
@@ -656,6 +669,7 @@ km_volume(const Tile *__restrict__ tiles, int ntiles, double *__restrict__ Q, co
         for (int v = 0; v < NV; ++v) gq[d][v] = U[(d * NV + v) * S + g];
     const double dl = edelta[e];
     const double jg = Jac[g];
+    if (tile + (int)gridDim.x < ntiles) nm = load_map(tmap, tile + gridDim.x, t);   // the next tile's
     if (active) {
         double Fvc[3][NV];
         if (!ph.euler) {
