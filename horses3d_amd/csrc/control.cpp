@@ -313,11 +313,13 @@ int fill_case(hx3d_control *c, const char *root, hx3d_mesh **mesh, hx3d_case_des
     desc.euler = lower(str("flow equations")) == "euler";
     if (!desc.euler && (rc = need_real(c, "reynolds number", desc.reynolds))) return rc;
 
-    // discretization nodes: the names are matched as written
+    // discretization nodes: the names are matched as written (InitializeNodalStorage_controlVars,
+    // NodalStorageClass.f90:80-105)
     const std::string nodes = str("discretization nodes");
-    if (nodes == "Gauss")
-        return fail(c, HX3D_EUNSUPPORTED, "Gauss nodes: the engine runs on Gauss-Lobatto nodes");
-    if (nodes != "Gauss-Lobatto") return fail(c, HX3D_EINVAL, "Unknown discretization nodes \"" + nodes + "\"");
+    if (nodes == "Gauss") desc.nodes = HX3D_NODES_GAUSS;
+    else if (nodes == "Gauss-Lobatto") desc.nodes = HX3D_NODES_GAUSSLOBATTO;
+    else return fail(c, HX3D_EINVAL, "Unknown discretization nodes \"" + nodes + "\"");
+    const bool gauss = desc.nodes == HX3D_NODES_GAUSS;
 
     const std::string inv = lower(str("inviscid discretization"));
     if (inv == "standard") desc.inviscid = HX3D_INVISCID_STANDARD;
@@ -451,11 +453,11 @@ int fill_case(hx3d_control *c, const char *root, hx3d_mesh **mesh, hx3d_case_des
     const std::string mfile = path_of(str("mesh file name"));
     hx3d_mesh *m = nullptr;
     if (Nuni == -1)
-        rc = hx3d_front::read_specmesh(&m, mfile.c_str(), 0, orders.data(), hx3d_front::ORDERS_ONE_ROW, resolve);
+        rc = hx3d_front::read_specmesh(&m, mfile.c_str(), 0, orders.data(), hx3d_front::ORDERS_ONE_ROW, resolve, gauss);
     else if (!orders.empty())
-        rc = hx3d_front::read_specmesh(&m, mfile.c_str(), 0, orders.data(), (long)orders.size() / 3, resolve);
+        rc = hx3d_front::read_specmesh(&m, mfile.c_str(), 0, orders.data(), (long)orders.size() / 3, resolve, gauss);
     else
-        rc = hx3d_front::read_specmesh(&m, mfile.c_str(), Nuni, nullptr, hx3d_front::ORDERS_UNCHECKED, resolve);
+        rc = hx3d_front::read_specmesh(&m, mfile.c_str(), Nuni, nullptr, hx3d_front::ORDERS_UNCHECKED, resolve, gauss);
     *mesh = m;
     if (rc) return fail(c, rc, m ? hx3d_mesh_error(m) : "out of host memory");
     hx3d_mesh_fill_desc(m, &desc);

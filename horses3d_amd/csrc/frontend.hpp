@@ -19,8 +19,9 @@ constexpr long ORDERS_UNCHECKED = -1;   // one row per element, count not known 
 constexpr long ORDERS_ONE_ROW = -2;     // a single row that applies to every element
 
 // Reads a SpecMesh file (ConstructMesh_FromSpecMeshFile_, Read_SpecMesh.f90:36-348) with the
-// uniform order N, or the per-element orders when orders is given.
+// uniform order N, or the per-element orders when orders is given; the geometry on Gauss nodes
+// (gauss) or Gauss-Lobatto nodes.
 int read_specmesh(hx3d_mesh **out, const char *path, int N, const int *orders, long norders,
-                  const ZoneResolver &resolve);
+                  const ZoneResolver &resolve, bool gauss = false);
 
 }  // namespace hx3d_front

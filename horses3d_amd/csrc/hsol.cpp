@@ -4,7 +4,7 @@
 //
 //   byte   0  file name (128 characters, the name's own length, zero padded)    POS_FILENAME
 //        128  int32 file type (2: solution, 3: solution and gradients)          POS_FILETYPE
-//        132  int32 node type (2: Gauss-Lobatto)                                POS_NODETYPE
+//        132  int32 node type (1: Gauss, 2: Gauss-Lobatto)                      POS_NODETYPE
 //        136  int32 number of elements                                          POS_NOOFELEMENTS
 //        140  int32 iteration                                                   POS_ITER
 //        144  float64 time                                                      POS_TIME
@@ -34,7 +34,7 @@ int fail(int code, const std::string &m)
     return code;
 }
 
-constexpr int SOLUTION_FILE = 2, SOLUTION_AND_GRADIENTS_FILE = 3, GAUSSLOBATTO = 2;
+constexpr int SOLUTION_FILE = 2, SOLUTION_AND_GRADIENTS_FILE = 3, GAUSS = 1, GAUSSLOBATTO = 2;
 constexpr int BEGINNING_DATA = 88, END_OF_FILE = 99, NAME_LEN = 128, POS_INIT_DATA = 204;
 
 struct File {
@@ -84,7 +84,16 @@ int hx3d_hsol_reference_values(double mach, double refs[6])
 int hx3d_hsol_write(const char *path, const char *header_name, int K, const int *N, int iter, double time,
                     const double refs[6], const double *Q, const double *grad)
 {
+    return hx3d_hsol_write_nodes(path, header_name, K, N, iter, time, refs, Q, grad, HX3D_NODES_GAUSSLOBATTO);
+}
+
+int hx3d_hsol_write_nodes(const char *path, const char *header_name, int K, const int *N, int iter, double time,
+                          const double refs[6], const double *Q, const double *grad, int node_kind)
+{
     if (!path || !N || !Q || !refs || K <= 0) return fail(HX3D_EINVAL, "hx3d_hsol_write: missing argument");
+    if (node_kind != HX3D_NODES_GAUSSLOBATTO && node_kind != HX3D_NODES_GAUSS) return fail(HX3D_EINVAL, "unknown node type");
+    // the file's node type field: the reference's GAUSS = 1, GAUSSLOBATTO = 2 (NodalStorageClass.f90:16-17)
+    const int32_t node_type = node_kind == HX3D_NODES_GAUSS ? GAUSS : GAUSSLOBATTO;
     long nodes = 0;
     for (int e = 0; e < K; ++e) {
         if (N[3 * e] < 0 || N[3 * e + 1] < 0 || N[3 * e + 2] < 0) return fail(HX3D_EINVAL, "negative order");
@@ -98,7 +107,7 @@ int hx3d_hsol_write(const char *path, const char *header_name, int K, const int 
     std::memcpy(name, nm.data(), std::min<size_t>(nm.size(), NAME_LEN));
     bool ok = std::fwrite(name, 1, NAME_LEN, f.f) == (size_t)NAME_LEN;
     const int32_t type = grad ? SOLUTION_AND_GRADIENTS_FILE : SOLUTION_FILE;
-    ok = ok && put(f.f, type) && put(f.f, (int32_t)GAUSSLOBATTO) && put(f.f, (int32_t)K) && put(f.f, (int32_t)iter);
+    ok = ok && put(f.f, type) && put(f.f, node_type) && put(f.f, (int32_t)K) && put(f.f, (int32_t)iter);
     ok = ok && put(f.f, time);
     for (int r = 0; r < 6; ++r) ok = ok && put(f.f, refs[r]);
     ok = ok && put(f.f, (int32_t)BEGINNING_DATA);
@@ -153,7 +162,9 @@ int hx3d_hsol_read(const char *path, int K, const int *N, double *Q, double *gra
     if (type != SOLUTION_FILE && type != SOLUTION_AND_GRADIENTS_FILE)
         return fail(HX3D_EUNSUPPORTED, "solution file type " + std::to_string(type) + " is not a solution file");
     if (grad && type != SOLUTION_AND_GRADIENTS_FILE) return fail(HX3D_EINVAL, "solution file holds no gradients");
-    if (nt != GAUSSLOBATTO) return fail(HX3D_EUNSUPPORTED, "solution file is not on Gauss-Lobatto nodes");
+    // a file of the other node type is loaded as it is, without interpolation, as HexMesh_LoadSolution
+    // does (HexMesh.f90:4798-4806, with a warning there); the caller sees the type in the header
+    if (nt != GAUSSLOBATTO && nt != GAUSS) return fail(HX3D_EUNSUPPORTED, "solution file: unknown node type " + std::to_string(nt));
     long nodes = 0;
     for (int e = 0; e < K; ++e) nodes += (long)(N[3 * e] + 1) * (N[3 * e + 1] + 1) * (N[3 * e + 2] + 1);
     File f(std::fopen(path, "rb"));

@@ -379,12 +379,12 @@ __global__ void k_fill_int(int *__restrict__ X, int n, int val)
 // host side: spectral operators (restated from NodalStorageClass.f90 for the engine)
 // ------------------------------------------------------------------------------------
 
-Ops make_ops(int N)
+Ops make_ops(int N, bool gauss = false)
 {
     Ops o;
     std::memset(&o, 0, sizeof(o));
     double x[8], w[8], wb[8];
-    gl_nodes(N, x, w);
+    hxs::nodes_of(gauss, N, x, w);
     for (int j = 0; j <= N; ++j) wb[j] = 1.0;
     for (int j = 1; j <= N; ++j)
         for (int k = 0; k < j; ++k) { wb[k] *= (x[k] - x[j]); wb[j] *= (x[j] - x[k]); }
@@ -403,6 +403,7 @@ Ops make_ops(int N)
         }
     o.sharpD[0] = 2.0 * D[0][0] + 1.0 / w[0];
     o.sharpD[N * 8 + N] = 2.0 * D[N][N] - 1.0 / w[N];
+    if (gauss) std::memset(o.sharpD, 0, sizeof(o.sharpD));   // split forms run on Gauss-Lobatto nodes
     for (int i = 0; i <= N; ++i) o.w[i] = w[i];
     return o;
 }
@@ -462,6 +463,7 @@ struct hx3d_engine {
     int4 *dFnode = nullptr;
     // mixed orders (p-anisotropic elements, mortar faces)
     bool mixed = false;
+    bool gauss = false;   // Gauss nodes (always on the face-trace kernels of the mixed path)
     long FSm = 0;
     int *dN3 = nullptr, *dEfoff = nullptr, *dEofn = nullptr;
     long *dOff = nullptr;
@@ -664,34 +666,37 @@ template <int FORM>
 size_t mvolume_lds() { return sizeof(double) * (2 * hxm::MAXN1 * 64 + (size_t)hxm::mvolume_per_node<FORM>() * hxm::TILE); }
 size_t mgrad_lds() { return sizeof(double) * (hxm::MAXN1 * 64 + (size_t)NV * hxm::TILE); }
 
-template <int FORM>
+template <int FORM, bool GAUSS = false>
 void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
-    hipLaunchKernelGGL((hxm::km_volume<FORM>), dim3(eng->grid_mvol), dim3(hxm::TILE), mvolume_lds<FORM>(), eng->stream,
-                       eng->dTmap, eng->ntiles, eng->dQ, eng->dU, eng->dEfoff, eng->dJa, eng->dJ,
+    hipLaunchKernelGGL((hxm::km_volume<FORM, GAUSS>), dim3(eng->grid_mvol), dim3(hxm::TILE), mvolume_lds<FORM>(),
+                       eng->stream, eng->dTmap, eng->ntiles, eng->dQ, eng->dU, eng->dEfoff, eng->dJa, eng->dJ,
                        eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG, qdot, a_k, c_k, update, dta, eng->S, eng->ES,
                        eng->ph);
 }
 
-void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
-                 hipEvent_t *ev)
+// One stage of the mixed path; GAUSS: Gauss nodes (standard form only)
+template <bool GAUSS>
+void stage_mixed_nodes(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
+                       hipEvent_t *ev)
 {
     if (ev) (void)hipEventRecord(ev[0], eng->stream);
     hxk::DtLimits lim = dta.lim;
     if (!eng->ph.euler) {
-        hipLaunchKernelGGL(hxm::km_face_grad, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dN3,
-                           eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift, eng->F,
-                           eng->S, eng->ES, eng->FSm);
-        hipLaunchKernelGGL(hxm::km_grad, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream, eng->dTmap,
-                           eng->ntiles, eng->dQ, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
+        hipLaunchKernelGGL(hxm::km_face_grad<GAUSS>, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ,
+                           eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift,
+                           eng->F, eng->S, eng->ES, eng->FSm);
+        hipLaunchKernelGGL(hxm::km_grad<GAUSS>, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream,
+                           eng->dTmap, eng->ntiles, eng->dQ, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
                            eng->dOpsAll, eng->dU, eng->S, eng->ES, eng->ph, lim);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
-    hipLaunchKernelGGL(hxm::km_face, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dU, eng->dN3, eng->dOff,
-                       eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll, eng->dFe, eng->F, eng->S,
-                       eng->ES, eng->FSm, eng->ph, dta.red);
+    hipLaunchKernelGGL(hxm::km_face<GAUSS>, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dU,
+                       eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll,
+                       eng->dFe, eng->F, eng->S, eng->ES, eng->FSm, eng->ph, dta.red);
     if (ev) (void)hipEventRecord(ev[2], eng->stream);
-    if (eng->inviscid != HX3D_INVISCID_SPLIT) launch_mvolume<hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
+    if (GAUSS) launch_mvolume<hxk::FORM_STANDARD, true>(eng, a_k, c_k, update, dta, qdot);
+    else if (eng->inviscid != HX3D_INVISCID_SPLIT) launch_mvolume<hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
     else switch (eng->ph.averaging) {
         case HX3D_AVG_DUCROS: launch_mvolume<hxk::AVG_DUCROS>(eng, a_k, c_k, update, dta, qdot); break;
         case HX3D_AVG_KENNEDYGRUBER: launch_mvolume<hxk::AVG_KG>(eng, a_k, c_k, update, dta, qdot); break;
@@ -700,6 +705,13 @@ void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtA
         default: launch_mvolume<hxk::AVG_STANDARD>(eng, a_k, c_k, update, dta, qdot); break;
         }
     if (ev) (void)hipEventRecord(ev[3], eng->stream);
+}
+
+void stage_mixed(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
+                 hipEvent_t *ev)
+{
+    if (eng->gauss) stage_mixed_nodes<true>(eng, a_k, c_k, update, dta, qdot, ev);
+    else stage_mixed_nodes<false>(eng, a_k, c_k, update, dta, qdot, ev);
 }
 
 // MaxTimeStep limits of the resident state into the dt slot bits[0..1]
@@ -868,19 +880,21 @@ int set_tiles(hx3d_engine *eng, int policy)
     int dev = 0, cus = 0, bg = 0, bv = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "no HIP device");
-    hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bg, reinterpret_cast<const void *>(&hxm::km_grad),
-                                                                 hxm::TILE, mgrad_lds());
+    hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &bg, eng->gauss ? reinterpret_cast<const void *>(&hxm::km_grad<true>) : reinterpret_cast<const void *>(&hxm::km_grad<false>),
+        hxm::TILE, mgrad_lds());
     const bool split = eng->inviscid == HX3D_INVISCID_SPLIT;
     const int avg = eng->ph.averaging;
     auto occ = [&](const void *k, size_t lds) {
         if (oe == hipSuccess) oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k, hxm::TILE, lds);
     };
-    if (!split) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::FORM_STANDARD>), mvolume_lds<hxk::FORM_STANDARD>());
-    else if (avg == HX3D_AVG_DUCROS) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_DUCROS>), mvolume_lds<hxk::AVG_DUCROS>());
-    else if (avg == HX3D_AVG_KENNEDYGRUBER) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_KG>), mvolume_lds<hxk::AVG_KG>());
-    else if (avg == HX3D_AVG_PIROZZOLI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_PIROZZOLI>), mvolume_lds<hxk::AVG_PIROZZOLI>());
-    else if (avg == HX3D_AVG_MORINISHI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_MORINISHI>), mvolume_lds<hxk::AVG_MORINISHI>());
-    else occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_STANDARD>), mvolume_lds<hxk::AVG_STANDARD>());
+    if (eng->gauss) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::FORM_STANDARD, true>), mvolume_lds<hxk::FORM_STANDARD>());
+    else if (!split) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::FORM_STANDARD, false>), mvolume_lds<hxk::FORM_STANDARD>());
+    else if (avg == HX3D_AVG_DUCROS) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_DUCROS, false>), mvolume_lds<hxk::AVG_DUCROS>());
+    else if (avg == HX3D_AVG_KENNEDYGRUBER) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_KG, false>), mvolume_lds<hxk::AVG_KG>());
+    else if (avg == HX3D_AVG_PIROZZOLI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_PIROZZOLI, false>), mvolume_lds<hxk::AVG_PIROZZOLI>());
+    else if (avg == HX3D_AVG_MORINISHI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_MORINISHI, false>), mvolume_lds<hxk::AVG_MORINISHI>());
+    else occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_STANDARD, false>), mvolume_lds<hxk::AVG_STANDARD>());
     if (oe != hipSuccess) return eng->fail(HX3D_EDEVICE, std::string("occupancy query: ") + hipGetErrorString(oe));
     eng->grid_mgrad = std::max(1, std::min(eng->ntiles, std::max(1, bg) * cus));
     eng->grid_mvol = std::max(1, std::min(eng->ntiles, std::max(1, bv) * cus));
@@ -890,6 +904,8 @@ int set_tiles(hx3d_engine *eng, int policy)
 int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
 {
     eng->mixed = true;
+    const bool gauss = d->nodes == HX3D_NODES_GAUSS;
+    eng->gauss = gauss;
     const int K = d->K, F = d->F;
     eng->K = K; eng->F = F;
     if (!(d->fNormal && d->fT1 && d->fT2 && d->fJac))
@@ -946,15 +962,25 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
     std::vector<double> wq((size_t)S, 0.0);
     double xn[8][8], wn[8][8];
     for (int n = 1; n <= MAXN; ++n) {
-        const Ops o = make_ops(n);
+        const Ops o = make_ops(n, gauss);
         std::memcpy(oa.D[n], o.D, sizeof(o.D));
         std::memcpy(oa.hatD[n], o.hatD, sizeof(o.hatD));
         std::memcpy(oa.sharpD[n], o.sharpD, sizeof(o.sharpD));
-        gl_nodes(n, xn[n], wn[n]);
+        hxs::nodes_of(gauss, n, xn[n], wn[n]);
         oa.b0[n] = 1.0 / wn[n][0];
         oa.dcsi[n] = 1.0 / std::fabs(xn[n][1] - xn[n][0]);
+        // boundary values v(-1), v(+1) of the Lagrange basis and b = v / w (NodalStorageClass.f90:
+        // 257-273): one-hot on Gauss-Lobatto nodes, dense on Gauss nodes
+        double wb[8];
+        hxs::bary(n, xn[n], wb);
+        hxs::interp_vector(-1.0, n, xn[n], wb, oa.vL[n]);
+        hxs::interp_vector(1.0, n, xn[n], wb, oa.vR[n]);
+        for (int i = 0; i <= n; ++i) {
+            oa.bL[n][i] = oa.vL[n][i] / wn[n][i];
+            oa.bR[n][i] = oa.vR[n][i] / wn[n][i];
+        }
         for (int m = 1; m <= MAXN; ++m)
-            if (m != n) tset_host(n, m, oa.T[n][m]);
+            if (m != n) tset_host(n, m, oa.T[n][m], gauss);
     }
     // geometry, weights, LES filter widths (SD.f90:420, :1378)
     std::vector<double> ja((size_t)9 * S, 0.0), jj((size_t)S, 1.0), edelta(K), fdelta(F),
@@ -1093,12 +1119,22 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         return bail(eng->fail(HX3D_EINVAL, "descriptor: selector out of range"));
     if (d->mach <= 0 || (!d->euler && d->reynolds <= 0))
         return bail(eng->fail(HX3D_EINVAL, "descriptor: mach and reynolds must be positive"));
+    if (d->nodes != HX3D_NODES_GAUSSLOBATTO && d->nodes != HX3D_NODES_GAUSS)
+        return bail(eng->fail(HX3D_EINVAL, "descriptor: unknown node type"));
+    if (d->nodes == HX3D_NODES_GAUSS && d->inviscid == HX3D_INVISCID_SPLIT)
+        return bail(eng->fail(HX3D_EINVAL, "split-form discretizations run on Gauss-Lobatto nodes (the reference "
+                                           "switches a split-form case to them)"));
+    if (d->nodes == HX3D_NODES_GAUSS && !(d->fNormal && d->fT1 && d->fT2 && d->fJac))
+        return bail(eng->fail(HX3D_EINVAL, "Gauss nodes: the descriptor must give the face geometry (fNormal, fT1, "
+                                           "fT2, fJac): no element node lies on a face"));
     set_physics(eng, d);
     eng->N3h.assign(d->N, d->N + 3L * d->K);
     eng->mach = d->mach;
     // descriptor checks come first (they need no device); the device is recorded from here on
     if (hipGetDevice(&eng->device) != hipSuccess) return bail(eng->fail(HX3D_EDEVICE, "no HIP device"));
-    if (!uniform) return bail(create_mixed(eng, d));
+    // per-element orders, and Gauss nodes (whose face values are interpolated, not copied), take
+    // the face-trace kernels
+    if (!uniform || d->nodes == HX3D_NODES_GAUSS) return bail(create_mixed(eng, d));
     eng->K = d->K; eng->N = N; eng->n1 = N + 1; eng->nf = eng->n1 * eng->n1; eng->ne = eng->nf * eng->n1;
     eng->F = d->F;
     eng->nodes = (long)eng->K * eng->ne;
@@ -1287,6 +1323,8 @@ int hx3d_engine_create_multi(const hx3d_case_desc *descs, int ncases, hx3d_engin
             return reject(HX3D_EUNSUPPORTED, "several cases in one engine need the same discretisation (Riemann solver, "
                                              "volume form, averaging, LES model, lambda)");
         if (d.euler) return reject(HX3D_EUNSUPPORTED, "several cases in one engine: Navier-Stokes cases only");
+        if (d.nodes != HX3D_NODES_GAUSSLOBATTO)
+            return reject(HX3D_EUNSUPPORTED, "several cases in one engine: Gauss-Lobatto nodes only");
         if (fg != (d.fNormal && d.fT1 && d.fT2 && d.fJac))
             return reject(HX3D_EINVAL, "give the face geometry for every case or for none");
         if (d.mach <= 0 || d.reynolds <= 0) return reject(HX3D_EINVAL, "case " + std::to_string(c) + ": mach and reynolds must be positive");
@@ -1484,7 +1522,8 @@ int hx3d_save_solution(hx3d_engine *eng, const char *path, const char *header_na
     if (rc) return rc;
     double refs[6];
     hx3d_hsol_reference_values(eng->mach, refs);
-    rc = hx3d_hsol_write(path, header_name, eng->K, eng->N3h.data(), iter, time, refs, Q.data(), nullptr);
+    rc = hx3d_hsol_write_nodes(path, header_name, eng->K, eng->N3h.data(), iter, time, refs, Q.data(), nullptr,
+                               eng->gauss ? HX3D_NODES_GAUSS : HX3D_NODES_GAUSSLOBATTO);
     if (rc) return eng->fail(rc, hx3d_hsol_error());
     return HX3D_OK;
 }

@@ -36,6 +36,9 @@ struct OpsAll {
     double b0[MAXN1];     // 1/w_0 = 1/w_N, the GL surface weight
     double dcsi[MAXN1];   // 1/|x_1 - x_0|, MaxTimeStep (DGSEMClass.f90:948-959)
     double T[MAXN1][MAXN1][64];
+    // the Lagrange basis at -1 and +1 and b = v / w (NodalStorageClass.f90:257-273): the face
+    // values and surface weights of Gauss nodes, on which no node lies on a face
+    double vL[MAXN1][8], vR[MAXN1][8], bL[MAXN1][8], bR[MAXN1][8];
 };
 
 // Face record: elements, slots, orders, rotation and projection types (Face_LinkWithElements,
@@ -290,10 +293,15 @@ __device__ __forceinline__ void project_face(const double (*m)[64], const FaceRe
 }
 
 // Stage the side's element trace of K node arrays src[c*S + g] into tr[c*64 + q] in the face
-// orientation (the right side is rotated, Face_AdaptSolutionToFace).  One wave, lane = q.
-template <int K>
+// orientation (the right side is rotated, Face_AdaptSolutionToFace).  One wave, lane = q.  On
+// Gauss-Lobatto nodes the trace is the face's row of nodes; on Gauss nodes (GAUSS) it is the
+// Lagrange basis at the face's end of the normal direction applied along it,
+// acc = acc + value * v(l) for l = 0..N (HexElement_ProlongSolutionToFaces,
+// HexElementClass.f90:233-302, and _ProlongGradientsToFaces, :304-372).
+template <int K, bool GAUSS = false>
 __device__ __forceinline__ void gather_trace_k(const FaceRec &r, int side, const int *N3, const long *off,
-                                               const double *src, long S, double *tr, int lane)
+                                               const double *src, long S, double *tr, int lane,
+                                               const OpsAll *ops = nullptr)
 {
     const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
     const int nq = (N0 + 1) * (N1 + 1);
@@ -302,8 +310,24 @@ __device__ __forceinline__ void gather_trace_k(const FaceRec &r, int side, const
         int a = i, b = j;
         if (side == 1) left_to_right_d(i, j, N0, N1, r.rot, a, b);
         const long g = slot_node(N3, off, r.e[side], r.slot[side], a, b);
+        if (!GAUSS) {
 #pragma unroll
-        for (int c = 0; c < K; ++c) tr[c * 64 + lane] = src[c * S + g];
+            for (int c = 0; c < K; ++c) tr[c * 64 + lane] = src[c * S + g];
+        } else {
+            // g is the node at the face's end of the normal axis (index 0 or N); walk the axis
+            const int s = r.slot[side], ax = M_SLOT_AXIS[s];
+            const int *Ne = &N3[3 * r.e[side]];
+            const int Na = Ne[ax];
+            const long stride = ax == 0 ? 1 : (ax == 1 ? (Ne[0] + 1) : (long)(Ne[0] + 1) * (Ne[1] + 1));
+            const long g0 = M_SLOT_PLUS[s] ? g - (long)Na * stride : g;
+            const double *vv = M_SLOT_PLUS[s] ? ops->vR[Na] : ops->vL[Na];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                double acc = 0.0;
+                for (int l = 0; l <= Na; ++l) acc = acc + src[c * S + g0 + l * stride] * vv[l];
+                tr[c * 64 + lane] = acc;
+            }
+        }
     }
 }
 
@@ -320,6 +344,7 @@ __device__ __forceinline__ long elem_face_node(const FaceRec &r, int side, const
 // (BR1_ComputeElementInterfaceAverage, EllipticBR1.f90:571-627), projected onto both sides:
 // lift[(3v + d)... laid out as lift[(d*5 + v)*ES + x].
 // ------------------------------------------------------------------------------------
+template <bool GAUSS>
 __global__ void __launch_bounds__(64)
 km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const long *__restrict__ off,
              const int *__restrict__ efoff, const FaceRec *__restrict__ faces, const double *__restrict__ fgeo,
@@ -337,8 +362,8 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
-    gather_trace_k<NV>(r, 0, N3, off, Q, S, sT[0], lane);
-    gather_trace_k<NV>(r, 1, N3, off, Q, S, sT[1], lane);
+    gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, sT[0], lane, opsp);
+    gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, sT[1], lane, opsp);
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
     const bool on = lane < nq;
     const int i = on ? lane % (n0 + 1) : 0, j = on ? lane / (n0 + 1) : 0;
@@ -382,6 +407,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
 // BR1 lifting (BR1_GradientFaceLoop, EllipticBR1.f90:531-569, with the weights b of
 // VectorWeakIntegrals_StdFace): grad Q at every node, U[(5d + v)*S + g].
 // ------------------------------------------------------------------------------------
+template <bool GAUSS>
 __global__ void __launch_bounds__(TILE)
 km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__ Q,
         const int *__restrict__ efoff, const double *__restrict__ Ja,
@@ -443,30 +469,60 @@ km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__
 #pragma unroll
         for (int v = 0; v < NV; ++v) uzeta[v] = uzeta[v] + sQ[v * ne + i + nx * (j + ny * l)] * dk;
     }
-    const double bx = ops.b0[Nx], by = ops.b0[Ny], bz = ops.b0[Nz];
-    // lifting values of the node's faces (slot order 5/3, 0/1, 2/4); absent faces give 0.
-    // Loaded after the derivatives, so that they do not hold registers through them.
-    double hx_[3 * NV], hy_[3 * NV], hz_[3 * NV];
+    if (GAUSS) {
+        // every node takes the lifting values of all six faces with its surface weights b,
+        // in the order L, R (xi), FR, BK (eta), BOT, T (zeta) of VectorWeakIntegrals_StdFace
+        // (DGIntegrals.f90:365-443); slots 5, 3, 0, 1, 2, 4
+        const int *ef = &efoff[6 * e];
+        const long xl = ef[5] + j + ny * k, xr = ef[3] + j + ny * k;
+        const long xf = ef[0] + i + nx * k, xb = ef[1] + i + nx * k;
+        const long xo = ef[2] + i + nx * j, xt = ef[4] + i + nx * j;
+        const double bl = ops.bL[Nx][i], br = ops.bR[Nx][i], bf = ops.bL[Ny][j], bb = ops.bR[Ny][j];
+        const double bo = ops.bL[Nz][k], bt = ops.bR[Nz][k];
 #pragma unroll
-    for (int c = 0; c < 3 * NV; ++c) {
-        const double a = lift[c * ES + xx], b = lift[c * ES + xy], d = lift[c * ES + xz];
-        hx_[c] = onx ? a : 0.0;
-        hy_[c] = ony ? b : 0.0;
-        hz_[c] = onz ? d : 0.0;
-    }
+        for (int v = 0; v < NV; ++v) {
+            double gr[3];
+            gr[0] = (uxi[v] * ja[0] + ueta[v] * ja[3] + uzeta[v] * ja[6]) * iJ;
+            gr[1] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
+            gr[2] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double gr[3];
-        gr[0] = (uxi[v] * ja[0] + ueta[v] * ja[3] + uzeta[v] * ja[6]) * iJ;
-        gr[1] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
-        gr[2] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
+            for (int d = 0; d < 3; ++d) {
+                const double *lc = lift + (long)(d * NV + v) * ES;
+                double fi = lc[xl] * bl;
+                fi = fi + lc[xr] * br;
+                fi = fi + lc[xf] * bf;
+                fi = fi + lc[xb] * bb;
+                fi = fi + lc[xo] * bo;
+                fi = fi + lc[xt] * bt;
+                U[(d * NV + v) * S + g] = gr[d] + fi * iJ;
+            }
+        }
+    } else {
+        const double bx = ops.b0[Nx], by = ops.b0[Ny], bz = ops.b0[Nz];
+        // lifting values of the node's faces (slot order 5/3, 0/1, 2/4); absent faces give 0.
+        // Loaded after the derivatives, so that they do not hold registers through them.
+        double hx_[3 * NV], hy_[3 * NV], hz_[3 * NV];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double ax = onx ? hx_[d * NV + v] * bx : 0.0;
-            const double ay = ony ? hy_[d * NV + v] * by : 0.0;
-            const double az = onz ? hz_[d * NV + v] * bz : 0.0;
-            const double fi = ((0.0 + ax) + ay) + az;
-            U[(d * NV + v) * S + g] = (onx || ony || onz) ? gr[d] + fi * iJ : gr[d];
+        for (int c = 0; c < 3 * NV; ++c) {
+            const double a = lift[c * ES + xx], b = lift[c * ES + xy], d = lift[c * ES + xz];
+            hx_[c] = onx ? a : 0.0;
+            hy_[c] = ony ? b : 0.0;
+            hz_[c] = onz ? d : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double gr[3];
+            gr[0] = (uxi[v] * ja[0] + ueta[v] * ja[3] + uzeta[v] * ja[6]) * iJ;
+            gr[1] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
+            gr[2] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double ax = onx ? hx_[d * NV + v] * bx : 0.0;
+                const double ay = ony ? hy_[d * NV + v] * by : 0.0;
+                const double az = onz ? hz_[d * NV + v] * bz : 0.0;
+                const double fi = ((0.0 + ax) + ay) + az;
+                U[(d * NV + v) * S + g] = (onx || ony || onz) ? gr[d] + fi * iJ : gr[d];
+            }
         }
     }
     if (dtl.part) {
@@ -487,6 +543,7 @@ km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__
 // (FaceClass.f90:597-700) and scaled by the receiving element's surface weight:
 // fe[v*ES + x] = (+/-) projected flux * b.
 // ------------------------------------------------------------------------------------
+template <bool GAUSS>
 __global__ void __launch_bounds__(64)
 km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *__restrict__ N3,
         const long *__restrict__ off, const int *__restrict__ efoff, const FaceRec *__restrict__ faces,
@@ -505,9 +562,9 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
-    gather_trace_k<NV>(r, 0, N3, off, Q, S, sQ[0], lane);
-    gather_trace_k<NV>(r, 1, N3, off, Q, S, sQ[1], lane);
-    if (!ph.euler) gather_trace_k<3 * NV>(r, 0, N3, off, U, S, sG, lane);
+    gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, sQ[0], lane, opsp);
+    gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, sQ[1], lane, opsp);
+    if (!ph.euler) gather_trace_k<3 * NV, GAUSS>(r, 0, N3, off, U, S, sG, lane, opsp);
     __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
     // lanes past the face's nodes work on node (0, 0) and discard the result (every index stays
@@ -548,7 +605,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         side_grad(0, g);
         if (on) side_flux(0, g, Fs[0]);
         __syncthreads();   // side 0's gradients consumed
-        gather_trace_k<3 * NV>(r, 1, N3, off, U, S, sG, lane);
+        gather_trace_k<3 * NV, GAUSS>(r, 1, N3, off, U, S, sG, lane, opsp);
         __syncthreads();
         side_grad(1, g);
         if (on) {
@@ -586,10 +643,16 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         project_face<NV, false>(sM[side], r, side, sF, sTmp, lane, si, sj, son, a);
         if (son) {
             const long x = elem_face_node(r, side, efoff, si, sj);
-            const int e = r.e[side], s = r.slot[side];
-            const double b = opsp->b0[N3[3 * e + M_SLOT_AXIS[s]]];
+            if (GAUSS) {
+                // the surface weights b(i) of every node are applied by the volume kernel
 #pragma unroll
-            for (int v = 0; v < NV; ++v) fe[v * ES + x] = (side == 0 ? a[v] : -1.0 * a[v]) * b;
+                for (int v = 0; v < NV; ++v) fe[v * ES + x] = side == 0 ? a[v] : -1.0 * a[v];
+            } else {
+                const int e = r.e[side], s = r.slot[side];
+                const double b = opsp->b0[N3[3 * e + M_SLOT_AXIS[s]]];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fe[v * ES + x] = (side == 0 ? a[v] : -1.0 * a[v]) * b;
+            }
         }
     }
 }
@@ -606,7 +669,7 @@ constexpr int mvolume_per_node()
     return 3 * (FORM >= 0 ? 4 : 5) + (FORM >= 0 ? hx::node_data_size(FORM) + 3 : 0);
 }
 
-template <int FORM>
+template <int FORM, bool GAUSS>
 __global__ void __launch_bounds__(TILE)
 km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, const double *__restrict__ U,
           const int *__restrict__ efoff,
@@ -646,7 +709,30 @@ km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, 
     double *sJ = sN + (SPLIT ? ND * ne : 0);        // [3][ne]
     // loads, unconditional (see hxk::k_volume)
     double sx[NV], sy[NV], sz[NV];
-    {
+    if (GAUSS) {
+        // surface integral on every node: the face fluxes of all six faces with the node's
+        // surface weights, L, R, FR, BK, BOT, T (ScalarWeakIntegrals_StdFace, DGIntegrals.f90:
+        // 214-273), accumulated in sx (sy, sz unused)
+        const int *ef = &efoff[6 * e];
+        const long xl = ef[5] + j + ny * k, xr = ef[3] + j + ny * k;
+        const long xf = ef[0] + i + nx * k, xb = ef[1] + i + nx * k;
+        const long xo = ef[2] + i + nx * j, xt = ef[4] + i + nx * j;
+        const double bl = ops.bL[Nx][i], br = ops.bR[Nx][i], bf = ops.bL[Ny][j], bb = ops.bR[Ny][j];
+        const double bo = ops.bL[Nz][k], bt = ops.bR[Nz][k];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double *fv = fe + (long)v * ES;
+            double x = fv[xl] * bl;
+            x = x + fv[xr] * br;
+            x = x + fv[xf] * bf;
+            x = x + fv[xb] * bb;
+            x = x + fv[xo] * bo;
+            x = x + fv[xt] * bt;
+            sx[v] = x;
+            sy[v] = 0.0;
+            sz[v] = 0.0;
+        }
+    } else {
         const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
         const long xy = efoff[6 * e + (j == 0 ? 0 : 1)] + i + nx * k;
         const long xz = efoff[6 * e + (k == 0 ? 2 : 4)] + i + nx * j;
@@ -716,7 +802,7 @@ km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, 
     }
     double xs[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) xs[v] = ((0.0 + sx[v]) + sy[v]) + sz[v];
+    for (int v = 0; v < NV; ++v) xs[v] = GAUSS ? sx[v] : ((0.0 + sx[v]) + sy[v]) + sz[v];
     double qd[NV], gk[NV], dt0 = 0.0, dt1 = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) qd[v] = 0.0;

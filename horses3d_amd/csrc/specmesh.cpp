@@ -38,6 +38,7 @@
 struct hx3d_mesh {
     std::string err;
     int K = 0, F = 0, bface = 0;
+    int nodes = HX3D_NODES_GAUSSLOBATTO;   // the node type the geometry is built on
     std::vector<int> N, faceID, faceSide, fElem, fESide, fRot, fNf;
     std::vector<long> off, foff;
     std::vector<double> x, Ja, J, fNormal, fT1, fT2, fJac;
@@ -413,8 +414,9 @@ int fail(hx3d_mesh *m, int code, const std::string &msg)
 }
 
 int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, long norders,
-                  const hx3d_front::ZoneResolver &resolve)
+                  const hx3d_front::ZoneResolver &resolve, bool gauss)
 {
+    m->nodes = gauss ? HX3D_NODES_GAUSS : HX3D_NODES_GAUSSLOBATTO;
     std::ifstream in(path);
     if (!in) return fail(m, HX3D_EINVAL, std::string("cannot open mesh file ") + path);
     long nn = 0, ne = 0;
@@ -695,7 +697,7 @@ int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, l
     m->J.assign(nodes, 0.0);
     double xg[8][8], wg[8][8], xc[8][8], Dc[8][64], Tc[8][64], wbc[8][8], Dg[8][64];
     for (int n = 1; n <= 7; ++n) {
-        hxs::gl_nodes(n, xg[n], wg[n]);
+        hxs::nodes_of(gauss, n, xg[n], wg[n]);
         hxs::cgl_nodes(n, xc[n]);
         hxs::bary(n, xc[n], wbc[n]);
         hxs::deriv_matrix(n, xc[n], Dc[n]);
@@ -845,8 +847,8 @@ int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, l
             nrm = dS;
         } else {
             double T1[64], T2[64];
-            if (p1) hxs::tset_host(Ne1, Nf1, T1);
-            if (p2) hxs::tset_host(Ne2, Nf2, T2);
+            if (p1) hxs::tset_host(Ne1, Nf1, T1, gauss);
+            if (p2) hxs::tset_host(Ne2, Nf2, T2, gauss);
             std::fill(nrm.begin(), nrm.end(), 0.0);
             if (p1 && !p2) {
                 for (int j = 0; j < fy; ++j)
@@ -902,7 +904,7 @@ int read_specmesh(hx3d_mesh *m, const char *path, int Nuni, const int *orders, l
 }  // namespace
 
 int hx3d_front::read_specmesh(hx3d_mesh **out, const char *path, int N, const int *orders, long norders,
-                              const ZoneResolver &resolve)
+                              const ZoneResolver &resolve, bool gauss)
 {
     if (!out) return HX3D_EINVAL;
     *out = nullptr;
@@ -911,7 +913,7 @@ int hx3d_front::read_specmesh(hx3d_mesh **out, const char *path, int N, const in
     *out = m;
     if (!path) return fail(m, HX3D_EINVAL, "no mesh file name");
     try {
-        return ::read_specmesh(m, path, N, orders, norders, resolve);
+        return ::read_specmesh(m, path, N, orders, norders, resolve, gauss);
     } catch (const std::bad_alloc &) {
         return fail(m, HX3D_ENOMEM, "out of host memory reading the mesh");
     } catch (...) {
@@ -923,6 +925,16 @@ extern "C" {
 
 int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const char *periodic, hx3d_mesh **out)
 {
+    return hx3d_mesh_read_specmesh_nodes(path, N, orders, periodic, HX3D_NODES_GAUSSLOBATTO, out);
+}
+
+int hx3d_mesh_read_specmesh_nodes(const char *path, int N, const int *orders, const char *periodic, int nodes,
+                                  hx3d_mesh **out)
+{
+    if (nodes != HX3D_NODES_GAUSSLOBATTO && nodes != HX3D_NODES_GAUSS) {
+        if (out) *out = nullptr;
+        return HX3D_EINVAL;
+    }
     // periodic pairs "a=b;c=d": boundary a is periodic, coupled with b; other zones are not periodic
     std::map<std::string, std::string> pairs;
     if (periodic) {
@@ -943,7 +955,8 @@ int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const ch
         c = it->second;
         return 1;
     };
-    return hx3d_front::read_specmesh(out, path, N, orders, hx3d_front::ORDERS_UNCHECKED, resolve);
+    return hx3d_front::read_specmesh(out, path, N, orders, hx3d_front::ORDERS_UNCHECKED, resolve,
+                                     nodes == HX3D_NODES_GAUSS);
 }
 
 const char *hx3d_mesh_error(const hx3d_mesh *m) { return m ? m->err.c_str() : "no mesh"; }
@@ -971,6 +984,7 @@ int hx3d_mesh_fill_desc(const hx3d_mesh *m, hx3d_case_desc *d)
     d->fT1 = m->fT1.data();
     d->fT2 = m->fT2.data();
     d->fJac = m->fJac.data();
+    d->nodes = m->nodes;
     return HX3D_OK;
 }
 

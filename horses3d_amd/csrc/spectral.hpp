@@ -1,5 +1,5 @@
 // Host-side spectral building blocks shared by the engine and the mesh front end: Gauss-Lobatto
-// nodes and weights, barycentric interpolation, derivative matrices and the mortar matrices,
+// and Gauss nodes and weights, barycentric interpolation, derivative matrices and the mortar matrices,
 // each restated from the reference routine it names (operation order kept: the geometry and the
 // operators must come out with the reference's bits).
 #pragma once
@@ -44,6 +44,57 @@ inline void gl_nodes(int N, double *x, double *w)
         for (int k = 2; k <= N; ++k) { L = ((2 * k - 1) * 0.0 * a1 - (k - 1) * a2) / k; a2 = a1; a1 = L; }
         x[N / 2] = 0.0; w[N / 2] = 2.0 / (N * (N + 1) * L * L);
     }
+}
+
+inline void legendre(int N, double x, double &L, double &dL)
+{
+    // LegendrePolyAndDerivative (LegendreAlgorithms.f90:88-128), three-term recursion
+    if (N == 0) { L = 1.0; dL = 0.0; return; }
+    if (N == 1) { L = x; dL = 1.0; return; }
+    double Lm2 = 1.0, dLm2 = 0.0, Lm1 = x, dLm1 = 1.0;
+    for (int k = 2; k <= N; ++k) {
+        L = ((2 * k - 1) * x * Lm1 - (k - 1) * Lm2) / k;
+        dL = dLm2 + (2 * k - 1) * Lm1;
+        Lm2 = Lm1; Lm1 = L;
+        dLm2 = dLm1; dLm1 = dL;
+    }
+}
+
+inline void gauss_nodes(int N, double *x, double *w)
+{
+    // GaussLegendreNodesAndWeights (LegendreAlgorithms.f90:137-214): Newton on L_{N+1} from the
+    // Chebyshev-Gauss points, 4 eps relative; the other half mirrored, the middle node apart
+    const double tol = 4.0 * 2.220446049250313e-16;
+    double L = 0.0, dL = 0.0;
+    if (N == 0) { x[0] = 0.0; w[0] = 2.0; return; }
+    if (N == 1) {
+        x[0] = -std::sqrt(1.0 / 3.0); w[0] = 1.0; x[1] = -x[0]; w[1] = w[0];
+    } else {
+        for (int j = 0; j < (N + 1) / 2; ++j) {
+            double xj = -std::cos((2 * j + 1) * M_PI / (2 * N + 2));
+            for (int k = 0; k <= 10; ++k) {
+                legendre(N + 1, xj, L, dL);
+                const double d = -L / dL;
+                xj = xj + d;
+                if (std::fabs(d) <= tol * std::fabs(xj)) break;
+            }
+            legendre(N + 1, xj, L, dL);
+            x[j] = xj; w[j] = 2.0 / ((1.0 - xj * xj) * (dL * dL));
+            x[N - j] = -xj; w[N - j] = w[j];
+        }
+    }
+    if (N % 2 == 0) {
+        legendre(N + 1, 0.0, L, dL);
+        x[N / 2] = 0.0; w[N / 2] = 2.0 / (dL * dL);
+    }
+}
+
+// the nodes and weights of "discretization nodes": Gauss (gauss = true) or Gauss-Lobatto
+// (NodalStorageClass.f90:208-226)
+inline void nodes_of(bool gauss, int N, double *x, double *w)
+{
+    if (gauss) gauss_nodes(N, x, w);
+    else gl_nodes(N, x, w);
 }
 
 inline void left_to_right(int i, int j, int Nx, int Ny, int rot, int &ii, int &jj)
@@ -102,11 +153,11 @@ inline void interp_matrix(int N, int M, const double *xo, const double *wb, cons
 // Tset(N -> M) (InterpolationMatrix_Construct, InterpolationMatrices.f90:42-108): Lagrange
 // interpolation when N < M, otherwise the transpose of the M -> N interpolation weighted by
 // w_N(j) / w_M(i) (the L2 projection)
-inline void tset_host(int N, int M, double *T)
+inline void tset_host(int N, int M, double *T, bool gauss = false)
 {
     double xo[8], wo[8], xd[8], wd[8], wb[8];
-    gl_nodes(N, xo, wo);
-    gl_nodes(M, xd, wd);
+    nodes_of(gauss, N, xo, wo);
+    nodes_of(gauss, M, xd, wd);
     if (N < M) {
         bary(N, xo, wb);
         interp_matrix(N, M, xo, wb, xd, T);

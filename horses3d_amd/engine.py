@@ -22,6 +22,7 @@ INVISCID = {"standard": 0, "split-form": 1}
 AVERAGING = {"standard": 0, "ducros": 1, "kennedy-gruber": 2, "pirozzoli": 3, "morinishi": 4}
 LES = {None: 0, "none": 0, "smagorinsky": 1, "wale": 2}
 LES_DEFAULT_C = {0: 0.0, 1: 0.2, 2: 0.325}   # LESModels.f90:248, :352
+NODES = {"gauss-lobatto": 0, "gauss": 1}      # "discretization nodes"
 MONITORS = {"kinetic energy": 0, "kinetic energy rate": 1, "enstrophy": 2, "mean velocity": 3,
             "internal energy": 4, "entropy": 5, "math entropy": 6}
 
@@ -35,7 +36,7 @@ class CaseDesc(C.Structure):
                 ("mach", C.c_double), ("reynolds", C.c_double), ("prandtl", C.c_double),
                 ("euler", C.c_int), ("riemann", C.c_int), ("inviscid", C.c_int),
                 ("averaging", C.c_int), ("lambda_stab", C.c_double), ("les", C.c_int),
-                ("les_c", C.c_double)]
+                ("les_c", C.c_double), ("nodes", C.c_int)]
 
 
 class ControlRun(C.Structure):
@@ -70,9 +71,11 @@ EXPORTS = ["hx3d_engine_create", "hx3d_engine_create_multi", "hx3d_multi_set_act
            "hx3d_kinetic_energy", "hx3d_check_nan", "hx3d_synchronize", "hx3d_query", "hx3d_get_time", "hx3d_engine_device_bytes",
            "hx3d_device_memory", "hx3d_set_timing", "hx3d_set_step_graph",
            "hx3d_autotune", "hx3d_set_tiling", "hx3d_get_tiling",
-           "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_error", "hx3d_mesh_num_nodes",
+           "hx3d_kernel_times", "hx3d_mesh_read_specmesh", "hx3d_mesh_read_specmesh_nodes", "hx3d_mesh_error",
+           "hx3d_mesh_num_nodes",
            "hx3d_mesh_num_face_nodes", "hx3d_mesh_coords", "hx3d_mesh_fill_desc", "hx3d_mesh_destroy",
-           "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_read_header",
+           "hx3d_hsol_error", "hx3d_hsol_reference_values", "hx3d_hsol_write", "hx3d_hsol_write_nodes",
+           "hx3d_hsol_read_header",
            "hx3d_hsol_read", "hx3d_save_solution", "hx3d_load_solution", "hx3d_volume_monitor",
            "hx3d_control_read", "hx3d_control_error", "hx3d_control_value", "hx3d_control_boundary",
            "hx3d_control_case", "hx3d_control_destroy"]
@@ -123,6 +126,7 @@ def load_library(path=LIB_PATH):
     L.hx3d_get_tiling.argtypes = [P, C.POINTER(I), C.POINTER(I)]
     L.hx3d_kernel_times.argtypes = [P, P]
     L.hx3d_mesh_read_specmesh.argtypes = [C.c_char_p, I, P, C.c_char_p, C.POINTER(C.c_void_p)]
+    L.hx3d_mesh_read_specmesh_nodes.argtypes = [C.c_char_p, I, P, C.c_char_p, I, C.POINTER(C.c_void_p)]
     L.hx3d_mesh_error.argtypes = [P]
     L.hx3d_mesh_error.restype = C.c_char_p
     L.hx3d_mesh_num_nodes.argtypes = [P]
@@ -137,6 +141,7 @@ def load_library(path=LIB_PATH):
     L.hx3d_hsol_error.restype = C.c_char_p
     L.hx3d_hsol_reference_values.argtypes = [D, P]
     L.hx3d_hsol_write.argtypes = [C.c_char_p, C.c_char_p, I, P, I, D, P, P, P]
+    L.hx3d_hsol_write_nodes.argtypes = [C.c_char_p, C.c_char_p, I, P, I, D, P, P, P, I]
     L.hx3d_hsol_read_header.argtypes = [C.c_char_p, P, P, P, P, P, P]
     L.hx3d_hsol_read.argtypes = [C.c_char_p, I, P, P, P]
     L.hx3d_save_solution.argtypes = [P, C.c_char_p, C.c_char_p, I, D]
@@ -158,11 +163,12 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def read_specmesh(path, N=None, orders=None, periodic=None):
-    """A SpecMesh file as a case dict, through the C front end (hx3d_mesh_read_specmesh).
+def read_specmesh(path, N=None, orders=None, periodic=None, nodes="gauss-lobatto"):
+    """A SpecMesh file as a case dict, through the C front end (hx3d_mesh_read_specmesh_nodes).
 
     N: one order for every element, or orders: (K, 3) per-element orders.  periodic: the
-    periodic boundaries of the .control file as {name: coupled name} (or pairs).  Runs on the
+    periodic boundaries of the .control file as {name: coupled name} (or pairs).  nodes: the
+    "discretization nodes" the geometry is built on ("gauss-lobatto" or "gauss").  Runs on the
     host only."""
     L = load_library()
     if orders is not None:
@@ -176,7 +182,8 @@ def read_specmesh(path, N=None, orders=None, periodic=None):
         items = periodic.items() if isinstance(periodic, dict) else periodic
         spec = ";".join("%s=%s" % (a, b) for a, b in items).encode()
     h = C.c_void_p()
-    rc = L.hx3d_mesh_read_specmesh(os.fsencode(path), int(N or 0), op, spec, C.byref(h))
+    rc = L.hx3d_mesh_read_specmesh_nodes(os.fsencode(path), int(N or 0), op, spec, NODES[nodes.lower()],
+                                         C.byref(h))
     try:
         if rc != 0:
             msg = L.hx3d_mesh_error(h).decode() if h else "out of memory"
@@ -271,7 +278,7 @@ class Control:
         physics = dict(mach=d.mach, reynolds=d.reynolds, prandtl=d.prandtl, euler=bool(d.euler),
                        riemann=_NAMES(RIEMANN)[d.riemann], inviscid=_NAMES(INVISCID)[d.inviscid],
                        averaging=_NAMES(AVERAGING)[d.averaging], lambda_stab=d.lambda_stab,
-                       les=_NAMES(LES)[d.les], les_c=d.les_c)
+                       les=_NAMES(LES)[d.les], les_c=d.les_c, nodes=_NAMES(NODES)[d.nodes])
         run = dict(cfl=r.cfl, dcfl=r.dcfl, dt=r.dt, tolerance=r.tolerance, steps=r.steps,
                    output_interval=r.output_interval, restart=bool(r.restart))
         return case, physics, run
@@ -296,7 +303,7 @@ def read_control(path, root=None):
 
 
 def _desc(case, keep, mach, reynolds, prandtl=0.72, riemann="roe", inviscid="standard", averaging="standard",
-          les=None, les_c=None, lambda_stab=1.0, euler=False):
+          les=None, les_c=None, lambda_stab=1.0, euler=False, nodes="gauss-lobatto"):
     """A CaseDesc of a case dict; the arrays it points to are kept in `keep`."""
     keep.update(
         N=np.ascontiguousarray(case["N"], dtype=np.int32),
@@ -318,18 +325,19 @@ def _desc(case, keep, mach, reynolds, prandtl=0.72, riemann="roe", inviscid="sta
                     _p(k["fElem"]), _p(k["fESide"]), _p(k["fRot"]), _p(k["Ja"]), _p(k["J"]),
                     fg("normal"), fg("t1"), fg("t2"), fg("fjac"), mach, reynolds, prandtl, 1 if euler else 0,
                     RIEMANN[riemann], INVISCID[inviscid], AVERAGING[averaging], lambda_stab, les_id,
-                    LES_DEFAULT_C[les_id] if les_c is None else les_c)
+                    LES_DEFAULT_C[les_id] if les_c is None else les_c, NODES[nodes.lower()])
 
 
 class Engine:
     """Device-resident DGSEM case.  State arrays on the host are (nodes, 5) float64."""
 
     def __init__(self, case, mach, reynolds, prandtl=0.72, riemann="roe", inviscid="standard",
-                 averaging="standard", les=None, les_c=None, lambda_stab=1.0, euler=False):
+                 averaging="standard", les=None, les_c=None, lambda_stab=1.0, euler=False,
+                 nodes="gauss-lobatto"):
         L = load_library()
         self._keep = {}
         d = _desc(case, self._keep, mach, reynolds, prandtl, riemann, inviscid, averaging, les, les_c,
-                  lambda_stab, euler)
+                  lambda_stab, euler, nodes)
         h = C.c_void_p()
         rc = L.hx3d_engine_create(C.byref(d), C.byref(h))
         self._created(L, h, rc, "hx3d_engine_create")

@@ -23,6 +23,8 @@ module hx3d_binding
                                 HX3D_AVG_MORINISHI = 4
    ! "les model"
    integer(c_int), parameter :: HX3D_LES_NONE = 0, HX3D_LES_SMAGORINSKY = 1, HX3D_LES_WALE = 2
+   ! "discretization nodes"
+   integer(c_int), parameter :: HX3D_NODES_GAUSSLOBATTO = 0, HX3D_NODES_GAUSS = 1
 
    type, bind(C) :: hx3d_case_desc
       integer(c_int) :: K = 0
@@ -46,6 +48,7 @@ module hx3d_binding
       real(c_double) :: lambda_stab = 1.0_c_double
       integer(c_int) :: les = HX3D_LES_NONE
       real(c_double) :: les_c = 0.0_c_double
+      integer(c_int) :: nodes = HX3D_NODES_GAUSSLOBATTO
    end type hx3d_case_desc
 
    ! volume monitors (hx3d_volume_monitor kinds)
@@ -198,6 +201,16 @@ module hx3d_binding
          integer(c_int), value              :: N
          type(c_ptr), value                 :: orders
          character(kind=c_char), intent(in) :: periodic(*)
+         type(c_ptr), intent(out)           :: mesh
+      end function
+      integer(c_int) function hx3d_mesh_read_specmesh_nodes(path, N, orders, periodic, nodes, mesh) &
+            bind(C, name="hx3d_mesh_read_specmesh_nodes")
+         import :: c_int, c_ptr, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value              :: N
+         type(c_ptr), value                 :: orders
+         character(kind=c_char), intent(in) :: periodic(*)
+         integer(c_int), value              :: nodes
          type(c_ptr), intent(out)           :: mesh
       end function
       type(c_ptr) function hx3d_mesh_error(mesh) bind(C, name="hx3d_mesh_error")

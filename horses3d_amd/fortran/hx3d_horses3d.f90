@@ -27,6 +27,7 @@ module hx3d_horses3d
 #endif
    use DGSEMClass, only: ComputeTimeDerivative_f
    use ExplicitMethods, only: CTD_AFTER_STEPS
+   use NodalStorageClass, only: CurrentNodes, GAUSS
    use hx3d_binding
    implicit none
    private
@@ -116,6 +117,8 @@ contains
       d % mach = mach ; d % reynolds = reynolds ; d % prandtl = prandtl
       d % riemann = riemann ; d % inviscid = inviscid ; d % averaging = averaging
       d % les = les ; d % les_c = les_c
+      ! the node type the reference built its NodalStorage and geometry on
+      d % nodes = merge(HX3D_NODES_GAUSS, HX3D_NODES_GAUSSLOBATTO, CurrentNodes == GAUSS)
       call check(hx3d_engine_create(d, eng))
    end subroutine hx3d_attach
 

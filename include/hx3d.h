@@ -53,6 +53,7 @@ enum { HX3D_INVISCID_STANDARD = 0, HX3D_INVISCID_SPLIT = 1 };           /* "invi
 enum { HX3D_AVG_STANDARD = 0, HX3D_AVG_DUCROS = 1, HX3D_AVG_KENNEDYGRUBER = 2,
        HX3D_AVG_PIROZZOLI = 3, HX3D_AVG_MORINISHI = 4 };                 /* "averaging" */
 enum { HX3D_LES_NONE = 0, HX3D_LES_SMAGORINSKY = 1, HX3D_LES_WALE = 2 }; /* "les model" */
+enum { HX3D_NODES_GAUSSLOBATTO = 0, HX3D_NODES_GAUSS = 1 };             /* "discretization nodes" */
 
 typedef struct {
     /* mesh (ConstructMesh_FromSpecMeshFile_ / HexMesh, HexMesh.f90:71-180) */
@@ -84,13 +85,17 @@ typedef struct {
     double lambda_stab;     /* "lambda stabilization" (LxF), default 1 */
     int les;                /* "les model" */
     double les_c;           /* "les model intensity" (Smagorinsky 0.2, WALE 0.325) */
+    /* "discretization nodes" (NodalStorageClass.f90:80-105): HX3D_NODES_GAUSSLOBATTO (0, so a
+     * zero-filled descriptor keeps Gauss-Lobatto) or HX3D_NODES_GAUSS, the reference's default
+     * (main.f90:278-281; standard form only, and the face geometry must be given) */
+    int nodes;
 } hx3d_case_desc;
 
 typedef struct hx3d_engine hx3d_engine;
 
 /* Creates an engine on the current HIP device: uploads geometry, builds the
- * element/face gather tables.  Supported: Gauss-Lobatto nodes, orders 1 <= N <= 7 (one
- * order everywhere, or per-element (Nx, Ny, Nz) with mortar faces, whose face frames must
+ * element/face gather tables.  Supported: Gauss-Lobatto or Gauss nodes, orders 1 <= N <= 7
+ * (one order everywhere, or per-element (Nx, Ny, Nz) with mortar faces, whose face frames must
  * then be given at the face orders), every face interior or periodic. */
 int hx3d_engine_create(const hx3d_case_desc *desc, hx3d_engine **out);
 int hx3d_engine_destroy(hx3d_engine *eng);
@@ -224,6 +229,11 @@ int hx3d_hsol_reference_values(double mach, double refs[6]);
 /* header_name: the name stored in the file (NULL: path); grad may be NULL (type 2 file) */
 int hx3d_hsol_write(const char *path, const char *header_name, int K, const int *N, int iter, double time,
                     const double refs[6], const double *Q, const double *grad);
+/* the same with the node type field of the file set from nodes (HX3D_NODES_*); hx3d_hsol_write
+ * writes Gauss-Lobatto files.  A reader takes a file of either type as it is (node_type in
+ * the header: 1 Gauss, 2 Gauss-Lobatto), as HexMesh_LoadSolution does without interpolation. */
+int hx3d_hsol_write_nodes(const char *path, const char *header_name, int K, const int *N, int iter, double time,
+                          const double refs[6], const double *Q, const double *grad, int nodes);
 int hx3d_hsol_read_header(const char *path, int *type, int *node_type, int *K, int *iter, double *time,
                           double refs[6]);
 /* checks the element count and each element's orders against N; grad may be NULL */
@@ -245,6 +255,12 @@ int hx3d_load_solution(hx3d_engine *eng, const char *path, int *iter, double *ti
  * object with the message (hx3d_mesh_error); destroy it. */
 typedef struct hx3d_mesh hx3d_mesh;
 int hx3d_mesh_read_specmesh(const char *path, int N, const int *orders, const char *periodic, hx3d_mesh **out);
+/* The same with the geometry built on the given "discretization nodes" (HX3D_NODES_*): node
+ * coordinates, metrics and face frames at Gauss or Gauss-Lobatto points (the NodalStorage the
+ * reference's ConstructMappedGeometry uses); hx3d_mesh_read_specmesh is the Gauss-Lobatto form.
+ * hx3d_mesh_fill_desc then sets desc->nodes. */
+int hx3d_mesh_read_specmesh_nodes(const char *path, int N, const int *orders, const char *periodic, int nodes,
+                                  hx3d_mesh **out);
 const char *hx3d_mesh_error(const hx3d_mesh *m);
 long hx3d_mesh_num_nodes(const hx3d_mesh *m);
 long hx3d_mesh_num_face_nodes(const hx3d_mesh *m);

@@ -76,6 +76,43 @@ static void gl_nodes(int N, double *x, double *w)
     }
 }
 
+static void gauss_nodes(int N, double *x, double *w)
+{
+    /* LegendreAlgorithms.f90:137-214 GaussLegendreNodesAndWeights: Newton on L_{N+1} from
+     * the Chebyshev-Gauss points, 4 eps relative, half the nodes mirrored */
+    const double tol = 4.0 * 2.220446049250313e-16;
+    double L, dL;
+    if (N == 0) { x[0] = 0.0; w[0] = 2.0; return; }
+    if (N == 1) {
+        x[0] = -sqrt(1.0 / 3.0); w[0] = 1.0; x[1] = -x[0]; w[1] = w[0];
+    } else {
+        int nd2 = (N + 1) / 2;
+        for (int j = 0; j < nd2; ++j) {
+            double xj = -cos((2 * j + 1) * M_PI / (2 * N + 2));
+            for (int k = 0; k <= 10; ++k) {
+                legendre_poly_and_derivative(N + 1, xj, &L, &dL);
+                double delta = -L / dL;
+                xj = xj + delta;
+                if (fabs(delta) <= tol * fabs(xj)) break;
+            }
+            legendre_poly_and_derivative(N + 1, xj, &L, &dL);
+            x[j] = xj; w[j] = 2.0 / ((1.0 - xj * xj) * (dL * dL));
+            x[N - j] = -xj; w[N - j] = w[j];
+        }
+    }
+    if (N % 2 == 0) {
+        legendre_poly_and_derivative(N + 1, 0.0, &L, &dL);
+        x[N / 2] = 0.0; w[N / 2] = 2.0 / (dL * dL);
+    }
+}
+
+/* the nodes of the case's node type (NodalStorageClass.f90:208-226) */
+static void nodes_of(int type, int N, double *x, double *w)
+{
+    if (type == OX_NODES_GAUSS) gauss_nodes(N, x, w);
+    else gl_nodes(N, x, w);
+}
+
 static int almost_equal(double a, double b)
 {
     /* libs/foundation/Utilities.f90:110-139 AlmostEqual (tol = 2 eps) */
@@ -168,6 +205,25 @@ int ox_nodal_gl(int N, double *x, double *w, double *D, double *hatD, double *sh
     return 0;
 }
 
+int ox_nodal_gauss(int N, double *x, double *w, double *D, double *hatD, double *vL, double *vR,
+                   double *bL, double *bR)
+{
+    /* NodalStorageClass.f90:170-337 ConstructNodalStorage (GAUSS): no sharpD, the boundary
+     * values v(+-1) and b = v / w are dense */
+    if (N < 1 || N >= MAXN) return 1;
+    int n1 = N + 1;
+    double wb[MAXN];
+    gauss_nodes(N, x, w);
+    deriv_matrix(N, x, D);
+    for (int j = 0; j <= N; ++j)
+        for (int i = 0; i <= N; ++i) hatD[i * n1 + j] = D[j * n1 + i] * w[j] / w[i];
+    bary_weights(N, x, wb);
+    interp_vector(1.0, N, x, wb, vR);
+    interp_vector(-1.0, N, x, wb, vL);
+    for (int i = 0; i <= N; ++i) { bL[i] = vL[i] / w[i]; bR[i] = vR[i] / w[i]; }
+    return 0;
+}
+
 int ox_nodal_cgl(int N, double *xcgl, double *DCGL, double *T)
 {
     /* NodalStorageClass.f90:282-296 (Chebyshev-Gauss-Lobatto framework) */
@@ -181,15 +237,18 @@ int ox_nodal_cgl(int N, double *xcgl, double *DCGL, double *T)
     return 0;
 }
 
-int ox_tset(int N, int M, double *T)
+int ox_tset(int N, int M, double *T) { return ox_tset_nodes(OX_NODES_GAUSSLOBATTO, N, M, T); }
+
+int ox_tset_nodes(int nodes, int N, int M, double *T)
 {
-    /* libs/spectral/InterpolationMatrices.f90:42-108 InterpolationMatrix_Construct:
+    /* libs/spectral/InterpolationMatrices.f90:42-108 InterpolationMatrix_Construct on the
+     * case's nodes:
      * N < M: Lagrange interpolation from N-nodes to M-nodes;
      * N > M: L2 projection = transpose of the M->N interpolation weighted by w_N(j)/w_M(i). */
     if (N < 1 || M < 1 || N >= MAXN || M >= MAXN) return 1;
     double xo[MAXN], wo[MAXN], xd[MAXN], wd[MAXN], wb[MAXN];
-    gl_nodes(N, xo, wo);
-    gl_nodes(M, xd, wd);
+    nodes_of(nodes, N, xo, wo);
+    nodes_of(nodes, M, xd, wd);
     if (N == M) {
         for (int i = 0; i <= M; ++i)
             for (int j = 0; j <= N; ++j) T[i * (N + 1) + j] = (i == j) ? 1.0 : 0.0;
@@ -258,7 +317,7 @@ static double *tset(ox_case *c, int N, int M)
 {
     if (!c->Tset[N][M]) {
         c->Tset[N][M] = (double *)malloc(sizeof(double) * (M + 1) * (N + 1));
-        ox_tset(N, M, c->Tset[N][M]);
+        ox_tset_nodes(c->p.nodes, N, M, c->Tset[N][M]);
     }
     return c->Tset[N][M];
 }
@@ -286,6 +345,9 @@ ox_case *ox_case_create(int K, const int *N, const long *off, const int *faceID,
                         const long *foff, const double *normal, const double *t1,
                         const double *t2, const double *fjac, const ox_params *p)
 {
+    /* split forms need the Gauss-Lobatto SBP operators (the reference switches a split-form
+     * case on Gauss nodes to Gauss-Lobatto, main.f90:305-313) */
+    if (p->nodes == OX_NODES_GAUSS && p->inviscid == OX_SPLIT) return NULL;
     ox_case *c = (ox_case *)calloc(1, sizeof(ox_case));
     c->K = K; c->F = F; c->p = *p;
     c->nodes = off[K];
@@ -310,7 +372,10 @@ ox_case *ox_case_create(int K, const int *N, const long *off, const int *faceID,
     for (int n = 1; n < MAXN; ++n) {
         nodal_t *nd = &c->nod[n];
         nd->N = n;
-        ox_nodal_gl(n, nd->x, nd->w, nd->D, nd->hatD, nd->sharpD, nd->vL, nd->vR, nd->bL, nd->bR);
+        if (p->nodes == OX_NODES_GAUSS)
+            ox_nodal_gauss(n, nd->x, nd->w, nd->D, nd->hatD, nd->vL, nd->vR, nd->bL, nd->bR);
+        else
+            ox_nodal_gl(n, nd->x, nd->w, nd->D, nd->hatD, nd->sharpD, nd->vL, nd->vR, nd->bL, nd->bR);
     }
     /* PhysicsStorage_NS.f90:145-439 with ThermodynamicsAir (:115-132) */
     c->gamma = 1.4; c->gm1 = 1.4 - 1.0;

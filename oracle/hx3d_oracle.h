@@ -30,6 +30,7 @@ enum { OX_ROE = 0, OX_LXF = 1 };
 enum { OX_STANDARD = 0, OX_SPLIT = 1 };
 enum { OX_AVG_STANDARD = 0, OX_AVG_DUCROS = 1, OX_AVG_KG = 2, OX_AVG_PIROZZOLI = 3, OX_AVG_MORINISHI = 4 };
 enum { OX_LES_NONE = 0, OX_LES_SMAGORINSKY = 1, OX_LES_WALE = 2 };
+enum { OX_NODES_GAUSSLOBATTO = 0, OX_NODES_GAUSS = 1 };   /* "discretization nodes" */
 
 typedef struct {
     double mach, reynolds, prandtl;
@@ -37,6 +38,7 @@ typedef struct {
     double lambda_stab;   /* LxF stabilisation factor (default 1) */
     double les_c;         /* Smagorinsky Cs or WALE Cw */
     int euler;            /* 1: inviscid (flow equations = "Euler") */
+    int nodes;            /* OX_NODES_GAUSSLOBATTO or OX_NODES_GAUSS (standard form only) */
 } ox_params;
 
 typedef struct ox_case ox_case;
@@ -45,10 +47,15 @@ typedef struct ox_case ox_case;
  * Arrays are (N+1) or (N+1)^2 row-major: D[i*(N+1)+j] = D(i,j). */
 int ox_nodal_gl(int N, double *x, double *w, double *D, double *hatD, double *sharpD,
                 double *vL, double *vR, double *bL, double *bR);
+/* the same on Gauss(-Legendre) nodes (no sharpD: split forms run on Gauss-Lobatto nodes) */
+int ox_nodal_gauss(int N, double *x, double *w, double *D, double *hatD,
+                   double *vL, double *vR, double *bL, double *bR);
 int ox_nodal_cgl(int N, double *xcgl, double *DCGL, double *Tcheb2gauss);
 /* Tset(N->M): interpolation (N<M) or weighted projection (N>M), InterpolationMatrices.f90:42-108.
  * T[(M+1)x(N+1)] row-major. */
 int ox_tset(int N, int M, double *T);
+/* the same on the given node type (OX_NODES_*) */
+int ox_tset_nodes(int nodes, int N, int M, double *T);
 
 ox_case *ox_case_create(int K, const int *N, const long *off,
                         const int *faceID, const int *faceSide,

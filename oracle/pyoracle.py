@@ -19,21 +19,24 @@ RIEMANN = {"roe": 0, "lax-friedrichs": 1, "lxf": 1}
 INVISCID = {"standard": 0, "split-form": 1}
 AVERAGING = {"standard": 0, "ducros": 1, "kennedy-gruber": 2, "pirozzoli": 3, "morinishi": 4}
 LES = {None: 0, "none": 0, "smagorinsky": 1, "wale": 2}
+NODES = {"gauss-lobatto": 0, "gauss": 1}
 
 
 class Params(C.Structure):
     _fields_ = [("mach", C.c_double), ("reynolds", C.c_double), ("prandtl", C.c_double),
                 ("riemann", C.c_int), ("inviscid", C.c_int), ("averaging", C.c_int),
                 ("les", C.c_int), ("lambda_stab", C.c_double), ("les_c", C.c_double),
-                ("euler", C.c_int)]
+                ("euler", C.c_int), ("nodes", C.c_int)]
 
 
 def make_params(mach, reynolds, prandtl=0.72, riemann="roe", inviscid="standard",
-                averaging="standard", les=None, lambda_stab=1.0, les_c=None, euler=False):
+                averaging="standard", les=None, lambda_stab=1.0, les_c=None, euler=False,
+                nodes="gauss-lobatto"):
     if les_c is None:
         les_c = {0: 0.0, 1: 0.2, 2: 0.325}[LES[les]]
     return Params(mach, reynolds, prandtl, RIEMANN[riemann], INVISCID[inviscid],
-                  AVERAGING[averaging], LES[les], lambda_stab, les_c, 1 if euler else 0)
+                  AVERAGING[averaging], LES[les], lambda_stab, les_c, 1 if euler else 0,
+                  NODES[nodes])
 
 
 _lib = None
@@ -57,8 +60,10 @@ def lib():
         L.ox_kinetic_energy.restype = C.c_double
         L.ox_kinetic_energy.argtypes = [P, P]
         L.ox_nodal_gl.argtypes = [C.c_int] + [P] * 9
+        L.ox_nodal_gauss.argtypes = [C.c_int] + [P] * 8
         L.ox_nodal_cgl.argtypes = [C.c_int, P, P, P]
         L.ox_tset.argtypes = [C.c_int, C.c_int, P]
+        L.ox_tset_nodes.argtypes = [C.c_int, C.c_int, C.c_int, P]
         _lib = L
     return _lib
 
@@ -76,6 +81,15 @@ def nodal_gl(N):
     return dict(x=x, w=w, D=D, hatD=hatD, sharpD=sharpD, vL=vL, vR=vR, bL=bL, bR=bR)
 
 
+def nodal_gauss(N):
+    n = N + 1
+    x, w = np.zeros(n), np.zeros(n)
+    D, hatD = np.zeros((n, n)), np.zeros((n, n))
+    vL, vR, bL, bR = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().ox_nodal_gauss(N, *[_p(a) for a in (x, w, D, hatD, vL, vR, bL, bR)])
+    return dict(x=x, w=w, D=D, hatD=hatD, vL=vL, vR=vR, bL=bL, bR=bR)
+
+
 def nodal_cgl(N):
     n = N + 1
     x, D, T = np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
@@ -83,9 +97,9 @@ def nodal_cgl(N):
     return dict(xcgl=x, DCGL=D, Tcheb2gauss=T)
 
 
-def tset(N, M):
+def tset(N, M, nodes="gauss-lobatto"):
     T = np.zeros((M + 1, N + 1))
-    lib().ox_tset(N, M, _p(T))
+    lib().ox_tset_nodes(NODES[nodes], N, M, _p(T))
     return T
 
 
@@ -145,6 +159,8 @@ class Oracle:
                                       _p(c["fElem"]), _p(c["fESide"]), _p(c["fRot"]), _p(c["fNf"]),
                                       _p(c["foff"]), _p(c["normal"]), _p(c["t1"]), _p(c["t2"]),
                                       _p(c["fjac"]), C.byref(params))
+        if not self.h:
+            raise ValueError("oracle: unsupported case (split forms run on Gauss-Lobatto nodes)")
 
     def __del__(self):
         if getattr(self, "h", None):

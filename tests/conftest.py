@@ -47,7 +47,8 @@ def physics_kwargs(meta):
     return dict(mach=meta["mach"], reynolds=meta["re"], riemann=meta.get("riemann", "roe"),
                 inviscid=meta.get("inviscid", "standard"),
                 averaging=meta.get("averaging", "standard").lower(),
-                les=les.lower() if les else None, les_c=meta.get("les_intensity"))
+                les=les.lower() if les else None, les_c=meta.get("les_intensity"),
+                nodes=meta.get("nodes", "Gauss-Lobatto").lower())
 
 
 @pytest.fixture(params=GOLDEN, ids=lambda p: os.path.basename(p)[:-4])

@@ -1,10 +1,10 @@
 """Drop-in fixtures: the reference solver (oracle/_ref/bin/horses3d.ns, its own TakeRK3Step on the
-CPU) on two small cases, keeping the residual history (ComputeMaxResiduals after every step,
+CPU) on three small cases, keeping the residual history (ComputeMaxResiduals after every step,
 RESULTS/case.residuals) and the kinetic-energy monitor (RESULTS/case.kinen.volume) it writes.
 tests/test_dropin_gpu.py runs the same set-up with the RK3 step taken by the hx3d engine through
 the Fortran adapter (oracle/_ref/bin/hx3d_dropin.ns) and compares the two.
 
-    python tests/golden/make_dropin.py     # writes tests/golden/dropin/*.npz
+    python tests/golden/make_dropin.py [case ...]    # writes tests/golden/dropin/*.npz
 """
 import os
 import shutil
@@ -30,6 +30,11 @@ CASES = {
     "c2_tgv_n3_p4_curved_split_pirozzoli_lxf_20steps": dict(
         n=3, N=4, amp=0.1, seed=6, steps=20, ctd_after=True,
         ctl=dict(riemann="lax-friedrichs", inviscid="split-form", averaging="Pirozzoli")),
+    # 3^3 curved box, N = 3 on Gauss nodes (the reference's default node type), Roe, standard
+    # form, 30 steps, the last RK stage's residuals
+    "c3_tgv_n3_p3_curved_gauss_roe_30steps": dict(
+        n=3, N=3, amp=0.1, seed=9, steps=30, ctd_after=False,
+        ctl=dict(riemann="roe", inviscid="standard", averaging="Standard", nodes="Gauss")),
 }
 
 
@@ -55,11 +60,12 @@ def read_residuals(d):
     return np.array(rows)
 
 
-def main():
+def main(names):
     os.makedirs(OUT, exist_ok=True)
     work = tempfile.mkdtemp(prefix="hx3d_dropin_")
     try:
-        for name, spec in CASES.items():
+        for name in names or list(CASES):
+            spec = CASES[name]
             d = os.path.join(work, name)
             os.makedirs(d)
             write_case(d, spec)
@@ -72,4 +78,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

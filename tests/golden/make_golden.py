@@ -77,6 +77,18 @@ CASES = {
     "x_noise_n3_p2_standard_roe": dict(
         n=3, N=2, amp=0.1, seed=10, steps=2, dumps=[0], noise=0.05, noise_seed=101,
         ctl=dict(mach=0.8, re=400.0)),
+    # Gauss nodes (the reference's default "discretization nodes"; standard form only): dense
+    # prolongation to the faces and surface weights on every node
+    "g1_tgv_n3_p3_gauss_roe_standard": dict(
+        n=3, N=3, amp=0.0, seed=0, steps=4, dumps=[0, 2],
+        ctl=dict(mach=0.1, re=1600.0, nodes="Gauss")),
+    "g2_noise_n2_p4_curved_gauss_lxf_smagorinsky": dict(
+        n=2, N=4, amp=0.1, seed=14, steps=2, dumps=[0], noise=0.05, noise_seed=141,
+        ctl=dict(mach=0.3, re=900.0, riemann="lax-friedrichs", les="Smagorinsky", les_intensity=0.2,
+                 nodes="Gauss")),
+    "g3_noise_n3_mixed_p2to5_gauss_roe": dict(
+        n=3, N=5, orders=(2, 5, 55), amp=0.1, seed=15, steps=2, dumps=[0], noise=0.02, noise_seed=151,
+        ctl=dict(mach=0.2, re=1600.0, nodes="Gauss")),
 }
 
 

@@ -1,8 +1,8 @@
-"""Solution-file fixtures: runs the reference build (oracle/_ref/bin/horses3d.ns) on two small
+"""Solution-file fixtures: runs the reference build (oracle/_ref/bin/horses3d.ns) on three small
 periodic boxes and keeps the .hsol files it writes (HexMesh_SaveSolution, HexMesh.f90:3376-3556)
 together with the state, derivative and time its ProblemFile dump holds at the same instant.
 
-    python tests/golden/make_hsol.py          # writes tests/golden/hsol/*.hsol and *.npz
+    python tests/golden/make_hsol.py [case ...]    # writes tests/golden/hsol/*.hsol and *.npz
 """
 import os
 import shutil
@@ -24,6 +24,8 @@ CASES = {
     "box2_p3_curved": dict(n=2, N=3, amp=0.1, seed=3, steps=3, orders=None),
     # 2^3 straight box with per-element orders on {2..4} (element arrays of different sizes)
     "box2_mixed_p2to4": dict(n=2, N=4, amp=0.0, seed=0, steps=2, orders=(2, 4, 21)),
+    # 2^3 curved box, N = 3 on Gauss nodes (the file's node type field is 1)
+    "box2_p3_curved_gauss": dict(n=2, N=3, amp=0.1, seed=4, steps=2, orders=None, nodes="Gauss"),
 }
 
 
@@ -38,6 +40,8 @@ def run(name, spec, work):
         orders = np.random.default_rng(seed).integers(lo, hi + 1, (spec["n"] ** 3, 3))
         rc.write_orders(os.path.join(d, "orders.txt"), orders)
         kw["order_file"] = "orders.txt"
+    if spec.get("nodes"):
+        kw["nodes"] = spec["nodes"]
     rc.write_control(os.path.join(d, "case.control"), "box.mesh", spec["N"], spec["steps"], 0.1, 1600.0, **kw)
     rc.run_reference(d, env={"HX3D_DUMP_PREFIX": os.path.join(d, "dump"), "HX3D_DUMP_STEPS": "0"})
     for f, tag in (("case.hsol", "final"), ("case_0000000000.hsol", "step0")):
@@ -55,16 +59,16 @@ def run(name, spec, work):
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
 
 
-def main():
+def main(names):
     os.makedirs(OUT, exist_ok=True)
     work = tempfile.mkdtemp(prefix="hx3d_hsol_")
     try:
-        for name, spec in CASES.items():
-            run(name, spec, work)
+        for name in names or list(CASES):
+            run(name, CASES[name], work)
             print(name, flush=True)
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

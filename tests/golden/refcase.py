@@ -24,7 +24,7 @@ from horses3d_amd.specmesh import (CORNER_XI, FACE_FIXED, FACE_NAMES_BOX, FACE_U
 def write_control(path, mesh, order, steps, mach, re, riemann="roe", inviscid="standard",
                   averaging=None, cfl=0.4, dcfl=0.4, les=None, les_intensity=None,
                   order_file=None, pr=0.72, lambda_stab=None, ctd_after=True, extra=None,
-                  monitors=(("KinEn", "Kinetic energy"),)):
+                  monitors=(("KinEn", "Kinetic energy"),), nodes="Gauss-Lobatto"):
     kv = [("Flow equations", '"NS"'),
           ("mesh file name", '"%s"' % mesh),
           ("solution file name", '"RESULTS/case.hsol"'),
@@ -41,7 +41,7 @@ def write_control(path, mesh, order, steps, mach, re, riemann="roe", inviscid="s
           ("riemann solver", '"%s"' % riemann),
           ("inviscid discretization", '"%s"' % inviscid),
           ("viscous discretization", '"BR1"'),
-          ("discretization nodes", '"Gauss-Lobatto"'),
+          ("discretization nodes", '"%s"' % nodes),
           ("compute time derivative after timestep", ".true." if ctd_after else ".false."),
           ("AOA theta", "0.0"),
           ("AOA phi", "0.0")]

@@ -105,7 +105,8 @@ def test_settings_of_the_syntax_case(tmp_path):
     assert c.get("prandtl number") is None and c.get("name") is None
     case, phys, run = c.case(str(tmp_path))
     assert phys == dict(mach=0.25, reynolds=1200.0, prandtl=0.72, euler=False, riemann="lax-friedrichs",
-                        inviscid="split-form", averaging="kennedy-gruber", lambda_stab=0.5, les="none", les_c=0.0)
+                        inviscid="split-form", averaging="kennedy-gruber", lambda_stab=0.5, les="none", les_c=0.0,
+                        nodes="gauss-lobatto")
     assert run == dict(cfl=0.3, dcfl=0.25, dt=0.0, tolerance=1e-30, steps=1, output_interval=1, restart=False)
 
 
@@ -149,7 +150,7 @@ def case_of(d, text):
 def test_defaults_and_fixed_dt(tmp_path):
     case, phys, run = case_of(tmp_path, BASE + PERIODIC)
     assert phys == dict(mach=0.1, reynolds=1600.0, prandtl=0.72, euler=False, riemann="roe", inviscid="standard",
-                        averaging="standard", lambda_stab=1.0, les="none", les_c=0.0)
+                        averaging="standard", lambda_stab=1.0, les="none", les_c=0.0, nodes="gauss-lobatto")
     assert run == dict(cfl=0.0, dcfl=0.0, dt=1e-3, tolerance=1e-10, steps=10, output_interval=5, restart=True)
     assert case["K"] == 8 and (case["N"] == 2).all()
 
@@ -171,13 +172,29 @@ def test_defaults_and_fixed_dt(tmp_path):
     (BASE + 'viscous discretization = "IP"\n', 2, "BR1"),
     (BASE + 'les model = "vreman"\n', 2, "vreman"),
     (BASE + 'les model = "dsm"\n', 1, "not implemented"),
-    (BASE.replace('"Gauss-Lobatto"', '"Gauss"'), 2, "Gauss nodes"),
     (BASE.replace('"Gauss-Lobatto"', '"gauss-lobatto"'), 1, "Unknown discretization nodes"),
 ])
 def test_errors(tmp_path, text, code, msg):
     with pytest.raises(E.HX3DError) as ei:
         case_of(tmp_path, text + PERIODIC)
     assert msg in str(ei.value) and "(code %d)" % code in str(ei.value)
+
+
+def test_discretization_nodes(tmp_path):
+    """Gauss nodes (the reference's default when the key is absent, main.f90:278-281) reach the
+    descriptor and the front end builds the geometry on them: no node on the element boundary;
+    a split-form case on Gauss nodes runs on Gauss-Lobatto nodes, as the reference switches it."""
+    case, phys, _ = case_of(tmp_path, BASE.replace('"Gauss-Lobatto"', '"Gauss"') + PERIODIC)
+    assert phys["nodes"] == "gauss"
+    x = np.asarray(case["x"]).reshape(-1, 3)
+    assert np.abs(x[:3, 0] + np.pi).min() > 0.05   # element 0's first nodes sit inside it
+    case, phys, _ = case_of(tmp_path, BASE.replace('discretization nodes = "Gauss-Lobatto"\n', "") + PERIODIC)
+    assert phys["nodes"] == "gauss"
+    case, phys, _ = case_of(tmp_path, BASE.replace('discretization nodes = "Gauss-Lobatto"\n', "") +
+                            'inviscid discretization = "split-form"\n' + PERIODIC)
+    assert phys["nodes"] == "gauss-lobatto" and phys["inviscid"] == "split-form"
+    x = np.asarray(case["x"]).reshape(-1, 3)
+    assert x[0, 0] == -np.pi
 
 
 def test_boundary_errors(tmp_path):

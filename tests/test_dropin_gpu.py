@@ -39,13 +39,14 @@ def test_fixture_initial_residuals_match_oracle(name, tmp_path):
     spec = md.CASES[name]
     md.write_case(str(tmp_path), spec)
     from horses3d_amd import specmesh
-    case = read_specmesh(str(tmp_path / "box.mesh"), N=spec["N"], periodic=dict(specmesh.PERIODIC_PAIRS))
+    ctl = spec["ctl"]
+    nodes = ctl.get("nodes", "Gauss-Lobatto").lower()
+    case = read_specmesh(str(tmp_path / "box.mesh"), N=spec["N"], periodic=dict(specmesh.PERIODIC_PAIRS), nodes=nodes)
     n1 = spec["N"] + 1
     case["fNf"] = np.full((case["F"], 2), spec["N"], dtype=np.int32)   # one order: every face of order N
     case["foff"] = np.arange(case["F"] + 1, dtype=np.int64) * n1 * n1
-    ctl = spec["ctl"]
     o = po.Oracle(case, po.make_params(0.1, 1600.0, riemann=ctl["riemann"], inviscid=ctl["inviscid"],
-                                       averaging=ctl["averaging"].lower()))
+                                       averaging=ctl["averaging"].lower(), nodes=nodes))
     Q0 = mesh.taylor_green(case, 0.1)
     got = np.abs(o.qdot(Q0)).max(axis=0)
     ref = _fixture(name)["residuals"][0, 2:]

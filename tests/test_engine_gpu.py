@@ -20,7 +20,8 @@ def engine_for(case, kw):
 def oracle_for(case, kw):
     return po.Oracle(case, po.make_params(kw["mach"], kw["reynolds"], riemann=kw["riemann"],
                                           inviscid=kw["inviscid"], averaging=kw["averaging"],
-                                          les=kw["les"], les_c=kw["les_c"]))
+                                          les=kw["les"], les_c=kw["les_c"],
+                                          nodes=kw.get("nodes", "gauss-lobatto")))
 
 
 def test_qdot_matches_reference(golden):
@@ -534,7 +535,8 @@ def test_euler_advance_matches_oracle(mixed):
     eng.close()
 
 
-@pytest.mark.parametrize("name", ["c2s_tgv_n3_p4_curved_split_pirozzoli_lxf", "x_noise_n2_mixed_p2to6_curved_split_pirozzoli_lxf"])
+@pytest.mark.parametrize("name", ["c2s_tgv_n3_p4_curved_split_pirozzoli_lxf", "x_noise_n2_mixed_p2to6_curved_split_pirozzoli_lxf",
+                                  "g2_noise_n2_p4_curved_gauss_lxf_smagorinsky", "g3_noise_n3_mixed_p2to5_gauss_roe"])
 def test_specmesh_file_to_qdot(name, tmp_path):
     """End to end from a mesh file: the SpecMesh file the reference read, through the C front
     end (hx3d_mesh_read_specmesh) into the engine; QDot against the reference's."""
@@ -547,12 +549,64 @@ def test_specmesh_file_to_qdot(name, tmp_path):
                             bface_order=max(4, meta["order"]))
     mixed = len({tuple(r) for r in case["N"]}) > 1
     fc = read_specmesh(path, N=meta["order"], orders=case["N"] if mixed else None,
-                       periodic=dict(specmesh.PERIODIC_PAIRS))
+                       periodic=dict(specmesh.PERIODIC_PAIRS), nodes=meta.get("nodes", "Gauss-Lobatto"))
     eng = engine_for(fc, physics_kwargs(meta))
     eng.set_state(case["Q_step0"])
     err = po.normwise_error(eng.qdot(), case["QDot_step0"])
     print(name, err, flush=True)
     assert err.max() <= TOL, err
+    eng.close()
+
+
+def _face_orders(case):
+    """Face orders (the larger of the two sides per face direction, the right side's in the face
+    orientation) and face-node offsets of a case read by the front end (Face_LinkWithElements,
+    FaceClass.f90:187-278)."""
+    uv = {1: (0, 2), 2: (0, 2), 3: (0, 1), 4: (1, 2), 5: (0, 1), 6: (1, 2)}
+    N = np.asarray(case["N"])
+    fNf = np.zeros((case["F"], 2), np.int32)
+    for f in range(case["F"]):
+        (eL, eR), (sL, sR), rot = case["fElem"][f], case["fESide"][f], case["fRot"][f]
+        a, b = N[eL][list(uv[sL])], N[eR][list(uv[sR])]
+        if rot not in (0, 2, 5, 7):
+            b = b[::-1]
+        fNf[f] = np.maximum(a, b)
+    foff = np.concatenate([[0], np.cumsum(np.prod(fNf + 1, axis=1))]).astype(np.int64)
+    return fNf, foff
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["uniform_n4", "orders_2to6"])
+def test_gauss_nodes_steps_match_oracle(mixed, tmp_path):
+    """Gauss nodes (the reference's default "discretization nodes") at a larger size than the
+    fixtures: a 4^3 curvilinear box (amplitude 0.1) through the front end on Gauss nodes, N = 4
+    everywhere or per-element orders on {2..6} with mortar faces; Taylor-Green + 2e-2 velocity noise,
+    Mach 0.3, Re 900, Roe + BR1 + Smagorinsky.  QDot (<= 1e-12 normwise) and three RK3 steps with
+    the device's own dt against the oracle."""
+    import os
+    from horses3d_amd import specmesh
+    from horses3d_amd.engine import read_specmesh
+    path = os.path.join(str(tmp_path), "box.mesh")
+    specmesh.write_specmesh(path, 4, 0.1, mesh.box_phases(3), bface_order=6)
+    orders = mesh.random_orders(64, 2, 6, 7) if mixed else None
+    fc = read_specmesh(path, N=4, orders=orders, periodic=dict(specmesh.PERIODIC_PAIRS), nodes="gauss")
+    fc["fNf"], fc["foff"] = _face_orders(fc)
+    assert fc["foff"][-1] == len(fc["fjac"])
+    kw = dict(mach=0.3, reynolds=900.0, riemann="roe", inviscid="standard", averaging="standard",
+              les="smagorinsky", les_c=0.2, nodes="gauss")
+    print("gauss", "mixed" if mixed else "uniform", fc["K"], len(fc["J"]), flush=True)
+    Q0 = mesh.taylor_green(fc, 0.3, noise=0.02, seed=5)
+    eng, ora = engine_for(fc, kw), oracle_for(fc, kw)
+    eng.set_state(Q0)
+    err = po.normwise_error(eng.qdot(), ora.qdot(Q0))
+    assert err.max() <= TOL, err
+    t = eng.advance(3, 0.4, 0.4, t=0.0)
+    Q, G, to = Q0.copy(), np.zeros_like(Q0), 0.0
+    for _ in range(3):
+        dt = ora.max_dt(Q, 0.4, 0.4)
+        ora.rk3_step(Q, G, to, dt)
+        to += dt
+    assert t == pytest.approx(to, rel=1e-13)
+    assert po.normwise_error(eng.get_state() - Q0, Q - Q0).max() <= TOL
     eng.close()
 
 
@@ -583,7 +637,7 @@ def test_control_file_to_qdot(name, tmp_path):
     eng.close()
 
 
-@pytest.mark.parametrize("name", ["box2_p3_curved", "box2_mixed_p2to4"])
+@pytest.mark.parametrize("name", ["box2_p3_curved", "box2_mixed_p2to4", "box2_p3_curved_gauss"])
 def test_restart_from_reference_solution_file(name, tmp_path):
     """A reference run's mesh file and solution file (.hsol) into the engine: the loaded state's
     QDot against the reference's at that instant, and the state saved again is the reference's
@@ -599,9 +653,11 @@ def test_restart_from_reference_solution_file(name, tmp_path):
     specmesh.write_specmesh(path, spec["n"], spec["amp"], mesh.box_phases(spec["seed"]),
                             bface_order=max(4, spec["N"]))
     mixed = spec["orders"] is not None
-    case = read_specmesh(path, N=spec["N"], orders=z["N"] if mixed else None, periodic=dict(specmesh.PERIODIC_PAIRS))
+    nodes = spec.get("nodes", "Gauss-Lobatto").lower()
+    case = read_specmesh(path, N=spec["N"], orders=z["N"] if mixed else None, periodic=dict(specmesh.PERIODIC_PAIRS),
+                         nodes=nodes)
     eng = engine_for(case, dict(mach=spec["mach"], reynolds=spec["re"], riemann="roe", inviscid="standard",
-                                averaging="standard", les=None, les_c=None))
+                                averaging="standard", les=None, les_c=None, nodes=nodes))
     it, t = eng.load_solution(os.path.join(hs, name + "_final.hsol"))
     assert it == int(z["steps"]) and t == float(z["t_final"])
     err = po.normwise_error(eng.qdot(), z["QDot_final"])

@@ -1,7 +1,7 @@
 """Solution files (.hsol) against files the reference solver wrote.
 
-tests/golden/hsol holds .hsol files from two reference runs (a curved uniform-order box and a
-box with per-element orders), made by make_hsol.py, with the state the reference's ProblemFile
+tests/golden/hsol holds .hsol files from three reference runs (a curved uniform-order box, a
+box with per-element orders, and a curved box on Gauss nodes), made by make_hsol.py, with the state the reference's ProblemFile
 dump held at the same instant.  Reading them must give that state bit for bit, and writing the
 same state with the same header must give the same bytes.  Host only (ctypes, no GPU)."""
 import ctypes as C
@@ -13,7 +13,8 @@ import pytest
 from horses3d_amd import engine as E
 
 HS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hsol")
-CASES = ["box2_p3_curved", "box2_mixed_p2to4"]
+CASES = ["box2_p3_curved", "box2_mixed_p2to4", "box2_p3_curved_gauss"]
+NODE_TYPE = {"box2_p3_curved_gauss": 1}   # the file's node type field: 1 Gauss, 2 Gauss-Lobatto
 
 
 def lib():
@@ -49,7 +50,7 @@ def test_read_reference_files(name):
     for tag, it in (("step0", 0), ("final", int(z["steps"]))):
         p = os.path.join(HS, "%s_%s.hsol" % (name, tag))
         t, nt, k, iters, tm, refs = header(p)
-        assert (t, nt, k, iters) == (2, 2, len(z["N"]), it)
+        assert (t, nt, k, iters) == (2, NODE_TYPE.get(name, 2), len(z["N"]), it)
         assert tm == float(z["t_" + tag])
         # the default reference state of the physics setup, bit for bit
         assert np.array_equal(refs, np.array(ref[:]))
@@ -69,8 +70,9 @@ def test_write_reproduces_reference_bytes(name, tmp_path):
     out = os.path.join(str(tmp_path), "w.hsol")
     R = (C.c_double * 6)(*refs)
     # the reference stores the name it was given: RESULTS/case.hsol
-    rc = L.hx3d_hsol_write(os.fsencode(out), b"RESULTS/case.hsol", len(N), N.ctypes.data_as(C.c_void_p), it,
-                           C.c_double(tm), R, Q.ctypes.data_as(C.c_void_p), None)
+    nodes = E.NODES["gauss" if nt == 1 else "gauss-lobatto"]
+    rc = L.hx3d_hsol_write_nodes(os.fsencode(out), b"RESULTS/case.hsol", len(N), N.ctypes.data_as(C.c_void_p), it,
+                                 C.c_double(tm), R, Q.ctypes.data_as(C.c_void_p), None, nodes)
     assert rc == 0, L.hx3d_hsol_error()
     assert open(out, "rb").read() == open(src, "rb").read()
 

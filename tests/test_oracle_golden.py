@@ -19,7 +19,8 @@ def make_oracle(case, meta):
     kw = physics_kwargs(meta)
     return po.Oracle(case, po.make_params(kw["mach"], kw["reynolds"], riemann=kw["riemann"],
                                           inviscid=kw["inviscid"], averaging=kw["averaging"],
-                                          les=kw["les"], les_c=kw["les_c"]))
+                                          les=kw["les"], les_c=kw["les_c"],
+                                          nodes=kw.get("nodes", "gauss-lobatto")))
 
 
 def test_qdot_matches_reference(golden):

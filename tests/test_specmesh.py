@@ -5,7 +5,8 @@ file (Read_SpecMesh.f90, HexMesh.f90, MappedGeometry.f90): element faces and sid
 elements, slots and rotations, node coordinates, metrics Ja, Jacobians, face normals, tangents
 and surface Jacobians.  The same SpecMesh file, written again from the fixture's parameters,
 read by the C front end must give the same integers and the same doubles bit for bit (straight,
-curved with face-patch projection, and p-anisotropic meshes with mortar faces).  Host only."""
+curved with face-patch projection, and p-anisotropic meshes with mortar faces; geometry on
+Gauss-Lobatto and on Gauss nodes).  Host only."""
 import ast
 import os
 
@@ -35,7 +36,8 @@ def write_fixture_mesh(name, d):
 @pytest.mark.parametrize("name", FIXTURES)
 def test_front_end_matches_reference_bitwise(name, tmp_path):
     z, meta, path, orders = write_fixture_mesh(name, tmp_path)
-    c = E.read_specmesh(path, N=meta["order"], orders=orders, periodic=dict(specmesh.PERIODIC_PAIRS))
+    c = E.read_specmesh(path, N=meta["order"], orders=orders, periodic=dict(specmesh.PERIODIC_PAIRS),
+                        nodes=meta.get("nodes", "Gauss-Lobatto"))
     assert c["K"] == int(z["K"]) and c["F"] == int(z["F"])
     for k in INTS:
         assert np.array_equal(np.asarray(c[k]).reshape(z[k].shape), z[k]), k
