@@ -46,6 +46,57 @@ def test_derivative_matrices(N):
     assert np.allclose(mesh.deriv_matrix(x), D, atol=1e-13)
 
 
+@pytest.mark.parametrize("N", ORDERS)
+def test_gauss_nodes_and_operators_closed_form(N):
+    """Gauss(-Legendre) NodalStorage (GaussLegendreNodesAndWeights, LegendreAlgorithms.f90:137-214):
+    nodes the roots of P_{N+1}, weights 2 / ((1 - x^2) P'_{N+1}(x)^2), the rule exact for degree
+    2N + 1; D exact on degree <= N, hatD = W^-1 D^T W; the boundary vectors the Lagrange basis at
+    -1 and +1 (exact on degree <= N, dense: no node on the boundary), b = v / w; the same nodes in
+    the product's host code (through the front end's geometry pins)."""
+    op = po.nodal_gauss(N)
+    x, w, D = op["x"], op["w"], op["D"]
+    c = np.zeros(N + 2); c[N + 1] = 1.0
+    x_ref = np.sort(npl.legroots(c))
+    w_ref = 2.0 / ((1.0 - x_ref ** 2) * npl.legval(x_ref, npl.legder(c)) ** 2)
+    assert np.allclose(x, x_ref, atol=4e-15, rtol=0) and np.allclose(w, w_ref, atol=4e-15, rtol=0)
+    assert np.array_equal(x, -x[::-1]) and np.array_equal(w, w[::-1])
+    for p in range(2 * N + 2):
+        exact = 2.0 / (p + 1) if p % 2 == 0 else 0.0
+        assert abs(w @ x ** p - exact) < 2e-14, p
+    for p in range(N + 1):
+        assert np.allclose(D @ x ** p, p * x ** max(p - 1, 0) if p else 0 * x, atol=1e-12)
+        assert abs(op["vL"] @ x ** p - (-1.0) ** p) < 1e-13 and abs(op["vR"] @ x ** p - 1.0) < 1e-13
+    assert np.allclose(op["hatD"], np.diag(1 / w) @ D.T @ np.diag(w), atol=1e-13)
+    assert np.count_nonzero(op["vL"]) == N + 1 and np.count_nonzero(op["vR"]) == N + 1
+    assert np.allclose(op["bL"], op["vL"] / w, rtol=1e-15) and np.allclose(op["bR"], op["vR"] / w, rtol=1e-15)
+    # summation by parts on Gauss nodes holds with the boundary vectors: W D + D^T W = vR vR^T - vL vL^T
+    W = np.diag(w)
+    assert np.allclose(W @ D + D.T @ W, np.outer(op["vR"], op["vR"]) - np.outer(op["vL"], op["vL"]), atol=1e-12)
+
+
+@pytest.mark.parametrize("N,M", [(2, 5), (4, 7), (5, 3), (7, 2)])
+def test_tset_gauss_interpolation_and_projection(N, M):
+    """Mortar matrices on Gauss nodes (InterpolationMatrix_Construct on the case's nodes)."""
+    T = po.tset(N, M, nodes="gauss")
+    gN, gM = po.nodal_gauss(N), po.nodal_gauss(M)
+    xN, wN, xM, wM = gN["x"], gN["w"], gM["x"], gM["w"]
+    if N <= M:
+        for p in range(N + 1):
+            assert np.allclose(T @ xN ** p, xM ** p, atol=1e-13)
+    else:
+        Tin = po.tset(M, N, nodes="gauss")
+        assert np.allclose(T, (Tin * wN[:, None]).T / wM[:, None], atol=1e-13)
+        for p in range(M + 1):
+            assert abs(wM @ (T @ xN ** p) - wN @ xN ** p) < 1e-12
+    assert not np.allclose(T, po.tset(N, M))   # not the Gauss-Lobatto matrix
+
+
+def test_oracle_rejects_split_form_on_gauss_nodes():
+    case, _ = load_golden([p for p in GOLDEN if "g1_tgv" in p][0])
+    with pytest.raises(ValueError, match="Gauss-Lobatto"):
+        po.Oracle(case, po.make_params(0.1, 1600.0, inviscid="split-form", averaging="pirozzoli", nodes="gauss"))
+
+
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_chebyshev_gauss_lobatto_operators(N):
     """The Chebyshev-Gauss-Lobatto framework of NodalStorage (NodalStorageClass.f90:282-296):
