@@ -120,6 +120,14 @@ __device__ __forceinline__ long slot_node(const int *N3, const long *off, int e,
 // face) and Tset(n0 -> N0s), Tset(n1 -> N1s) (face to side), one entry per lane: the
 // interpolation loops then read them at LDS latency instead of through the cache from the
 // operator table.
+// One side's mortar matrices as staged by stage_mortar: a0, a1 side to face along the two face
+// indices, p0, p1 face to side.  (Reading them from the operator table through the cache
+// instead, which frees 4 KB of LDS per face, measured 19% slower in km_face.)
+struct Mort {
+    const double *a0, *a1, *p0, *p1;
+};
+__device__ __forceinline__ Mort mort_lds(const double (*m)[64]) { return Mort{m[0], m[1], m[2], m[3]}; }
+
 __device__ __forceinline__ void stage_mortar(const OpsAll &ops, const FaceRec &r, double (*m)[4][64], int lane)
 {
 #pragma unroll
@@ -137,7 +145,7 @@ __device__ __forceinline__ void stage_mortar(const OpsAll &ops, const FaceRec &r
 // source node's weight is formed once, and every value accumulates in the reference's order
 // with the reference's products.
 template <int K>
-__device__ __forceinline__ void adapt_k(const double (*m)[64], const FaceRec &r, int side, const double *tr,
+__device__ __forceinline__ void adapt_k(const Mort &m, const FaceRec &r, int side, const double *tr,
                                         int i, int j, double *out)
 {
     const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1], n0 = r.Nf[0];
@@ -147,7 +155,7 @@ __device__ __forceinline__ void adapt_k(const double (*m)[64], const FaceRec &r,
         for (int k = 0; k < K; ++k) out[k] = tr[k * 64 + i + (n0 + 1) * j];
         return;
     }
-    const double *T0 = m[0], *T1 = m[1];
+    const double *T0 = m.a0, *T1 = m.a1;
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = 0.0;
     if (pt == 1) {
@@ -177,7 +185,7 @@ __device__ __forceinline__ void adapt_k(const double (*m)[64], const FaceRec &r,
 
 // Face_ProjectFluxToElements for K values at once (values k at fl[k*64 + q], q = m + (Nf0+1) l)
 template <int K>
-__device__ __forceinline__ void project_k(const double (*m)[64], const FaceRec &r, int side, const double *fl,
+__device__ __forceinline__ void project_k(const Mort &m, const FaceRec &r, int side, const double *fl,
                                           int i, int j, double *out)
 {
     const int n0 = r.Nf[0], n1 = r.Nf[1];
@@ -187,7 +195,7 @@ __device__ __forceinline__ void project_k(const double (*m)[64], const FaceRec &
         for (int k = 0; k < K; ++k) out[k] = fl[k * 64 + i + (n0 + 1) * j];
         return;
     }
-    const double *T0 = m[2], *T1 = m[3];
+    const double *T0 = m.p0, *T1 = m.p1;
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = 0.0;
     if (pt == 1) {
@@ -265,11 +273,11 @@ __device__ __forceinline__ void interp2_k(const double *T0, const double *T1, in
 // volume integral, and factorising those measured 1.7e-12 on the 20^3 p-anisotropic case.
 // Uniform per face: every lane of the wave calls it.
 template <int K, bool FACT>
-__device__ __forceinline__ void adapt_face(const double (*m)[64], const FaceRec &r, int side, const double *tr,
+__device__ __forceinline__ void adapt_face(const Mort &m, const FaceRec &r, int side, const double *tr,
                                            double *tmp, int lane, int i, int j, bool on, double *out)
 {
     if (FACT && r.proj[side] == 3) {
-        interp2_k<K>(m[0], m[1], r.Nfs[side][0], r.Nfs[side][1], r.Nf[0], r.Nf[1], tr, tmp, lane, i, j, on, out);
+        interp2_k<K>(m.a0, m.a1, r.Nfs[side][0], r.Nfs[side][1], r.Nf[0], r.Nf[1], tr, tmp, lane, i, j, on, out);
     } else if (on) {
         adapt_k<K>(m, r, side, tr, i, j, out);
     } else {
@@ -279,11 +287,11 @@ __device__ __forceinline__ void adapt_face(const double (*m)[64], const FaceRec 
 }
 
 template <int K, bool FACT>
-__device__ __forceinline__ void project_face(const double (*m)[64], const FaceRec &r, int side, const double *fl,
+__device__ __forceinline__ void project_face(const Mort &m, const FaceRec &r, int side, const double *fl,
                                              double *tmp, int lane, int i, int j, bool on, double *out)
 {
     if (FACT && r.proj[side] == 3) {
-        interp2_k<K>(m[2], m[3], r.Nf[0], r.Nf[1], r.Nfs[side][0], r.Nfs[side][1], fl, tmp, lane, i, j, on, out);
+        interp2_k<K>(m.p0, m.p1, r.Nf[0], r.Nf[1], r.Nfs[side][0], r.Nfs[side][1], fl, tmp, lane, i, j, on, out);
     } else if (on) {
         project_k<K>(m, r, side, fl, i, j, out);
     } else {
@@ -362,6 +370,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
+    const Mort M[2] = {mort_lds(sM[0]), mort_lds(sM[1])};
     gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, sT[0], lane, opsp);
     gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, sT[1], lane, opsp);
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
@@ -375,8 +384,8 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     const double jac = fgeo[9 * FSm + fq];
     __syncthreads();
     double uS[NV], qL[NV], qR[NV];
-    adapt_face<NV, true>(sM[0], r, 0, sT[0], sTmp, lane, i, j, on, qL);
-    adapt_face<NV, true>(sM[1], r, 1, sT[1], sTmp, lane, i, j, on, qR);
+    adapt_face<NV, true>(M[0], r, 0, sT[0], sTmp, lane, i, j, on, qL);
+    adapt_face<NV, true>(M[1], r, 1, sT[1], sTmp, lane, i, j, on, qR);
 #pragma unroll
     for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
     // unStar(v) n_d, one direction at a time, projected onto both sides: lift[(5d + v)*ES + x]
@@ -392,7 +401,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
             const bool son = lane < (N0 + 1) * (N1 + 1);
             const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
             double lv[NV];
-            project_face<NV, true>(sM[side], r, side, sU, sTmp, lane, si, sj, son, lv);
+            project_face<NV, true>(M[side], r, side, sU, sTmp, lane, si, sj, son, lv);
             if (son) {
                 const long x = elem_face_node(r, side, efoff, si, sj);
 #pragma unroll
@@ -562,6 +571,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     if (f >= F) return;   // uniform per block
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
+    const Mort M[2] = {mort_lds(sM[0]), mort_lds(sM[1])};
     gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, sQ[0], lane, opsp);
     gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, sQ[1], lane, opsp);
     if (!ph.euler) gather_trace_k<3 * NV, GAUSS>(r, 0, N3, off, U, S, sG, lane, opsp);
@@ -577,8 +587,8 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     for (int c = 0; c < 3; ++c) nrm[c] = fgeo[c * FSm + fq];
     const double jac = fgeo[9 * FSm + fq];
     double qs[2][NV];
-    adapt_face<NV, false>(sM[0], r, 0, sQ[0], sTmp, lane, i, j, on, qs[0]);
-    adapt_face<NV, false>(sM[1], r, 1, sQ[1], sTmp, lane, i, j, on, qs[1]);
+    adapt_face<NV, false>(M[0], r, 0, sQ[0], sTmp, lane, i, j, on, qs[0]);
+    adapt_face<NV, false>(M[1], r, 1, sQ[1], sTmp, lane, i, j, on, qs[1]);
     double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (!ph.euler) {
         // BR1_RiemannSolver (EllipticBR1.f90:816-868): each side's viscous flux from its
@@ -586,7 +596,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         // gradients (15 values) are interpolated five at a time (the size of sTmp)
         auto side_grad = [&](int s, double *g) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(sM[s], r, s, sG + c * NV * 64, sTmp, lane, i, j, on, g + c * NV);
+            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(M[s], r, s, sG + c * NV * 64, sTmp, lane, i, j, on, g + c * NV);
         };
         auto side_flux = [&](int s, const double *g, double (*Fs)[3]) {
             double vg[12];
@@ -640,7 +650,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         const bool son = lane < (N0 + 1) * (N1 + 1);
         const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
         double a[NV];
-        project_face<NV, false>(sM[side], r, side, sF, sTmp, lane, si, sj, son, a);
+        project_face<NV, false>(M[side], r, side, sF, sTmp, lane, si, sj, son, a);
         if (son) {
             const long x = elem_face_node(r, side, efoff, si, sj);
             if (GAUSS) {
