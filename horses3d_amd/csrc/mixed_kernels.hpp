@@ -14,7 +14,11 @@
 //   km_volume     element tiles: volume integral, surface integral, 1/J, RK update and
 //                 the next step's dt limits.
 // The element kernels are persistent: one launch per stage sweeps every element, each block
-// taking tiles (elements packed into at most 512 nodes, one thread per node) in turn.
+// taking tiles (elements packed into at most 512 nodes, one thread per node) in turn, each
+// thread reading what it works on from the tiles' node map.
+// The same kernels run Gauss-node cases (GAUSS template flag, one order or per-element
+// orders): the traces are the Lagrange basis at the face's end applied along the normal
+// direction, and the BR1 lift and the surface integral reach every node with b = v(+-1)/w.
 #pragma once
 #include <hip/hip_runtime.h>
 
