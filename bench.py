@@ -176,6 +176,60 @@ def mixed_bench(args):
                                              "%d nodes, TGV + 1e-3 noise, Mach 0.2, Re 1600, Roe + BR1" % nd}}))
 
 
+def gauss_bench(args):
+    """Gauss nodes (the reference's default node type): the 16^3 x N=4 curvilinear box
+    (amplitude 0.1) built by the SpecMesh front end on Gauss nodes, standard form, Roe + BR1, run
+    on the face-trace kernels; the same case on Gauss-Lobatto nodes (the uniform-order kernels)
+    beside it.  One JSON line."""
+    import tempfile
+    import torch
+    from horses3d_amd import mesh, specmesh
+    from horses3d_amd.engine import Engine, read_specmesh
+    steps = 30 if args.steps == 200 else args.steps
+    d = tempfile.mkdtemp(prefix="hx3d_gauss_")
+    path = os.path.join(d, "box.mesh")
+    specmesh.write_specmesh(path, 16, 0.1, mesh.box_phases(2), bface_order=4)
+    out = {}
+    for nodes in ("gauss", "gauss-lobatto"):
+        print("[bench] 16^3 x N=4 on %s nodes" % nodes, file=sys.stderr, flush=True)
+        case = read_specmesh(path, N=4, periodic=dict(specmesh.PERIODIC_PAIRS), nodes=nodes)
+        Q0 = mesh.taylor_green(case, 0.1)
+        nd = len(Q0)
+        eng = Engine(case, mach=0.1, reynolds=1600.0, riemann="roe", nodes=nodes)
+        del case
+        eng.set_state(Q0)
+        if nodes == "gauss":
+            eng.autotune(reps=5)
+        t_cw = time.perf_counter()
+        while time.perf_counter() - t_cw < args.clock_warmup_s:
+            eng.set_state(Q0)
+            eng.advance(10, 0.4, 0.4, t=0.0)
+            eng.synchronize()
+        eng.set_state(Q0)
+        eng.advance(args.warmup, 0.4, 0.4, t=0.0)
+        eng.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.advance(steps, 0.4, 0.4)
+        eng.synchronize()
+        el = time.perf_counter() - t0
+        assert not eng.has_nan()
+        eng.set_timing(True)
+        eng.advance(steps, 0.4, 0.4)
+        kt = eng.kernel_times()
+        eng.close()
+        out[nodes] = dict(value=nd * steps / el, ms_per_step=el / steps * 1e3,
+                          stage_ms={"gradient": kt[0], "face flux": kt[1], "volume": kt[2], "stage": kt[3]})
+    os.remove(path)
+    os.rmdir(d)
+    g = out["gauss"]
+    print(json.dumps({"metric": "DOF-updates/s (FP64)", "value": g["value"], "unit": "DOF-updates/s",
+                      "steps": steps, "ms_per_step": g["ms_per_step"], "dtype": "f64", "stage_ms": g["stage_ms"],
+                      "gauss_lobatto_same_case": out["gauss-lobatto"],
+                      "config": {"workload": "16^3 curvilinear hexes (amplitude 0.1) x N=4 on Gauss nodes (%d nodes), "
+                                             "standard form, Roe + BR1, TGV Mach 0.1 Re 1600, LS-RK3, CFL 0.4" % nd}}))
+
+
 def batch_bench(args):
     """Many small cases: one engine over their union against one engine per case, same steps."""
     import numpy as np
@@ -278,6 +332,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 5 times (the "
                          "known-size reference for calibrating FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--gauss", action="store_true",
+                    help="Gauss nodes: the 16^3 x N=4 curvilinear box from the SpecMesh front end on Gauss nodes "
+                         "(standard form, Roe + BR1) against the same case on Gauss-Lobatto nodes")
     ap.add_argument("--largest", action="store_true",
                     help="the largest single-GPU case instead: 48^3 curvilinear periodic hexes (amplitude 0.15) x N=5 (23.9 M "
                          "nodes), split-form Kennedy-Gruber + Roe + BR1 + WALE, 20 timed steps, no CPU baseline "
@@ -338,6 +395,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.gauss:
+        gauss_bench(args)
         return
     if args.largest:
         args.n, args.order, args.no_cpu_baseline = 48, 5, True

@@ -333,10 +333,20 @@ __device__ __forceinline__ void gather_trace_k(const FaceRec &r, int side, const
             const long stride = ax == 0 ? 1 : (ax == 1 ? (Ne[0] + 1) : (long)(Ne[0] + 1) * (Ne[1] + 1));
             const long g0 = M_SLOT_PLUS[s] ? g - (long)Na * stride : g;
             const double *vv = M_SLOT_PLUS[s] ? ops->vR[Na] : ops->vL[Na];
+            double w[MAXN1];
+#pragma unroll
+            for (int l = 0; l < MAXN1; ++l) w[l] = l <= Na ? vv[l] : 0.0;
+            // the line's loads all issued before the sum (a loop to Na would wait for each load in
+            // turn); lanes past the line's end load nothing and add nothing
 #pragma unroll
             for (int c = 0; c < K; ++c) {
+                double x[MAXN1];
+#pragma unroll
+                for (int l = 0; l < MAXN1; ++l) x[l] = l <= Na ? src[c * S + g0 + l * stride] : 0.0;
                 double acc = 0.0;
-                for (int l = 0; l <= Na; ++l) acc = acc + src[c * S + g0 + l * stride] * vv[l];
+#pragma unroll
+                for (int l = 0; l < MAXN1; ++l)
+                    if (l <= Na) acc = acc + x[l] * w[l];
                 tr[c * 64 + lane] = acc;
             }
         }
