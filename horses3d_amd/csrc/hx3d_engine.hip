@@ -39,6 +39,7 @@
 #include "physics.hpp"
 #include "stage_kernels.hpp"
 #include "mixed_kernels.hpp"
+#include "gauss_kernels.hpp"
 #include "spectral.hpp"
 
 using hx::NV;
@@ -456,6 +457,10 @@ struct hx3d_engine {
     hipStream_t stream = nullptr;
     double *dQ = nullptr, *dG = nullptr, *dVgrad = nullptr, *dJa = nullptr;
     double *dFe = nullptr, *dQDot = nullptr, *dEdelta = nullptr, *dFdelta = nullptr, *dWq = nullptr;
+    // Gauss nodes, one order: Q and gradient traces in element-face order, the neighbour table
+    double *dQtr = nullptr, *dGtr = nullptr;
+    int *dNbx = nullptr;
+    hxg::GaussOps gops{};
     double *dJ = nullptr, *dFgeo = nullptr, *dEgeo = nullptr;
     double *dScal = nullptr, *dPart = nullptr, *dStage = nullptr, *dDtPart = nullptr;
     unsigned long long *dBits = nullptr;
@@ -463,7 +468,7 @@ struct hx3d_engine {
     int4 *dFnode = nullptr;
     // mixed orders (p-anisotropic elements, mortar faces)
     bool mixed = false;
-    bool gauss = false;   // Gauss nodes (always on the face-trace kernels of the mixed path)
+    bool gauss = false;   // Gauss nodes (one order: hxg kernels; per-element orders: the mixed path's)
     long FSm = 0;
     int *dN3 = nullptr, *dEfoff = nullptr, *dEofn = nullptr;
     long *dOff = nullptr;
@@ -545,7 +550,28 @@ int elem_grid(const hx3d_engine *eng, int epb)
     if (eng->mc.blocks) return grid_for(eng->nblocks_multi);
     return grid_for((eng->K + epb - 1) / epb);
 }
-int grad_grid(const hx3d_engine *eng) { return elem_grid(eng, epb_for(eng->N)); }
+int grad_grid(const hx3d_engine *eng) { return elem_grid(eng, eng->gauss ? 1 : epb_for(eng->N)); }
+
+// Gauss nodes, one order: the Q traces of the resident state, after it is set from outside
+template <int N>
+void launch_prolong_n(hx3d_engine *eng)
+{
+    hipLaunchKernelGGL((hxg::kg_prolong<N>), dim3(elem_grid(eng, 1)), dim3(hxg::gauss_tpb<N>()), 0, eng->stream,
+                       eng->dQ, eng->dQtr, eng->K, eng->S, eng->ES, eng->gops);
+}
+void refresh_traces(hx3d_engine *eng)
+{
+    if (!eng->gauss || eng->mixed) return;
+    switch (eng->N) {
+    case 1: launch_prolong_n<1>(eng); break;
+    case 2: launch_prolong_n<2>(eng); break;
+    case 3: launch_prolong_n<3>(eng); break;
+    case 4: launch_prolong_n<4>(eng); break;
+    case 5: launch_prolong_n<5>(eng); break;
+    case 6: launch_prolong_n<6>(eng); break;
+    default: launch_prolong_n<7>(eng); break;
+    }
+}
 
 template <int N>
 void launch_grad(hx3d_engine *eng, const hxk::DtLimits &dtl)
@@ -602,6 +628,26 @@ template <int N>
 void stage_n(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot,
              hipEvent_t *ev)
 {
+    if (eng->gauss) {
+        // Gauss nodes: traces in element-face order (hxg)
+        constexpr int TPB = hxg::gauss_tpb<N>();
+        const int nb = elem_grid(eng, 1);
+        if (ev) (void)hipEventRecord(ev[0], eng->stream);
+        if (!eng->ph.euler)
+            hipLaunchKernelGGL((hxg::kg_grad<N>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dQtr, eng->dJa,
+                               eng->dJ, eng->dNbx, eng->dEgeo, eng->dGtr, eng->dVgrad, eng->K, eng->S, eng->ES,
+                               eng->ops, eng->gops, eng->ph, dta.lim);
+        if (ev) (void)hipEventRecord(ev[1], eng->stream);
+        const int nbf = grid_for((int)((eng->FS + hxk::FACE_TPB - 1) / hxk::FACE_TPB));
+        hipLaunchKernelGGL((hxg::kg_face<N>), dim3(nbf), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQtr, eng->dGtr,
+                           eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->ES, eng->ph, dta.red);
+        if (ev) (void)hipEventRecord(ev[2], eng->stream);
+        hipLaunchKernelGGL((hxg::kg_volume<N>), dim3(nb), dim3(TPB), 0, eng->stream, eng->dQ, eng->dQtr, eng->dVgrad,
+                           eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dG, qdot, a_k, c_k, update, dta, eng->K,
+                           eng->S, eng->ES, eng->ops, eng->gops, eng->ph);
+        if (ev) (void)hipEventRecord(ev[3], eng->stream);
+        return;
+    }
     if (ev) (void)hipEventRecord(ev[0], eng->stream);
     if (!eng->ph.euler) launch_grad<N>(eng, dta.lim);
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
@@ -1132,9 +1178,10 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     eng->mach = d->mach;
     // descriptor checks come first (they need no device); the device is recorded from here on
     if (hipGetDevice(&eng->device) != hipSuccess) return bail(eng->fail(HX3D_EDEVICE, "no HIP device"));
-    // per-element orders, and Gauss nodes (whose face values are interpolated, not copied), take
-    // the face-trace kernels
-    if (!uniform || d->nodes == HX3D_NODES_GAUSS) return bail(create_mixed(eng, d));
+    // per-element orders take the mortar (face-trace) kernels; Gauss nodes of one order the
+    // uniform layout with the hxg kernels
+    if (!uniform) return bail(create_mixed(eng, d));
+    eng->gauss = d->nodes == HX3D_NODES_GAUSS;
     eng->K = d->K; eng->N = N; eng->n1 = N + 1; eng->nf = eng->n1 * eng->n1; eng->ne = eng->nf * eng->n1;
     eng->F = d->F;
     eng->nodes = (long)eng->K * eng->ne;
@@ -1142,11 +1189,19 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     eng->FS = (long)eng->F * eng->nf;
     eng->ES = (((long)eng->K * 6 * eng->nf + 63) / 64) * 64;
     eng->inviscid = d->inviscid;
-    eng->ops = make_ops(N);
+    eng->ops = make_ops(N, eng->gauss);
     {
-        double x[8], w[8];
-        gl_nodes(N, x, w);
+        double x[8], w[8], wb[8];
+        hxs::nodes_of(eng->gauss, N, x, w);
         eng->dcsi = 1.0 / std::fabs(x[1] - x[0]);
+        // v(-1), v(+1), b = v / w (NodalStorageClass.f90:257-273)
+        hxs::bary(N, x, wb);
+        hxs::interp_vector(-1.0, N, x, wb, eng->gops.vL);
+        hxs::interp_vector(1.0, N, x, wb, eng->gops.vR);
+        for (int i = 0; i <= N; ++i) {
+            eng->gops.bL[i] = eng->gops.vL[i] / w[i];
+            eng->gops.bR[i] = eng->gops.vR[i] / w[i];
+        }
     }
 
     const int K = eng->K, n1 = eng->n1, nf = eng->nf, ne = eng->ne, F = eng->F;
@@ -1188,13 +1243,22 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
     }
     for (size_t x = 0; x < nbr.size(); ++x)
         if (nbr[x] < 0) return bail(eng->fail(HX3D_EINVAL, "an element face node has no neighbour"));
+    // Gauss nodes: 2 * (the neighbour's matching element face node) + (1 on the face's right side)
+    std::vector<int> nbx;
+    if (eng->gauss) {
+        nbx.assign(nbr.size(), 0);
+        for (long q = 0; q < FS; ++q) {
+            nbx[fnode[q].z] = 2 * fnode[q].w;
+            nbx[fnode[q].w] = 2 * fnode[q].z + 1;
+        }
+    }
     if (!offsets_fit(S, ES, FS))
         return bail(eng->fail(HX3D_EUNSUPPORTED, "mesh too large: more than 2^29 - 1 nodes (or face nodes) per array"));
     // geometry SoA, J, 1/J, quadrature weights, LES filter widths, face frames
     std::vector<double> ja((size_t)9 * S, 0.0), jj((size_t)S, 1.0), wq((size_t)S, 0.0),
         edelta(K), fdelta(F), fgeo((size_t)10 * FS, 0.0), egeo((size_t)4 * ES, 0.0);
     double xg[8], wg[8];
-    gl_nodes(N, xg, wg);
+    hxs::nodes_of(eng->gauss, N, xg, wg);
     for (long g = 0; g < nodes; ++g) {
         for (int c = 0; c < 9; ++c) ja[(size_t)c * S + g] = d->Ja[9 * g + c];
         jj[g] = d->J[g];
@@ -1270,6 +1334,9 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         (rc = dalloc(eng, &eng->dNbr, nbr.size())) ||
         (rc = dalloc(eng, &eng->dFlag, 2)) || (rc = dalloc(eng, &eng->dFnode, fnode.size())))
         return bail(rc);
+    if (eng->gauss && ((rc = dalloc(eng, &eng->dQtr, (size_t)NV * ES)) || (rc = dalloc(eng, &eng->dGtr, (size_t)15 * ES)) ||
+                       (rc = dalloc(eng, &eng->dNbx, nbx.size()))))
+        return bail(rc);
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(eng->fail(HX3D_EDEVICE, "hipStreamCreate failed"));
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -1279,13 +1346,18 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
         !up(eng->dJ, jj.data(), jj.size() * 8) || !up(eng->dFgeo, fgeo.data(), fgeo.size() * 8) ||
         !up(eng->dWq, wq.data(), wq.size() * 8) || !up(eng->dEdelta, edelta.data(), edelta.size() * 8) ||
         !up(eng->dFdelta, fdelta.data(), fdelta.size() * 8) || !up(eng->dNbr, nbr.data(), nbr.size() * 4) ||
-        !up(eng->dEgeo, egeo.data(), egeo.size() * 8) || !up(eng->dFnode, fnode.data(), fnode.size() * 16))
+        !up(eng->dEgeo, egeo.data(), egeo.size() * 8) || !up(eng->dFnode, fnode.data(), fnode.size() * 16) ||
+        (eng->gauss && !up(eng->dNbx, nbx.data(), nbx.size() * 4)))
         return bail(eng->fail(HX3D_EDEVICE, "upload failed"));
     // zero state, RK register and time
     const long nb = (NV * S + 255) / 256;
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, (long)NV * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dG, (long)NV * S, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, eng->stream, eng->dScal, 4L, 0.0);
+    if (eng->gauss) {
+        hipLaunchKernelGGL(k_fill, dim3((15 * ES + 255) / 256), dim3(256), 0, eng->stream, eng->dGtr, 15L * ES, 0.0);
+        refresh_traces(eng);
+    }
     if (int rc2 = init_flags(eng)) return bail(rc2);
     if (hipStreamSynchronize(eng->stream) != hipSuccess)
         return bail(eng->fail(HX3D_EDEVICE, "initialisation failed"));
@@ -1445,7 +1517,8 @@ int hx3d_engine_destroy(hx3d_engine *eng)
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff,
                     eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart, eng->dTmap,
-                    eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, eng->dTimeM};
+                    eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, eng->dTimeM,
+                    eng->dQtr, eng->dGtr, eng->dNbx};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : eng->evpool) (void)hipEventDestroy(e);
@@ -1461,6 +1534,7 @@ int hx3d_set_state_device(hx3d_engine *eng, const double *Qd)
     if (!eng || !Qd) return HX3D_EINVAL;
     const long nb = (eng->nodes + 255) / 256;
     hipLaunchKernelGGL(k_pack_in, dim3(nb), dim3(256), 0, eng->stream, Qd, eng->dQ, eng->nodes, eng->S);
+    refresh_traces(eng);   // Gauss nodes: the new state's face traces
     // a new state restarts the RK register (the reference allocates G_NS zeroed)
     const long nb2 = (NV * eng->S + 255) / 256;
     hipLaunchKernelGGL(k_fill, dim3(nb2), dim3(256), 0, eng->stream, eng->dG, (long)NV * eng->S, 0.0);

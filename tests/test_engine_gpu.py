@@ -681,6 +681,41 @@ def test_gauss_nodes_steps_match_oracle(mixed, tmp_path):
     eng.close()
 
 
+@pytest.mark.parametrize("N,euler", [(1, False), (2, False), (3, False), (4, False), (5, False), (6, False),
+                                     (7, False), (3, True)])
+def test_gauss_every_order_matches_oracle(N, euler, tmp_path):
+    """Gauss nodes of one order (the uniform-order Gauss kernels, traces in element-face order) for
+    every order 1..7 and an Euler case: a 2^3 curvilinear box from the front end on Gauss nodes,
+    perturbed Taylor-Green state, Roe + BR1 + Smagorinsky; QDot and two RK3 steps with the
+    device's dt against the oracle."""
+    import os
+    from horses3d_amd import specmesh
+    from horses3d_amd.engine import Engine, read_specmesh
+    path = os.path.join(str(tmp_path), "box.mesh")
+    specmesh.write_specmesh(path, 2, 0.1, mesh.box_phases(20 + N), bface_order=max(4, N))
+    fc = read_specmesh(path, N=N, periodic=dict(specmesh.PERIODIC_PAIRS), nodes="gauss")
+    fc["fNf"], fc["foff"] = _face_orders(fc)
+    kw = dict(mach=0.3, reynolds=900.0, riemann="roe", inviscid="standard", averaging="standard",
+              les=None if euler else "smagorinsky", les_c=None if euler else 0.2, nodes="gauss")
+    eng = Engine(fc, euler=euler, **kw)
+    ora = po.Oracle(fc, po.make_params(0.3, 900.0, riemann="roe", les=kw["les"], les_c=kw["les_c"], euler=euler,
+                                       nodes="gauss"))
+    Q0 = mesh.taylor_green(fc, 0.3, noise=0.02, seed=N)
+    eng.set_state(Q0)
+    err = po.normwise_error(eng.qdot(), ora.qdot(Q0))
+    print("gauss", N, "euler" if euler else "ns", err, flush=True)
+    assert err.max() <= TOL, err
+    t = eng.advance(2, 0.4, 0.4, t=0.0)
+    Q, G, to = Q0.copy(), np.zeros_like(Q0), 0.0
+    for _ in range(2):
+        dt = ora.max_dt(Q, 0.4, 0.4)
+        ora.rk3_step(Q, G, to, dt)
+        to += dt
+    assert t == pytest.approx(to, rel=1e-13)
+    assert po.normwise_error(eng.get_state() - Q0, Q - Q0).max() <= TOL
+    eng.close()
+
+
 @pytest.mark.parametrize("name", ["syntax_split_kg_lxf", "orders_ijk_smagorinsky", "order_file_euler_pirozzoli"])
 def test_control_file_to_qdot(name, tmp_path):
     """End to end from a .control file: the control file, mesh and orders file the reference ran,
