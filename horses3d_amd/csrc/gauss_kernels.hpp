@@ -105,10 +105,13 @@ kg_grad(const double *__restrict__ Q, const double *__restrict__ Qtr, const doub
         Phys ph, hxk::DtLimits dtl)
 {
     constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1;
-    constexpr int NW = 3 * NV * (6 * nf > ne ? 6 * nf : ne);
-    __shared__ double sQ[NV * ne];
+    // one LDS space, in turn: the element's state [5][ne]; the faces' unStar [6 nf][15]; the
+    // gradient [15][ne]
+    constexpr int NW0 = 3 * NV * 6 * nf > 3 * NV * ne ? 3 * NV * 6 * nf : 3 * NV * ne;
+    constexpr int NW = NW0 > NV * ne ? NW0 : NV * ne;
+    __shared__ double sW[NW];
     __shared__ double sD[n1 * n1];
-    __shared__ double sW[NW];   // the faces' unStar [6 nf][15], then the gradient [15][ne]
+    double *sQ = sW;
     const int lb = hxk::xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
     const int e = lb;
     if (e >= K) {   // a padding block
@@ -132,23 +135,26 @@ kg_grad(const double *__restrict__ Q, const double *__restrict__ Qtr, const doub
 #pragma unroll
         for (int v = 0; v < NV; ++v) sQ[v * ne + n] = q[v];
     }
-    // unStar at the element's face nodes, the same value on both sides of a face
-    for (int f = t; f < 6 * nf; f += blockDim.x) {
-        const unsigned x = (unsigned)(6 * e) * nf + f;
-        const int c2 = ldg(nbx, x);
-        const bool right = c2 & 1;
-        const unsigned xn = (unsigned)c2 >> 1;
-        double nr[3];
+    // unStar at the element's face nodes, the same value on both sides of a face.  The P passes
+    // over the face nodes are unrolled so that every pass's index and geometry loads, then its
+    // trace loads, are issued before the first is used (here, to arrive during the gradient).
+    constexpr int TPB = gauss_tpb<N>(), NFN = 6 * nf, P = (NFN + TPB - 1) / TPB;
+    int c2[P];
+    double fg[P][4], own[P][NV], nb[P][NV];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nr[c] = ldg(egeo + c * ES, x);
-        const double fj = ldg(egeo + 3 * ES, x);
+    for (int p = 0; p < P; ++p) {
+        const int f = t + p * TPB;
+        const unsigned x = (unsigned)(6 * e) * nf + (f < NFN ? f : 0);
+        c2[p] = ldg(nbx, x);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const double own = ldg(Qtr + v * ES, x), nb = ldg(Qtr + v * ES, xn);
-            const double uS = 0.5 * (right ? own - nb : nb - own) * fj;
+        for (int c = 0; c < 4; ++c) fg[p][c] = ldg(egeo + c * ES, x);
+    }
 #pragma unroll
-            for (int d = 0; d < 3; ++d) sW[f * 3 * NV + d * NV + v] = uS * nr[d];
-        }
+    for (int p = 0; p < P; ++p) {
+        const int f = t + p * TPB;
+        const unsigned x = (unsigned)(6 * e) * nf + (f < NFN ? f : 0), xn = (unsigned)c2[p] >> 1;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { own[p][v] = ldg(Qtr + v * ES, x); nb[p][v] = ldg(Qtr + v * ES, xn); }
     }
     __syncthreads();
     double tc = INFINITY, tv = INFINITY;
@@ -181,6 +187,23 @@ kg_grad(const double *__restrict__ Q, const double *__restrict__ Qtr, const doub
             gr[1][v] = (uxi[v] * ja[1] + ueta[v] * ja[4] + uzeta[v] * ja[7]) * iJ;
             gr[2][v] = (uxi[v] * ja[2] + ueta[v] * ja[5] + uzeta[v] * ja[8]) * iJ;
         }
+    }
+    __syncthreads();   // the state has been read: the space takes the faces' unStar
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int f = t + p * TPB;
+        if (f < NFN) {
+            const bool right = c2[p] & 1;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double uS = 0.5 * (right ? own[p][v] - nb[p][v] : nb[p][v] - own[p][v]) * fg[p][3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) sW[f * 3 * NV + d * NV + v] = uS * fg[p][d];
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
         // the six faces in the order L, R (xi), FR, BK (eta), BOT, T (zeta): slots 5, 3, 0, 1, 2, 4
         const int fl = 5 * nf + j + n1 * k, fr = 3 * nf + j + n1 * k;
         const int ff = 0 * nf + i + n1 * k, fb = 1 * nf + i + n1 * k;
