@@ -473,6 +473,8 @@ struct hx3d_engine {
     int *dN3 = nullptr, *dEfoff = nullptr, *dEofn = nullptr;
     long *dOff = nullptr;
     hxm::FaceRec *dFaces = nullptr;
+    int2 *dSched = nullptr;   // the face kernels' waves (hxm::face_lane): one face, or two of one shape
+    int nwaves = 0;
     hxm::OpsAll *dOpsAll = nullptr;
     double *dU = nullptr, *dLift = nullptr;
     // mixed orders: the element tiles of the persistent element kernels (packing policy
@@ -729,17 +731,17 @@ void stage_mixed_nodes(hx3d_engine *eng, double a_k, double c_k, int update, con
     if (ev) (void)hipEventRecord(ev[0], eng->stream);
     hxk::DtLimits lim = dta.lim;
     if (!eng->ph.euler) {
-        hipLaunchKernelGGL(hxm::km_face_grad<GAUSS>, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ,
-                           eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dOpsAll, eng->dLift,
-                           eng->F, eng->S, eng->ES, eng->FSm);
+        hipLaunchKernelGGL(hxm::km_face_grad<GAUSS>, dim3(grid_for(eng->nwaves)), dim3(64), 0, eng->stream, eng->dQ,
+                           eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dSched, eng->dFgeo, eng->dOpsAll,
+                           eng->dLift, eng->nwaves, eng->S, eng->ES, eng->FSm);
         hipLaunchKernelGGL(hxm::km_grad<GAUSS>, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream,
                            eng->dTmap, eng->ntiles, eng->dQ, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
                            eng->dOpsAll, eng->dU, eng->S, eng->ES, eng->ph, lim);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
-    hipLaunchKernelGGL(hxm::km_face<GAUSS>, dim3(grid_for(eng->F)), dim3(64), 0, eng->stream, eng->dQ, eng->dU,
-                       eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dFgeo, eng->dFdelta, eng->dOpsAll,
-                       eng->dFe, eng->F, eng->S, eng->ES, eng->FSm, eng->ph, dta.red);
+    hipLaunchKernelGGL(hxm::km_face<GAUSS>, dim3(grid_for(eng->nwaves)), dim3(64), 0, eng->stream, eng->dQ, eng->dU,
+                       eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dSched, eng->dFgeo, eng->dFdelta,
+                       eng->dOpsAll, eng->dFe, eng->nwaves, eng->S, eng->ES, eng->FSm, eng->ph, dta.red);
     if (ev) (void)hipEventRecord(ev[2], eng->stream);
     if (GAUSS) launch_mvolume<hxk::FORM_STANDARD, true>(eng, a_k, c_k, update, dta, qdot);
     else if (eng->inviscid != HX3D_INVISCID_SPLIT) launch_mvolume<hxk::FORM_STANDARD>(eng, a_k, c_k, update, dta, qdot);
@@ -947,6 +949,43 @@ int set_tiles(hx3d_engine *eng, int policy)
     return HX3D_OK;
 }
 
+// The waves of the mortar face kernels (hxm::face_lane): a face of more than 32 nodes alone, a
+// smaller one with the next face of the same shape (face orders and both sides' orders in the
+// face orientation), in face order at the second face of each pair; faces left without a
+// partner alone at the end.  Every face is in exactly one wave, and each face's results depend
+// only on its own data, so any grouping gives the same bits.
+#ifndef HX3D_FACE_PAIRS
+#define HX3D_FACE_PAIRS 1
+#endif
+std::vector<int2> face_waves(const std::vector<hxm::FaceRec> &faces)
+{
+    const int F = (int)faces.size();
+    std::vector<int2> w;
+    w.reserve(F);
+    std::vector<int> pending(1 << 18, -1);   // by shape: six orders of 3 bits
+    for (int f = 0; f < F; ++f) {
+        const hxm::FaceRec &r = faces[f];
+        if (!HX3D_FACE_PAIRS || (r.Nf[0] + 1) * (r.Nf[1] + 1) > 32) {
+            w.push_back(make_int2(f, -1));
+            continue;
+        }
+        const int key = r.Nf[0] | r.Nf[1] << 3 | r.Nfs[0][0] << 6 | r.Nfs[0][1] << 9 | r.Nfs[1][0] << 12 |
+                        r.Nfs[1][1] << 15;
+        if (pending[key] < 0) {
+            pending[key] = f;
+        } else {
+            w.push_back(make_int2(pending[key], f));
+            pending[key] = -1;
+        }
+    }
+    std::vector<int> rest;
+    for (int p : pending)
+        if (p >= 0) rest.push_back(p);
+    std::sort(rest.begin(), rest.end());
+    for (int f : rest) w.push_back(make_int2(f, -1));
+    return w;
+}
+
 int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
 {
     eng->mixed = true;
@@ -1002,6 +1041,8 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         fsm += (long)(r.Nf[0] + 1) * (r.Nf[1] + 1);
     }
     eng->FSm = fsm;
+    const std::vector<int2> sched = face_waves(faces);
+    eng->nwaves = (int)sched.size();
     // operators of every order
     hxm::OpsAll oa;
     std::memset(&oa, 0, sizeof(oa));
@@ -1090,7 +1131,8 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         (rc = dalloc(eng, &eng->dFlag, 2)) || (rc = dalloc(eng, &eng->dN3, N3.size())) ||
         (rc = dalloc(eng, &eng->dOff, off.size())) || (rc = dalloc(eng, &eng->dEfoff, efoff.size())) ||
         (rc = dalloc(eng, &eng->dEofn, eofn.size())) ||
-        (rc = dalloc(eng, &eng->dFaces, faces.size())) || (rc = dalloc(eng, &eng->dOpsAll, 1)))
+        (rc = dalloc(eng, &eng->dFaces, faces.size())) || (rc = dalloc(eng, &eng->dSched, sched.size())) ||
+        (rc = dalloc(eng, &eng->dOpsAll, 1)))
         return rc;
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "hipStreamCreate failed");
@@ -1104,7 +1146,7 @@ int create_mixed(hx3d_engine *eng, const hx3d_case_desc *d)
         !up(eng->dOff, off.data(), off.size() * 8) || !up(eng->dEfoff, efoff.data(), efoff.size() * 4) ||
         !up(eng->dEofn, eofn.data(), eofn.size() * 4) ||
         !up(eng->dFaces, faces.data(), faces.size() * sizeof(hxm::FaceRec)) ||
-        !up(eng->dOpsAll, &oa, sizeof(oa)))
+        !up(eng->dSched, sched.data(), sched.size() * sizeof(int2)) || !up(eng->dOpsAll, &oa, sizeof(oa)))
         return eng->fail(HX3D_EDEVICE, "upload failed");
     const long nb = (NV * S + 255) / 256;
     hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, eng->stream, eng->dQ, (long)NV * S, 0.0);
@@ -1516,7 +1558,7 @@ int hx3d_engine_destroy(hx3d_engine *eng)
     void *ptrs[] = {eng->dQ, eng->dG, eng->dVgrad, eng->dJa, eng->dJ, eng->dFgeo, eng->dEgeo, eng->dFe,
                     eng->dQDot, eng->dEdelta, eng->dFdelta, eng->dWq, eng->dScal, eng->dPart, eng->dStage,
                     eng->dBits, eng->dNbr, eng->dFlag, eng->dFnode, eng->dN3, eng->dEfoff,
-                    eng->dEofn, eng->dOff, eng->dFaces, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart, eng->dTmap,
+                    eng->dEofn, eng->dOff, eng->dFaces, eng->dSched, eng->dOpsAll, eng->dU, eng->dLift, eng->dDtPart, eng->dTmap,
                     eng->dBlocks, eng->dCC, eng->dActive, eng->dFcase, eng->dPfirst, eng->dTimeM,
                     eng->dQtr, eng->dGtr, eng->dNbx};
     for (void *p : ptrs)

@@ -305,7 +305,7 @@ __device__ __forceinline__ void project_face(const Mort &m, const FaceRec &r, in
 }
 
 // Stage the side's element trace of K node arrays src[c*S + g] into tr[c*64 + q] in the face
-// orientation (the right side is rotated, Face_AdaptSolutionToFace).  One wave, lane = q.  On
+// orientation (the right side is rotated, Face_AdaptSolutionToFace).  Lane q of the face.  On
 // Gauss-Lobatto nodes the trace is the face's row of nodes; on Gauss nodes (GAUSS) it is the
 // Lagrange basis at the face's end of the normal direction applied along it,
 // acc = acc + value * v(l) for l = 0..N (HexElement_ProlongSolutionToFaces,
@@ -353,6 +353,26 @@ __device__ __forceinline__ void gather_trace_k(const FaceRec &r, int side, const
     }
 }
 
+// The faces of one wave of the face kernels: sched[w] = (face, -1) puts one face on all 64
+// lanes; (face a, face b) puts two faces of one shape (the same orders on both sides, so the same
+// projection types and mortar matrices, at most 32 face nodes) on the two half-waves, so that
+// small faces do not leave most of a wave idle.  Each half works in its own 32 columns of the
+// per-face LDS rows (every grid of such a face, and every half-interpolated grid, has at most
+// 32 nodes); the mortar matrices are shared.  Both halves take the same branches at every
+// barrier.
+struct FaceLane {
+    int f;     // the lane's face
+    int q;     // the lane's index within its face
+    int col;   // first LDS column of its face
+};
+__device__ __forceinline__ FaceLane face_lane(const int2 *sched, int w, int lane)
+{
+    const int2 s = sched[w];
+    const bool two = s.y >= 0;
+    const int h = two ? lane >> 5 : 0;
+    return FaceLane{h ? s.y : s.x, two ? (lane & 31) : lane, 32 * h};
+}
+
 // element face node of the side's face-orientation node (i, j)
 __device__ __forceinline__ long elem_face_node(const FaceRec &r, int side, const int *efoff, int i, int j)
 {
@@ -369,28 +389,32 @@ __device__ __forceinline__ long elem_face_node(const FaceRec &r, int side, const
 template <bool GAUSS>
 __global__ void __launch_bounds__(64)
 km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const long *__restrict__ off,
-             const int *__restrict__ efoff, const FaceRec *__restrict__ faces, const double *__restrict__ fgeo,
-             const OpsAll *__restrict__ opsp, double *__restrict__ lift, int F, long S, long ES, long FSm)
+             const int *__restrict__ efoff, const FaceRec *__restrict__ faces, const int2 *__restrict__ sched,
+             const double *__restrict__ fgeo, const OpsAll *__restrict__ opsp, double *__restrict__ lift, int W,
+             long S, long ES, long FSm)
 {
     // LDS: both sides' Q traces, then one direction's unStar in their place; the mortar
     // matrices; the half-interpolated grid of a factorised interpolation (11.5 KB)
     __shared__ double sT[2][NV * 64];
     __shared__ double sM[2][4][64];
     __shared__ double sTmp[NV * 64];
-    double *sU = sT[0];   // [NV][64], once both traces are at the face order
-    // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
+    // waves in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
-    const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
-    if (f >= F) return;   // uniform per block
-    const FaceRec r = faces[f];
+    const int w = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
+    if (w >= W) return;   // uniform per block
+    const FaceLane fl = face_lane(sched, w, lane);
+    const int ql = fl.q;
+    double *const tL = sT[0] + fl.col, *const tR = sT[1] + fl.col, *const tmp = sTmp + fl.col;
+    double *const sU = tL;   // [NV][64], once both traces are at the face order
+    const FaceRec r = faces[fl.f];
     stage_mortar(*opsp, r, sM, lane);
     const Mort M[2] = {mort_lds(sM[0]), mort_lds(sM[1])};
-    gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, sT[0], lane, opsp);
-    gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, sT[1], lane, opsp);
+    gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, tL, ql, opsp);
+    gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, tR, ql, opsp);
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
-    const bool on = lane < nq;
-    const int i = on ? lane % (n0 + 1) : 0, j = on ? lane / (n0 + 1) : 0;
-    const long fq = r.foff + (on ? lane : 0);
+    const bool on = ql < nq;
+    const int i = on ? ql % (n0 + 1) : 0, j = on ? ql / (n0 + 1) : 0;
+    const long fq = r.foff + (on ? ql : 0);
     double nrm[3];
     nrm[0] = fgeo[fq];
     nrm[1] = fgeo[FSm + fq];
@@ -398,8 +422,8 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     const double jac = fgeo[9 * FSm + fq];
     __syncthreads();
     double uS[NV], qL[NV], qR[NV];
-    adapt_face<NV, true>(M[0], r, 0, sT[0], sTmp, lane, i, j, on, qL);
-    adapt_face<NV, true>(M[1], r, 1, sT[1], sTmp, lane, i, j, on, qR);
+    adapt_face<NV, true>(M[0], r, 0, tL, tmp, ql, i, j, on, qL);
+    adapt_face<NV, true>(M[1], r, 1, tR, tmp, ql, i, j, on, qR);
 #pragma unroll
     for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
     // unStar(v) n_d, one direction at a time, projected onto both sides: lift[(5d + v)*ES + x]
@@ -407,15 +431,15 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
         __syncthreads();   // the traces (d = 0), the previous direction's unStar (d > 0) have been read
         if (on) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) sU[v * 64 + lane] = uS[v] * nrm[d];
+            for (int v = 0; v < NV; ++v) sU[v * 64 + ql] = uS[v] * nrm[d];
         }
         __syncthreads();
         for (int side = 0; side < 2; ++side) {
             const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
-            const bool son = lane < (N0 + 1) * (N1 + 1);
-            const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
+            const bool son = ql < (N0 + 1) * (N1 + 1);
+            const int si = son ? ql % (N0 + 1) : 0, sj = son ? ql / (N0 + 1) : 0;
             double lv[NV];
-            project_face<NV, true>(M[side], r, side, sU, sTmp, lane, si, sj, son, lv);
+            project_face<NV, true>(M[side], r, side, sU, tmp, ql, si, sj, son, lv);
             if (son) {
                 const long x = elem_face_node(r, side, efoff, si, sj);
 #pragma unroll
@@ -570,39 +594,44 @@ template <bool GAUSS>
 __global__ void __launch_bounds__(64)
 km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *__restrict__ N3,
         const long *__restrict__ off, const int *__restrict__ efoff, const FaceRec *__restrict__ faces,
-        const double *__restrict__ fgeo, const double *__restrict__ fdelta, const OpsAll *__restrict__ opsp,
-        double *__restrict__ fe, int F, long S, long ES, long FSm, Phys ph, hxk::DtReduce dtr)
+        const int2 *__restrict__ sched, const double *__restrict__ fgeo, const double *__restrict__ fdelta,
+        const OpsAll *__restrict__ opsp, double *__restrict__ fe, int W, long S, long ES, long FSm, Phys ph,
+        hxk::DtReduce dtr)
 {
     // LDS: both sides' Q traces; one side's grad Q trace at a time, then the face flux
     __shared__ double sQ[2][NV * 64];
     __shared__ double sG[3 * NV * 64];
     __shared__ double sM[2][4][64];
     __shared__ double sTmp[NV * 64];   // the half-interpolated grid of a factorised interpolation
-    // faces in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
+    // waves in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
-    const int f = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
-    if (dtr.bits && f == 0) hxk::reduce_dt_partials(dtr.part, dtr.n, dtr.bits, lane);
-    if (f >= F) return;   // uniform per block
+    const int w = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
+    if (dtr.bits && w == 0) hxk::reduce_dt_partials(dtr.part, dtr.n, dtr.bits, lane);
+    if (w >= W) return;   // uniform per block
+    const FaceLane fl = face_lane(sched, w, lane);
+    const int f = fl.f, ql = fl.q;
+    double *const qLt = sQ[0] + fl.col, *const qRt = sQ[1] + fl.col, *const gT = sG + fl.col,
+                  *const tmp = sTmp + fl.col;
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
     const Mort M[2] = {mort_lds(sM[0]), mort_lds(sM[1])};
-    gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, sQ[0], lane, opsp);
-    gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, sQ[1], lane, opsp);
-    if (!ph.euler) gather_trace_k<3 * NV, GAUSS>(r, 0, N3, off, U, S, sG, lane, opsp);
+    gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, qLt, ql, opsp);
+    gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, qRt, ql, opsp);
+    if (!ph.euler) gather_trace_k<3 * NV, GAUSS>(r, 0, N3, off, U, S, gT, ql, opsp);
     __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
     // lanes past the face's nodes work on node (0, 0) and discard the result (every index stays
     // inside the face's arrays)
-    const bool on = lane < nq;
-    const int i = on ? lane % (n0 + 1) : 0, j = on ? lane / (n0 + 1) : 0;
-    const long fq = r.foff + (on ? lane : 0);
+    const bool on = ql < nq;
+    const int i = on ? ql % (n0 + 1) : 0, j = on ? ql / (n0 + 1) : 0;
+    const long fq = r.foff + (on ? ql : 0);
     double nrm[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) nrm[c] = fgeo[c * FSm + fq];
     const double jac = fgeo[9 * FSm + fq];
     double qs[2][NV];
-    adapt_face<NV, false>(M[0], r, 0, sQ[0], sTmp, lane, i, j, on, qs[0]);
-    adapt_face<NV, false>(M[1], r, 1, sQ[1], sTmp, lane, i, j, on, qs[1]);
+    adapt_face<NV, false>(M[0], r, 0, qLt, tmp, ql, i, j, on, qs[0]);
+    adapt_face<NV, false>(M[1], r, 1, qRt, tmp, ql, i, j, on, qs[1]);
     double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (!ph.euler) {
         // BR1_RiemannSolver (EllipticBR1.f90:816-868): each side's viscous flux from its
@@ -610,7 +639,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         // gradients (15 values) are interpolated five at a time (the size of sTmp)
         auto side_grad = [&](int s, double *g) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(M[s], r, s, sG + c * NV * 64, sTmp, lane, i, j, on, g + c * NV);
+            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(M[s], r, s, gT + c * NV * 64, tmp, ql, i, j, on, g + c * NV);
         };
         auto side_flux = [&](int s, const double *g, double (*Fs)[3]) {
             double vg[12];
@@ -629,7 +658,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         side_grad(0, g);
         if (on) side_flux(0, g, Fs[0]);
         __syncthreads();   // side 0's gradients consumed
-        gather_trace_k<3 * NV, GAUSS>(r, 1, N3, off, U, S, sG, lane, opsp);
+        gather_trace_k<3 * NV, GAUSS>(r, 1, N3, off, U, S, gT, ql, opsp);
         __syncthreads();
         side_grad(1, g);
         if (on) {
@@ -641,7 +670,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         }
         __syncthreads();   // sG takes the face flux next
     }
-    double *sF = sG;   // face flux [NV][64]
+    double *const sF = gT;   // face flux [NV][64]
     if (on) {
         double inv[NV];
         if (ph.riemann == 1) {
@@ -656,15 +685,15 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
             hx::roe_flux(ph, qs[0], qs[1], nrm, inv);
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sF[v * 64 + lane] = (inv[v] - visc[v]) * jac;
+        for (int v = 0; v < NV; ++v) sF[v * 64 + ql] = (inv[v] - visc[v]) * jac;
     }
     __syncthreads();
     for (int side = 0; side < 2; ++side) {
         const int N0 = r.Nfs[side][0], N1 = r.Nfs[side][1];
-        const bool son = lane < (N0 + 1) * (N1 + 1);
-        const int si = son ? lane % (N0 + 1) : 0, sj = son ? lane / (N0 + 1) : 0;
+        const bool son = ql < (N0 + 1) * (N1 + 1);
+        const int si = son ? ql % (N0 + 1) : 0, sj = son ? ql / (N0 + 1) : 0;
         double a[NV];
-        project_face<NV, false>(M[side], r, side, sF, sTmp, lane, si, sj, son, a);
+        project_face<NV, false>(M[side], r, side, sF, tmp, ql, si, sj, son, a);
         if (son) {
             const long x = elem_face_node(r, side, efoff, si, sj);
             if (GAUSS) {
