@@ -120,27 +120,26 @@ __device__ __forceinline__ long slot_node(const int *N3, const long *off, int e,
 // copy), 1 Tset along the first face index, 2 along the second, 3 along both, accumulated as
 // acc += T0[i][m] * T1[j][l] * value in the loop order l (outer), m (inner) of the reference.
 
-// The face's mortar matrices in LDS, m[side][0..3] = Tset(N0s -> n0), Tset(N1s -> n1) (side to
-// face) and Tset(n0 -> N0s), Tset(n1 -> N1s) (face to side), one entry per lane: the
-// interpolation loops then read them at LDS latency instead of through the cache from the
-// operator table.
-// One side's mortar matrices as staged by stage_mortar: a0, a1 side to face along the two face
-// indices, p0, p1 face to side.  (Reading them from the operator table through the cache
-// instead, which frees 4 KB of LDS per face, measured 19% slower in km_face.)
+// The face's mortar matrices in LDS.  Along each face index d at most one side interpolates
+// (the face order is the larger of the two sides' orders, so the other side's order is the face
+// order): m[d] = Tset(N_s -> n_d) (side to face) and m[2 + d] = Tset(n_d -> N_s) (face to side)
+// of that side serve both sides, one entry per lane.  The interpolation loops then read them at
+// LDS latency instead of through the cache from the operator table.  (Reading them from the
+// operator table instead measured 19% slower in km_face.)
 struct Mort {
-    const double *a0, *a1, *p0, *p1;
+    const double *a0, *a1, *p0, *p1;   // side to face along the two face indices, face to side
 };
 __device__ __forceinline__ Mort mort_lds(const double (*m)[64]) { return Mort{m[0], m[1], m[2], m[3]}; }
 
-__device__ __forceinline__ void stage_mortar(const OpsAll &ops, const FaceRec &r, double (*m)[4][64], int lane)
+__device__ __forceinline__ void stage_mortar(const OpsAll &ops, const FaceRec &r, double (*m)[64], int lane)
 {
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const int N0 = r.Nfs[s][0], N1 = r.Nfs[s][1], n0 = r.Nf[0], n1 = r.Nf[1];
-        m[s][0][lane] = ops.T[N0][n0][lane];
-        m[s][1][lane] = ops.T[N1][n1][lane];
-        m[s][2][lane] = ops.T[n0][N0][lane];
-        m[s][3][lane] = ops.T[n1][N1][lane];
+    for (int d = 0; d < 2; ++d) {
+        const int n = r.Nf[d], Ns = r.Nfs[0][d] < n ? r.Nfs[0][d] : r.Nfs[1][d];
+        if (Ns < n) {
+            m[d][lane] = ops.T[Ns][n][lane];
+            m[2 + d][lane] = ops.T[n][Ns][lane];
+        }
     }
 }
 
@@ -394,9 +393,9 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
              long S, long ES, long FSm)
 {
     // LDS: both sides' Q traces, then one direction's unStar in their place; the mortar
-    // matrices; the half-interpolated grid of a factorised interpolation (11.5 KB)
+    // matrices; the half-interpolated grid of a factorised interpolation (9.5 KB)
     __shared__ double sT[2][NV * 64];
-    __shared__ double sM[2][4][64];
+    __shared__ double sM[4][64];
     __shared__ double sTmp[NV * 64];
     // waves in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
@@ -408,7 +407,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     double *const sU = tL;   // [NV][64], once both traces are at the face order
     const FaceRec r = faces[fl.f];
     stage_mortar(*opsp, r, sM, lane);
-    const Mort M[2] = {mort_lds(sM[0]), mort_lds(sM[1])};
+    const Mort M = mort_lds(sM);
     gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, tL, ql, opsp);
     gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, tR, ql, opsp);
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
@@ -422,8 +421,8 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
     const double jac = fgeo[9 * FSm + fq];
     __syncthreads();
     double uS[NV], qL[NV], qR[NV];
-    adapt_face<NV, true>(M[0], r, 0, tL, tmp, ql, i, j, on, qL);
-    adapt_face<NV, true>(M[1], r, 1, tR, tmp, ql, i, j, on, qR);
+    adapt_face<NV, true>(M, r, 0, tL, tmp, ql, i, j, on, qL);
+    adapt_face<NV, true>(M, r, 1, tR, tmp, ql, i, j, on, qR);
 #pragma unroll
     for (int v = 0; v < NV; ++v) uS[v] = 0.5 * (qR[v] - qL[v]) * jac;
     // unStar(v) n_d, one direction at a time, projected onto both sides: lift[(5d + v)*ES + x]
@@ -439,7 +438,7 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
             const bool son = ql < (N0 + 1) * (N1 + 1);
             const int si = son ? ql % (N0 + 1) : 0, sj = son ? ql / (N0 + 1) : 0;
             double lv[NV];
-            project_face<NV, true>(M[side], r, side, sU, tmp, ql, si, sj, son, lv);
+            project_face<NV, true>(M, r, side, sU, tmp, ql, si, sj, son, lv);
             if (son) {
                 const long x = elem_face_node(r, side, efoff, si, sj);
 #pragma unroll
@@ -591,18 +590,19 @@ km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__
 // fe[v*ES + x] = (+/-) projected flux * b.
 // ------------------------------------------------------------------------------------
 template <bool GAUSS>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *__restrict__ N3,
         const long *__restrict__ off, const int *__restrict__ efoff, const FaceRec *__restrict__ faces,
         const int2 *__restrict__ sched, const double *__restrict__ fgeo, const double *__restrict__ fdelta,
         const OpsAll *__restrict__ opsp, double *__restrict__ fe, int W, long S, long ES, long FSm, Phys ph,
         hxk::DtReduce dtr)
 {
-    // LDS: both sides' Q traces; one side's grad Q trace at a time, then the face flux
-    __shared__ double sQ[2][NV * 64];
-    __shared__ double sG[3 * NV * 64];
-    __shared__ double sM[2][4][64];
-    __shared__ double sTmp[NV * 64];   // the half-interpolated grid of a factorised interpolation
+    // LDS: both sides' Q traces (rows 0-4, 5-9), then one side's grad Q trace at a time, then
+    // the face flux, in one space; the mortar matrices; the half-interpolated grid of a
+    // factorised interpolation (12.25 KB: three waves per SIMD)
+    __shared__ double sR[3 * NV * 64];
+    __shared__ double sM[4][64];
+    __shared__ double sTmp[NV * 64];
     // waves in XCD-contiguous ranges (the grid is a multiple of 8): faces that share elements
     // mostly run on one XCD and find the elements' traces in its L2
     const int w = hxk::xcd_block(blockIdx.x, gridDim.x), lane = threadIdx.x;
@@ -610,14 +610,13 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     if (w >= W) return;   // uniform per block
     const FaceLane fl = face_lane(sched, w, lane);
     const int f = fl.f, ql = fl.q;
-    double *const qLt = sQ[0] + fl.col, *const qRt = sQ[1] + fl.col, *const gT = sG + fl.col,
+    double *const qLt = sR + fl.col, *const qRt = sR + NV * 64 + fl.col, *const gT = sR + fl.col,
                   *const tmp = sTmp + fl.col;
     const FaceRec r = faces[f];
     stage_mortar(*opsp, r, sM, lane);
-    const Mort M[2] = {mort_lds(sM[0]), mort_lds(sM[1])};
+    const Mort M = mort_lds(sM);
     gather_trace_k<NV, GAUSS>(r, 0, N3, off, Q, S, qLt, ql, opsp);
     gather_trace_k<NV, GAUSS>(r, 1, N3, off, Q, S, qRt, ql, opsp);
-    if (!ph.euler) gather_trace_k<3 * NV, GAUSS>(r, 0, N3, off, U, S, gT, ql, opsp);
     __syncthreads();
     const int n0 = r.Nf[0], n1 = r.Nf[1], nq = (n0 + 1) * (n1 + 1);
     // lanes past the face's nodes work on node (0, 0) and discard the result (every index stays
@@ -630,16 +629,17 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
     for (int c = 0; c < 3; ++c) nrm[c] = fgeo[c * FSm + fq];
     const double jac = fgeo[9 * FSm + fq];
     double qs[2][NV];
-    adapt_face<NV, false>(M[0], r, 0, qLt, tmp, ql, i, j, on, qs[0]);
-    adapt_face<NV, false>(M[1], r, 1, qRt, tmp, ql, i, j, on, qs[1]);
+    adapt_face<NV, false>(M, r, 0, qLt, tmp, ql, i, j, on, qs[0]);
+    adapt_face<NV, false>(M, r, 1, qRt, tmp, ql, i, j, on, qs[1]);
+    __syncthreads();   // the Q traces are read: the space takes a grad Q trace or the face flux
     double visc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (!ph.euler) {
         // BR1_RiemannSolver (EllipticBR1.f90:816-868): each side's viscous flux from its
-        // gradients at the face order, the sides one after the other through sG; the
+        // gradients at the face order, the sides one after the other through the trace space; the
         // gradients (15 values) are interpolated five at a time (the size of sTmp)
         auto side_grad = [&](int s, double *g) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(M[s], r, s, gT + c * NV * 64, tmp, ql, i, j, on, g + c * NV);
+            for (int c = 0; c < 3; ++c) adapt_face<NV, true>(M, r, s, gT + c * NV * 64, tmp, ql, i, j, on, g + c * NV);
         };
         auto side_flux = [&](int s, const double *g, double (*Fs)[3]) {
             double vg[12];
@@ -655,6 +655,8 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
             hx::viscous_flux(mu, kappa, qs[s][1] * ir, qs[s][2] * ir, qs[s][3] * ir, vg, Fs);
         };
         double Fs[2][NV][3], g[3 * NV];
+        gather_trace_k<3 * NV, GAUSS>(r, 0, N3, off, U, S, gT, ql, opsp);
+        __syncthreads();
         side_grad(0, g);
         if (on) side_flux(0, g, Fs[0]);
         __syncthreads();   // side 0's gradients consumed
@@ -668,7 +670,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
                 visc[v] = 0.5 * (Fs[0][v][0] + Fs[1][v][0]) * nrm[0] + 0.5 * (Fs[0][v][1] + Fs[1][v][1]) * nrm[1] +
                           0.5 * (Fs[0][v][2] + Fs[1][v][2]) * nrm[2];
         }
-        __syncthreads();   // sG takes the face flux next
+        __syncthreads();   // the space takes the face flux next
     }
     double *const sF = gT;   // face flux [NV][64]
     if (on) {
@@ -693,7 +695,7 @@ km_face(const double *__restrict__ Q, const double *__restrict__ U, const int *_
         const bool son = ql < (N0 + 1) * (N1 + 1);
         const int si = son ? ql % (N0 + 1) : 0, sj = son ? ql / (N0 + 1) : 0;
         double a[NV];
-        project_face<NV, false>(M[side], r, side, sF, tmp, ql, si, sj, son, a);
+        project_face<NV, false>(M, r, side, sF, tmp, ql, si, sj, son, a);
         if (son) {
             const long x = elem_face_node(r, side, efoff, si, sj);
             if (GAUSS) {
