@@ -49,21 +49,35 @@ __device__ __forceinline__ int line_node(int s, int a, int b, int l)
 
 // The traces of NVAL node arrays sV[c*ne + n] (LDS) of element e on its six faces:
 // out[c*ES + (6e + s) nf + a + (N+1) b] = acc, acc = acc + value(l) v(l) for l = 0..N along the
-// slot's axis (the reference's order).  The 6 (N+1)^2 NVAL (face node, value) pairs are spread
-// evenly over the block's threads, consecutive threads on consecutive face nodes of one value.
+// slot's axis (the reference's order).  One work item is a line of nodes along an axis at face
+// coordinates (a, b) and five of the values: it gives both ends of the line, the slot pair
+// 5 / 3 (xi), 0 / 1 (eta) or 2 / 4 (zeta) with v(-1) and v(+1), from one read of each node
+// value.  Consecutive threads take consecutive lines of one axis (consecutive face nodes).
 template <int N, int NVAL>
 __device__ __forceinline__ void write_traces(const double *sV, double *out, long ES, int e, const GaussOps &go,
                                              int t, int nthr)
 {
-    constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1, NFN = 6 * nf;
-    for (int it = t; it < NFN * NVAL; it += nthr) {
-        const int c = it / NFN, f = it - c * NFN;
-        const int s = f / nf, r = f - s * nf, a = r % n1, b = r / n1;
-        const bool plus = s == 1 || s == 3 || s == 4;
-        double acc = 0.0;
+    static_assert(NVAL % NV == 0, "values in groups of five");
+    constexpr int n1 = N + 1, nf = n1 * n1, ne = nf * n1, NL = 3 * nf, NG = NVAL / NV;
+    for (int it = t; it < NL * NG; it += nthr) {
+        const int grp = it / NL, L = it - grp * NL;
+        const int ax = L / nf, r = L - ax * nf;
+        const int sm = ax == 0 ? 5 : (ax == 1 ? 0 : 2), sp = ax == 0 ? 3 : (ax == 1 ? 1 : 4);
+        const int n0 = line_node<N>(sm, r % n1, r / n1, 0), st = ax == 0 ? 1 : (ax == 1 ? n1 : nf);
+        const long om = (long)(6 * e + sm) * nf + r, op = (long)(6 * e + sp) * nf + r;
 #pragma unroll
-        for (int l = 0; l < n1; ++l) acc = acc + sV[c * ne + line_node<N>(s, a, b, l)] * (plus ? go.vR[l] : go.vL[l]);
-        out[(long)c * ES + (long)(6 * e) * nf + f] = acc;   // c varies across the wave: a 64-bit address
+        for (int v = 0; v < NV; ++v) {
+            const int c = grp * NV + v;
+            double am = 0.0, ap = 0.0;
+#pragma unroll
+            for (int l = 0; l < n1; ++l) {
+                const double x = sV[c * ne + n0 + l * st];
+                am = am + x * go.vL[l];
+                ap = ap + x * go.vR[l];
+            }
+            out[(long)c * ES + om] = am;   // c varies across the wave: 64-bit addresses
+            out[(long)c * ES + op] = ap;
+        }
     }
 }
 
