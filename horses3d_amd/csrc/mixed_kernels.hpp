@@ -5,10 +5,11 @@
 // 865-964).  Included by hx3d_engine.hip after stage_kernels.hpp.
 //
 // One stage = km_face_grad -> km_grad -> km_face -> km_volume:
-//   km_face_grad  one wave per face: both Q traces to the face order, BR1 unStar there,
+//   km_face_grad  one wave per face (or two small faces of one shape, see face_lane): both Q
+//                 traces to the face order, BR1 unStar there,
 //                 projected back onto each side's element face (lift[15][ES]);
 //   km_grad       element tiles: local gradient + lifting -> grad Q (15 values);
-//   km_face       one wave per face: Q and grad Q traces to the face order, Riemann + BR1
+//   km_face       as km_face_grad: Q and grad Q traces to the face order, Riemann + BR1
 //                 fluxes there, projected back (right side negated) and scaled by the
 //                 receiving element's surface weight -> fe[5][ES];
 //   km_volume     element tiles: volume integral, surface integral, 1/J, RK update and
