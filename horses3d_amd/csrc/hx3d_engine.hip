@@ -33,6 +33,8 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/hx3d.h"
@@ -480,7 +482,11 @@ struct hx3d_engine {
     // mixed orders: the element tiles of the persistent element kernels (packing policy
     // tile_policy, see hx3d_autotune) and each kernel's grid (blocks resident at once)
     hxm::NodeMap *dTmap = nullptr;   // per tile thread: node, element, node indices and orders
-    int ntiles = 0, tile_policy = 0, grid_mgrad = 0, grid_mvol = 0;
+    // tiles of TILE_SMALL threads (elements of at most that many nodes), then of TILE threads;
+    // map_big: the first map entry of the larger tiles; grids per kernel and tile size
+    int ntiles = 0, tile_policy = 0, ntiles_s = 0, ntiles_b = 0;
+    long map_big = 0;
+    int grid_mgrad_s = 0, grid_mgrad_b = 0, grid_mvol_s = 0, grid_mvol_b = 0;
     std::vector<int> N3h_mixed;   // [3K] element orders (host copy for re-tiling)
     // side streams for the element-bucket launches (the buckets touch disjoint elements, so
     // they run side by side), forked from and joined back to the engine stream with events
@@ -711,16 +717,25 @@ void launch_maxdt(hx3d_engine *eng, double cfl, double dcfl, int slot)
 // The persistent element kernels: one launch sweeps every element, grid_m* blocks (as many as
 // are resident at once) taking the tiles in turn.
 template <int FORM>
-size_t mvolume_lds() { return sizeof(double) * (2 * hxm::MAXN1 * 64 + (size_t)hxm::mvolume_per_node<FORM>() * hxm::TILE); }
-size_t mgrad_lds() { return sizeof(double) * (hxm::MAXN1 * 64 + (size_t)NV * hxm::TILE); }
+size_t mvolume_lds(int T) { return sizeof(double) * (2 * hxm::MAXN1 * 64 + (size_t)hxm::mvolume_per_node<FORM>() * T); }
+size_t mgrad_lds(int T) { return sizeof(double) * (hxm::MAXN1 * 64 + (size_t)NV * T); }
 
 template <int FORM, bool GAUSS = false>
 void launch_mvolume(hx3d_engine *eng, double a_k, double c_k, int update, const DtArgs &dta, double *qdot)
 {
-    hipLaunchKernelGGL((hxm::km_volume<FORM, GAUSS>), dim3(eng->grid_mvol), dim3(hxm::TILE), mvolume_lds<FORM>(),
-                       eng->stream, eng->dTmap, eng->ntiles, eng->dQ, eng->dU, eng->dEfoff, eng->dJa, eng->dJ,
-                       eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG, qdot, a_k, c_k, update, dta, eng->S, eng->ES,
-                       eng->ph);
+    if (eng->ntiles_s)
+        hipLaunchKernelGGL((hxm::km_volume<FORM, GAUSS, hxm::TILE_SMALL>), dim3(eng->grid_mvol_s), dim3(hxm::TILE_SMALL),
+                           mvolume_lds<FORM>(hxm::TILE_SMALL), eng->stream, eng->dTmap, eng->ntiles_s, eng->dQ, eng->dU,
+                           eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG, qdot, a_k, c_k,
+                           update, dta, eng->S, eng->ES, eng->ph);
+    if (eng->ntiles_b) {
+        DtArgs db = dta;   // the time advances once per stage: in the first launch
+        if (eng->ntiles_s) db.time = nullptr;
+        hipLaunchKernelGGL((hxm::km_volume<FORM, GAUSS, hxm::TILE>), dim3(eng->grid_mvol_b), dim3(hxm::TILE),
+                           mvolume_lds<FORM>(hxm::TILE), eng->stream, eng->dTmap + eng->map_big, eng->ntiles_b, eng->dQ,
+                           eng->dU, eng->dEfoff, eng->dJa, eng->dJ, eng->dEdelta, eng->dFe, eng->dOpsAll, eng->dG, qdot,
+                           a_k, c_k, update, db, eng->S, eng->ES, eng->ph);
+    }
 }
 
 // One stage of the mixed path; GAUSS: Gauss nodes (standard form only)
@@ -734,9 +749,16 @@ void stage_mixed_nodes(hx3d_engine *eng, double a_k, double c_k, int update, con
         hipLaunchKernelGGL(hxm::km_face_grad<GAUSS>, dim3(grid_for(eng->nwaves)), dim3(64), 0, eng->stream, eng->dQ,
                            eng->dN3, eng->dOff, eng->dEfoff, eng->dFaces, eng->dSched, eng->dFgeo, eng->dOpsAll,
                            eng->dLift, eng->nwaves, eng->S, eng->ES, eng->FSm);
-        hipLaunchKernelGGL(hxm::km_grad<GAUSS>, dim3(eng->grid_mgrad), dim3(hxm::TILE), mgrad_lds(), eng->stream,
-                           eng->dTmap, eng->ntiles, eng->dQ, eng->dEfoff, eng->dJa, eng->dJ, eng->dLift,
-                           eng->dOpsAll, eng->dU, eng->S, eng->ES, eng->ph, lim);
+        // dt partials: the small tiles' blocks first, then the large tiles'
+        if (eng->ntiles_s)
+            hipLaunchKernelGGL((hxm::km_grad<GAUSS, hxm::TILE_SMALL>), dim3(eng->grid_mgrad_s), dim3(hxm::TILE_SMALL),
+                               mgrad_lds(hxm::TILE_SMALL), eng->stream, eng->dTmap, eng->ntiles_s, eng->dQ, eng->dEfoff,
+                               eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES, eng->ph, lim, 0);
+        if (eng->ntiles_b)
+            hipLaunchKernelGGL((hxm::km_grad<GAUSS, hxm::TILE>), dim3(eng->grid_mgrad_b), dim3(hxm::TILE),
+                               mgrad_lds(hxm::TILE), eng->stream, eng->dTmap + eng->map_big, eng->ntiles_b, eng->dQ,
+                               eng->dEfoff, eng->dJa, eng->dJ, eng->dLift, eng->dOpsAll, eng->dU, eng->S, eng->ES,
+                               eng->ph, lim, eng->grid_mgrad_s);
     }
     if (ev) (void)hipEventRecord(ev[1], eng->stream);
     hipLaunchKernelGGL(hxm::km_face<GAUSS>, dim3(grid_for(eng->nwaves)), dim3(64), 0, eng->stream, eng->dQ, eng->dU,
@@ -829,90 +851,112 @@ void set_physics(hx3d_engine *eng, const hx3d_case_desc *d)
 //   1  elements of one order triple (Nx, Ny, Nz) together, in turn: tiles of like elements, so
 //      the contractions' loop bounds agree across a wave;
 //   2  one element per tile.
-// Every policy gives the same results (each element is computed from its own data alone; the
-// dt partials are minima).  Also sets each element kernel's grid: the blocks resident at once.
+// Tiles hold TILE_SMALL threads when every element has at most that many nodes, else TILE
+// (the kernels and launches take both sizes at once: small elements in small tiles, the others
+// in large ones, each size swept by its own launch).  Every policy
+// gives the same results (each element is computed from its own data alone; the dt partials are
+// minima).  Also sets each element kernel's grids: the blocks resident at once.
 int set_tiles(hx3d_engine *eng, int policy)
 {
     const std::vector<int> &N3 = eng->N3h_mixed;
     const int K = eng->K;
+    if (policy < 0 || policy > 2) return eng->fail(HX3D_EINVAL, "unknown tile packing policy");
     std::vector<int> ne(K);
     for (int e = 0; e < K; ++e) ne[e] = (N3[3 * e] + 1) * (N3[3 * e + 1] + 1) * (N3[3 * e + 2] + 1);
-    std::vector<int> order(K);
-    for (int e = 0; e < K; ++e) order[e] = e;
-    std::vector<hxm::Tile> tiles;
-    auto open_tile = [&]() {
-        hxm::Tile t;
-        std::memset(&t, 0, sizeof(t));
-        tiles.push_back(t);
-        return (int)tiles.size() - 1;
-    };
-    auto add = [&](int ti, int e) {
-        hxm::Tile &t = tiles[ti];
-        t.elem[t.count] = e;
-        t.off[t.count + 1] = t.off[t.count] + ne[e];
-        ++t.count;
-    };
-    auto fits = [&](int ti, int e) {
-        const hxm::Tile &t = tiles[ti];
-        return t.count < hxm::TILE_MAXE && t.off[t.count] + ne[e] <= hxm::TILE;
-    };
-    if (policy == 0) {
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ne[a] > ne[b]; });
-        const int min_ne = K ? ne[order[K - 1]] : 0;
-        int first_open = 0;   // tiles before it hold no element of any size left
-        for (int e : order) {
+    // the elements of each tile size, packed under the policy into tiles of cap threads
+    auto pack = [&](std::vector<int> order, int cap) {
+        std::vector<hxm::Tile> tiles;
+        auto open_tile = [&]() {
+            hxm::Tile t;
+            std::memset(&t, 0, sizeof(t));
+            tiles.push_back(t);
+            return (int)tiles.size() - 1;
+        };
+        auto add = [&](int ti, int e) {
+            hxm::Tile &t = tiles[ti];
+            t.elem[t.count] = e;
+            t.off[t.count + 1] = t.off[t.count] + ne[e];
+            ++t.count;
+        };
+        auto fits = [&](int ti, int e) {
+            const hxm::Tile &t = tiles[ti];
+            return t.count < hxm::TILE_MAXE && t.off[t.count] + ne[e] <= cap;
+        };
+        const int n = (int)order.size();
+        if (policy == 0) {
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ne[a] > ne[b]; });
+            const int min_ne = n ? ne[order[n - 1]] : 0;
+            int first_open = 0;   // tiles before it hold no element of any size left
+            for (int e : order) {
+                int ti = -1;
+                for (int x = first_open; x < (int)tiles.size(); ++x)
+                    if (fits(x, e)) { ti = x; break; }
+                if (ti < 0) ti = open_tile();
+                add(ti, e);
+                while (first_open < (int)tiles.size() &&
+                       (tiles[first_open].count == hxm::TILE_MAXE || tiles[first_open].off[tiles[first_open].count] + min_ne > cap))
+                    ++first_open;
+            }
+        } else if (policy == 1) {
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+                if (ne[a] != ne[b]) return ne[a] > ne[b];
+                for (int c = 0; c < 3; ++c)
+                    if (N3[3 * a + c] != N3[3 * b + c]) return N3[3 * a + c] < N3[3 * b + c];
+                return false;
+            });
             int ti = -1;
-            for (int x = first_open; x < (int)tiles.size(); ++x)
-                if (fits(x, e)) { ti = x; break; }
-            if (ti < 0) ti = open_tile();
-            add(ti, e);
-            while (first_open < (int)tiles.size() &&
-                   (tiles[first_open].count == hxm::TILE_MAXE || tiles[first_open].off[tiles[first_open].count] + min_ne > hxm::TILE))
-                ++first_open;
+            for (int x = 0; x < n; ++x) {
+                const int e = order[x];
+                const bool same = ti >= 0 && std::equal(&N3[3 * e], &N3[3 * e + 3], &N3[3 * tiles[ti].elem[0]]);
+                if (ti < 0 || !same || !fits(ti, e)) ti = open_tile();
+                add(ti, e);
+            }
+        } else {
+            for (int e : order) add(open_tile(), e);
         }
-    } else if (policy == 1) {
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            if (ne[a] != ne[b]) return ne[a] > ne[b];
-            for (int c = 0; c < 3; ++c)
-                if (N3[3 * a + c] != N3[3 * b + c]) return N3[3 * a + c] < N3[3 * b + c];
-            return false;
-        });
-        int ti = -1;
-        for (int x = 0; x < K; ++x) {
-            const int e = order[x];
-            const bool same = ti >= 0 && std::equal(&N3[3 * e], &N3[3 * e + 3], &N3[3 * tiles[ti].elem[0]]);
-            if (ti < 0 || !same || !fits(ti, e)) ti = open_tile();
-            add(ti, e);
-        }
-    } else if (policy == 2) {
-        for (int e = 0; e < K; ++e) add(open_tile(), e);
-    } else {
-        return eng->fail(HX3D_EINVAL, "unknown tile packing policy");
+        return tiles;
+    };
+    // one tile size per mesh: small tiles when every element fits (on a 24^3 box with orders on
+    // {2..5}, 5% faster per step than 512-thread tiles), else 512-thread tiles for all (with
+    // orders up to 7, a launch of each size measured 0.6% slower than the large tiles alone)
+    std::vector<int> small, big;
+    for (int e = 0; e < K; ++e) (ne[e] <= hxm::TILE_SMALL ? small : big).push_back(e);
+    if (!big.empty()) {
+        big.insert(big.end(), small.begin(), small.end());
+        std::sort(big.begin(), big.end());
+        small.clear();
     }
-    for (const hxm::Tile &t : tiles)
+    const std::vector<hxm::Tile> ts = pack(small, hxm::TILE_SMALL), tb = pack(big, hxm::TILE);
+    for (const hxm::Tile &t : tb)
         if (t.count < 1 || t.off[t.count] > hxm::TILE) return eng->fail(HX3D_EUNSUPPORTED, "an element has more nodes than a tile");
-    // the threads' node map (hxm::NodeMap): node offsets as the device's element offsets
+    // the threads' node map (hxm::NodeMap): node offsets as the device's element offsets; the
+    // small tiles' entries, then the large tiles'
     std::vector<long> eoff(K + 1, 0);
     for (int e = 0; e < K; ++e) eoff[e + 1] = eoff[e] + ne[e];
     if (eoff[K] > INT_MAX) return eng->fail(HX3D_EUNSUPPORTED, "mixed orders: more than 2^31 - 1 nodes");
-    std::vector<hxm::NodeMap> map(tiles.size() * (size_t)hxm::TILE);
-    for (size_t ti = 0; ti < tiles.size(); ++ti) {
-        const hxm::Tile &T = tiles[ti];
-        int slot = 0;
-        for (int t = 0; t < hxm::TILE; ++t) {
-            const bool active = t < T.off[T.count];
-            while (active && t >= T.off[slot + 1]) ++slot;
-            const int s = active ? slot : T.count - 1;
-            const int e = T.elem[s], n = active ? t - T.off[s] : 0;
-            const int nx = N3[3 * e] + 1, ny = N3[3 * e + 1] + 1;
-            const int i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
-            hxm::NodeMap &m = map[ti * hxm::TILE + t];
-            m.g = (int)(eoff[e] + n);
-            m.e = e;
-            m.code = i | j << 3 | k << 6 | N3[3 * e] << 9 | N3[3 * e + 1] << 12 | N3[3 * e + 2] << 15 |
-                     (active ? T.off[s] : 0) << 18 | (active ? 1 : 0) << 27;
+    const size_t map_big = ts.size() * (size_t)hxm::TILE_SMALL;
+    std::vector<hxm::NodeMap> map(map_big + tb.size() * (size_t)hxm::TILE);
+    auto fill = [&](const std::vector<hxm::Tile> &tiles, int cap, size_t base) {
+        for (size_t ti = 0; ti < tiles.size(); ++ti) {
+            const hxm::Tile &T = tiles[ti];
+            int slot = 0;
+            for (int t = 0; t < cap; ++t) {
+                const bool active = t < T.off[T.count];
+                while (active && t >= T.off[slot + 1]) ++slot;
+                const int s = active ? slot : T.count - 1;
+                const int e = T.elem[s], n = active ? t - T.off[s] : 0;
+                const int nx = N3[3 * e] + 1, ny = N3[3 * e + 1] + 1;
+                const int i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
+                hxm::NodeMap &m = map[base + ti * cap + t];
+                m.g = (int)(eoff[e] + n);
+                m.e = e;
+                m.code = i | j << 3 | k << 6 | N3[3 * e] << 9 | N3[3 * e + 1] << 12 | N3[3 * e + 2] << 15 |
+                         (active ? T.off[s] : 0) << 18 | (active ? 1 : 0) << 27;
+            }
         }
-    }
+    };
+    fill(ts, hxm::TILE_SMALL, 0);
+    fill(tb, hxm::TILE, map_big);
     if (eng->dTmap) {
         if (hipStreamSynchronize(eng->stream) != hipSuccess) return eng->fail(HX3D_EDEVICE, "synchronize failed");
         (void)hipFree(eng->dTmap);
@@ -922,30 +966,50 @@ int set_tiles(hx3d_engine *eng, int policy)
     if ((rc = dalloc(eng, &eng->dTmap, map.size()))) return rc;
     if (hipMemcpy(eng->dTmap, map.data(), map.size() * sizeof(hxm::NodeMap), hipMemcpyHostToDevice) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "tile upload failed");
-    eng->ntiles = (int)tiles.size();
+    eng->ntiles_s = (int)ts.size();
+    eng->ntiles_b = (int)tb.size();
+    eng->ntiles = eng->ntiles_s + eng->ntiles_b;
+    eng->map_big = (long)map_big;
     eng->tile_policy = policy;
     // grids: the blocks resident at once on the device (at least one), no more than the tiles
-    int dev = 0, cus = 0, bg = 0, bv = 0;
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return eng->fail(HX3D_EDEVICE, "no HIP device");
-    hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &bg, eng->gauss ? reinterpret_cast<const void *>(&hxm::km_grad<true>) : reinterpret_cast<const void *>(&hxm::km_grad<false>),
-        hxm::TILE, mgrad_lds());
+    hipError_t oe = hipSuccess;
+    auto occ = [&](const void *k, int T, size_t lds) {
+        int b = 0;
+        if (oe == hipSuccess) oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, T, lds);
+        return std::max(1, b) * cus;
+    };
+    const int gs = eng->gauss ? occ(reinterpret_cast<const void *>(&hxm::km_grad<true, hxm::TILE_SMALL>), hxm::TILE_SMALL, mgrad_lds(hxm::TILE_SMALL))
+                              : occ(reinterpret_cast<const void *>(&hxm::km_grad<false, hxm::TILE_SMALL>), hxm::TILE_SMALL, mgrad_lds(hxm::TILE_SMALL));
+    const int gb = eng->gauss ? occ(reinterpret_cast<const void *>(&hxm::km_grad<true, hxm::TILE>), hxm::TILE, mgrad_lds(hxm::TILE))
+                              : occ(reinterpret_cast<const void *>(&hxm::km_grad<false, hxm::TILE>), hxm::TILE, mgrad_lds(hxm::TILE));
     const bool split = eng->inviscid == HX3D_INVISCID_SPLIT;
     const int avg = eng->ph.averaging;
-    auto occ = [&](const void *k, size_t lds) {
-        if (oe == hipSuccess) oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bv, k, hxm::TILE, lds);
+    auto vol = [&](auto form_tag, auto gauss_tag) {
+        constexpr int FORM = decltype(form_tag)::value;
+        constexpr bool GAUSS = decltype(gauss_tag)::value;
+        const int a = occ(reinterpret_cast<const void *>(&hxm::km_volume<FORM, GAUSS, hxm::TILE_SMALL>), hxm::TILE_SMALL,
+                          mvolume_lds<FORM>(hxm::TILE_SMALL));
+        const int b = occ(reinterpret_cast<const void *>(&hxm::km_volume<FORM, GAUSS, hxm::TILE>), hxm::TILE,
+                          mvolume_lds<FORM>(hxm::TILE));
+        return std::make_pair(a, b);
     };
-    if (eng->gauss) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::FORM_STANDARD, true>), mvolume_lds<hxk::FORM_STANDARD>());
-    else if (!split) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::FORM_STANDARD, false>), mvolume_lds<hxk::FORM_STANDARD>());
-    else if (avg == HX3D_AVG_DUCROS) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_DUCROS, false>), mvolume_lds<hxk::AVG_DUCROS>());
-    else if (avg == HX3D_AVG_KENNEDYGRUBER) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_KG, false>), mvolume_lds<hxk::AVG_KG>());
-    else if (avg == HX3D_AVG_PIROZZOLI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_PIROZZOLI, false>), mvolume_lds<hxk::AVG_PIROZZOLI>());
-    else if (avg == HX3D_AVG_MORINISHI) occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_MORINISHI, false>), mvolume_lds<hxk::AVG_MORINISHI>());
-    else occ(reinterpret_cast<const void *>(&hxm::km_volume<hxk::AVG_STANDARD, false>), mvolume_lds<hxk::AVG_STANDARD>());
+    using F = std::false_type;
+    std::pair<int, int> gv;
+    if (eng->gauss) gv = vol(std::integral_constant<int, hxk::FORM_STANDARD>{}, std::true_type{});
+    else if (!split) gv = vol(std::integral_constant<int, hxk::FORM_STANDARD>{}, F{});
+    else if (avg == HX3D_AVG_DUCROS) gv = vol(std::integral_constant<int, hxk::AVG_DUCROS>{}, F{});
+    else if (avg == HX3D_AVG_KENNEDYGRUBER) gv = vol(std::integral_constant<int, hxk::AVG_KG>{}, F{});
+    else if (avg == HX3D_AVG_PIROZZOLI) gv = vol(std::integral_constant<int, hxk::AVG_PIROZZOLI>{}, F{});
+    else if (avg == HX3D_AVG_MORINISHI) gv = vol(std::integral_constant<int, hxk::AVG_MORINISHI>{}, F{});
+    else gv = vol(std::integral_constant<int, hxk::AVG_STANDARD>{}, F{});
     if (oe != hipSuccess) return eng->fail(HX3D_EDEVICE, std::string("occupancy query: ") + hipGetErrorString(oe));
-    eng->grid_mgrad = std::max(1, std::min(eng->ntiles, std::max(1, bg) * cus));
-    eng->grid_mvol = std::max(1, std::min(eng->ntiles, std::max(1, bv) * cus));
+    eng->grid_mgrad_s = eng->ntiles_s ? std::min(eng->ntiles_s, gs) : 0;
+    eng->grid_mgrad_b = eng->ntiles_b ? std::min(eng->ntiles_b, gb) : 0;
+    eng->grid_mvol_s = eng->ntiles_s ? std::min(eng->ntiles_s, gv.first) : 0;
+    eng->grid_mvol_b = eng->ntiles_b ? std::min(eng->ntiles_b, gv.second) : 0;
     return HX3D_OK;
 }
 
@@ -1776,7 +1840,7 @@ int hx3d_advance(hx3d_engine *eng, int nsteps, double cfl, double dcfl, double *
                                    eng->nodes, eng->S, eng->dcsi, cfl, dcfl, eng->ph, eng->dDtPart);
             nparts = (int)nb;
         } else {
-            nparts = eng->mixed ? eng->grid_mgrad : grad_grid(eng);
+            nparts = eng->mixed ? eng->grid_mgrad_s + eng->grid_mgrad_b : grad_grid(eng);
         }
         for (int k = 0; k < 3; ++k) {
             DtArgs a{};

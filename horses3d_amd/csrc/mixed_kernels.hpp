@@ -61,7 +61,10 @@ struct FaceRec {
 // elem[s], off[s] <= t < off[s + 1] (threads from off[count] on are idle).  Which elements share
 // a tile (the packing) is chosen on the host (hx3d_autotune); every packing gives the same
 // results.  The kernels read the tiles through their node map (NodeMap).
-constexpr int TILE = 512, TILE_MAXE = 30;
+// Tiles of two sizes, each swept by a launch of its own: 256 threads (two independent blocks
+// per CU, whose loads and barriers do not wait for each other) for elements of at most 256
+// nodes, 512 threads for larger elements (up to 8 x 8 x 8 nodes).  The host picks the sizes.
+constexpr int TILE = 512, TILE_SMALL = 256, TILE_MAXE = 30;
 struct Tile {
     int count;
     int elem[TILE_MAXE];
@@ -69,7 +72,7 @@ struct Tile {
 };
 
 // What thread t of a tile works on, built on the host with the tiles (one entry per thread of
-// every tile, at [tile * TILE + t]): node g of element e, and code = i | j << 3 | k << 6 |
+// every tile, at [tile * T + t] for tiles of T threads): node g of element e, and code = i | j << 3 | k << 6 |
 // Nx << 9 | Ny << 12 | Nz << 15 | (first thread of the element in the tile) << 18 | active << 27.
 // One load gives a thread everything the tile table, the orders and the node offsets would
 // give through three dependent ones; the element kernels load the next tile's entry while
@@ -78,9 +81,10 @@ struct Tile {
 struct NodeMap {
     int g, e, code;
 };
+template <int T>
 __device__ __forceinline__ NodeMap load_map(const NodeMap *__restrict__ map, int tile, int t)
 {
-    return map[(long)tile * TILE + t];
+    return map[(long)tile * T + t];
 }
 
 // element face slot 0..5 = EFRONT, EBACK, EBOTTOM, ERIGHT, ETOP, ELEFT: the fixed axis, its end,
@@ -454,12 +458,12 @@ km_face_grad(const double *__restrict__ Q, const int *__restrict__ N3, const lon
 // BR1 lifting (BR1_GradientFaceLoop, EllipticBR1.f90:531-569, with the weights b of
 // VectorWeakIntegrals_StdFace): grad Q at every node, U[(5d + v)*S + g].
 // ------------------------------------------------------------------------------------
-template <bool GAUSS>
-__global__ void __launch_bounds__(TILE)
+template <bool GAUSS, int T>
+__global__ void __launch_bounds__(T)
 km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__ Q,
         const int *__restrict__ efoff, const double *__restrict__ Ja,
         const double *__restrict__ Jac, const double *__restrict__ lift, const OpsAll *__restrict__ opsp,
-        double *__restrict__ U, long S, long ES, Phys ph, hxk::DtLimits dtl)
+        double *__restrict__ U, long S, long ES, Phys ph, hxk::DtLimits dtl, int pbase)
 {
     HIP_DYNAMIC_SHARED(double, smem)
     const OpsAll &ops = *opsp;
@@ -468,7 +472,7 @@ km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__
     double *sQall = smem + MAXN1 * 64;   // per element [NV][ne], at NV * off[slot]
     for (int x = t; x < MAXN1 * 64; x += blockDim.x) sD[x] = ops.D[x / 64][x % 64];
     double tc = INFINITY, tv = INFINITY;   // MaxTimeStep limits of this thread's nodes (stage 1)
-    NodeMap nm = load_map(tmap, blockIdx.x < ntiles ? blockIdx.x : 0, t);
+    NodeMap nm = load_map<T>(tmap, blockIdx.x < ntiles ? blockIdx.x : 0, t);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int e = nm.e, code = nm.code;
     const long g = nm.g;   // loads are unconditional (see k_volume)
@@ -484,7 +488,7 @@ km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__
 #pragma unroll
     for (int c = 0; c < 9; ++c) ja[c] = Ja[c * S + g];
     const double jg = Jac[g];
-    if (tile + (int)gridDim.x < ntiles) nm = load_map(tmap, tile + gridDim.x, t);   // the next tile's
+    if (tile + (int)gridDim.x < ntiles) nm = load_map<T>(tmap, tile + gridDim.x, t);   // the next tile's
     const double iJ = 1.0 / jg;   // the same bits as a stored 1/J
     const bool onx = i == 0 || i == Nx, ony = j == 0 || j == Ny, onz = k == 0 || k == Nz;
     const long xx = efoff[6 * e + (i == 0 ? 5 : 3)] + j + ny * k;
@@ -581,7 +585,7 @@ km_grad(const NodeMap *__restrict__ tmap, int ntiles, const double *__restrict__
     }
     __syncthreads();   // the tile's sQ has been read: the next tile overwrites it
     }
-    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, t, blockIdx.x);
+    if (dtl.part) hxk::block_min_pair(tc, tv, dtl.part, t, pbase + blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -725,8 +729,8 @@ constexpr int mvolume_per_node()
     return 3 * (FORM >= 0 ? 4 : 5) + (FORM >= 0 ? hx::node_data_size(FORM) + 3 : 0);
 }
 
-template <int FORM, bool GAUSS>
-__global__ void __launch_bounds__(TILE)
+template <int FORM, bool GAUSS, int T>
+__global__ void __launch_bounds__(T)
 km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, const double *__restrict__ U,
           const int *__restrict__ efoff,
           const double *__restrict__ Ja, const double *__restrict__ Jac, const double *__restrict__ edelta,
@@ -749,7 +753,7 @@ km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, 
     for (int x = t; x < 2 * MAXN1 * 64; x += blockDim.x)
         sOp[x] = x < MAXN1 * 64 ? ops.hatD[x / 64][x % 64] : ops.sharpD[(x / 64) - MAXN1][x % 64];
     double *chunk0 = smem + 2 * MAXN1 * 64;
-    NodeMap nm = load_map(tmap, blockIdx.x < ntiles ? blockIdx.x : 0, t);
+    NodeMap nm = load_map<T>(tmap, blockIdx.x < ntiles ? blockIdx.x : 0, t);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int e = nm.e, code = nm.code;
     const long g = nm.g;
@@ -811,7 +815,7 @@ km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, 
         for (int v = 0; v < NV; ++v) gq[d][v] = U[(d * NV + v) * S + g];
     const double dl = edelta[e];
     const double jg = Jac[g];
-    if (tile + (int)gridDim.x < ntiles) nm = load_map(tmap, tile + gridDim.x, t);   // the next tile's
+    if (tile + (int)gridDim.x < ntiles) nm = load_map<T>(tmap, tile + gridDim.x, t);   // the next tile's
     if (active) {
         double Fvc[3][NV];
         if (!ph.euler) {
@@ -941,7 +945,7 @@ km_volume(const NodeMap *__restrict__ tmap, int ntiles, double *__restrict__ Q, 
             hxk::hold(dt0);
             hxk::hold(dt1);
             const double dt = fmin(dt0, dt1);
-            if (dta.time && tile == 0 && t == 0) dta.time[0] += dt;   // once per launch
+            if (dta.time && tile == 0 && t == 0) dta.time[0] += dt;   // the host passes the time to one launch per stage
             const double cdt = c_k * dt;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
