@@ -480,6 +480,9 @@ __host__ __device__ constexpr int pair_slot(int n1, int lo, int hi)
 #ifndef HX3D_VOL_WAVES
 #define HX3D_VOL_WAVES 3
 #endif
+#ifndef HX3D_ROW5
+#define HX3D_ROW5 1
+#endif
 constexpr int volume_waves(int n, int form)
 {
     return form == -1 ? 3 : (form == 2 || form == 3) && n <= 5 ? HX3D_VOL_WAVES : 1;
@@ -502,7 +505,10 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
     // split form: node data [ND][ne] and metric rows [3][ne], then the pair fluxes
     // [pair][NV][line] of the current direction, in one space per element
     constexpr int NPAIR = n1 * (n1 - 1) / 2, NP = n1 / 2;
-    constexpr int RN = (ND + 3) * ne, RP = NPAIR * NV * nf;
+    // split form at N = 5: each node evaluates its own row of pair fluxes (ROW), without the
+    // pair table
+    constexpr bool ROW = SPLIT && N == 5 && HX3D_ROW5;
+    constexpr int RN = (ND + 3) * ne, RP = ROW ? 0 : NPAIR * NV * nf;
     constexpr int RSZ = SPLIT ? (RN > RP ? RN : RP) : 1;
     __shared__ double sF[EPB][3][NF][ne];
     __shared__ double sR[EPB][RSZ];
@@ -650,6 +656,41 @@ k_volume(double *__restrict__ Q, const double *__restrict__ vgrad, const double 
                     const int m = base + l * stride;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) qd[v] = qd[v] + hh * sF[el][d][v][m];
+                }
+            }
+            continue;
+        }
+        if (ROW) {
+            // each node evaluates its whole row of two-point fluxes from the line's node data (a
+            // pair twice, once by each of its nodes, with the same arithmetic) and accumulates it
+            // in l order; the diagonal is the node's inviscid contravariant flux
+            if (active) {
+                double fi[NV];
+                hx::euler_contravariant_row(q, pu, pv, pw, pp, &ja[3 * d], fi);
+#pragma unroll 1
+                for (int l = 0; l < n1; ++l) {
+                    const double sh = sSharp[own_i * n1 + l], hh = sHat[own_i * n1 + l];
+                    const int m = base + l * stride;
+                    double b[ND], jb[3], tp[NV];
+#pragma unroll
+                    for (int c = 0; c < ND; ++c) b[c] = sR[el][c * ne + m];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) jb[c] = sR[el][(ND + c) * ne + m];
+                    if (SYM) {
+                        hx::two_point<AVG>(own, jh, b, jb, tp);
+                    } else {
+                        const bool lo = own_i < l;
+                        double x1[ND], x2[ND], j1[3], j2[3];
+#pragma unroll
+                        for (int c = 0; c < ND; ++c) { x1[c] = lo ? own[c] : b[c]; x2[c] = lo ? b[c] : own[c]; }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) { j1[c] = lo ? jh[c] : jb[c]; j2[c] = lo ? jb[c] : jh[c]; }
+                        hx::two_point<AVG>(x1, j1, x2, j2, tp);
+                    }
+                    qd[0] = qd[0] + sh * (l == own_i ? fi[0] : tp[0]);
+#pragma unroll
+                    for (int v = 1; v < NV; ++v)
+                        qd[v] = qd[v] + sh * (l == own_i ? fi[v] : tp[v]) + hh * sF[el][d][v - V0][m];
                 }
             }
             continue;
