@@ -472,10 +472,11 @@ __host__ __device__ constexpr int pair_slot(int n1, int lo, int hi)
 // evaluates both (i, l) and (l, i) (HyperbolicSplitForm.f90:64-116), by node lo of the pair
 // (lo, lo + s mod N+1), s = 1..(N+1)/2 (for even N+1, s = (N+1)/2 only for lo < (N+1)/2); the
 // pair fluxes then replace the node data in LDS and every node reads its N off-diagonal
-// entries from there.  The diagonal entry is the node's inviscid contravariant flux.
+// entries from there (at N = 5 each node evaluates its own row of N pair fluxes instead: ROW).
+// The diagonal entry is the node's inviscid contravariant flux.
 // Minimum waves per SIMD asked of the register allocator: 3 for the standard form, and for
 // Kennedy-Gruber and Pirozzoli up to N = 5 (no spills in 168 VGPRs up to N = 4; at N = 5 the
-// 10 spilled registers cost less than a third wave gains), else no constraint (at N = 6 the
+// few spilled registers cost less than a third wave gains), else no constraint (at N = 6 the
 // forced third wave measured 14% slower than two waves without spills).
 #ifndef HX3D_VOL_WAVES
 #define HX3D_VOL_WAVES 3
