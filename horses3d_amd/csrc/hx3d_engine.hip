@@ -851,16 +851,20 @@ void set_physics(hx3d_engine *eng, const hx3d_case_desc *d)
 //   1  elements of one order triple (Nx, Ny, Nz) together, in turn: tiles of like elements, so
 //      the contractions' loop bounds agree across a wave;
 //   2  one element per tile.
-// Tiles hold TILE_SMALL threads when every element has at most that many nodes, else TILE
-// (the kernels and launches take both sizes at once: small elements in small tiles, the others
-// in large ones, each size swept by its own launch).  Every policy
+// Tile sizes (policy / 4): by default tiles hold TILE_SMALL threads when every element has at
+// most that many nodes, else TILE; the kernels and launches also take both sizes at once (small
+// elements in small tiles, the others in large ones, each size swept by its own launch).  Every policy
 // gives the same results (each element is computed from its own data alone; the dt partials are
 // minima).  Also sets each element kernel's grids: the blocks resident at once.
 int set_tiles(hx3d_engine *eng, int policy)
 {
     const std::vector<int> &N3 = eng->N3h_mixed;
     const int K = eng->K;
-    if (policy < 0 || policy > 2) return eng->fail(HX3D_EINVAL, "unknown tile packing policy");
+    // policy = packing + 4 * sizes; sizes 0: small tiles when every element fits, else large
+    // ones; 1: large tiles for every element; 2: small elements in small tiles, the others in
+    // large ones (a launch of each size)
+    const int packing = policy % 4, sizes = policy / 4;
+    if (policy < 0 || packing > 2 || sizes > 2) return eng->fail(HX3D_EINVAL, "unknown tile packing policy");
     std::vector<int> ne(K);
     for (int e = 0; e < K; ++e) ne[e] = (N3[3 * e] + 1) * (N3[3 * e + 1] + 1) * (N3[3 * e + 2] + 1);
     // the elements of each tile size, packed under the policy into tiles of cap threads
@@ -883,7 +887,7 @@ int set_tiles(hx3d_engine *eng, int policy)
             return t.count < hxm::TILE_MAXE && t.off[t.count] + ne[e] <= cap;
         };
         const int n = (int)order.size();
-        if (policy == 0) {
+        if (packing == 0) {
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ne[a] > ne[b]; });
             const int min_ne = n ? ne[order[n - 1]] : 0;
             int first_open = 0;   // tiles before it hold no element of any size left
@@ -897,7 +901,7 @@ int set_tiles(hx3d_engine *eng, int policy)
                        (tiles[first_open].count == hxm::TILE_MAXE || tiles[first_open].off[tiles[first_open].count] + min_ne > cap))
                     ++first_open;
             }
-        } else if (policy == 1) {
+        } else if (packing == 1) {
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
                 if (ne[a] != ne[b]) return ne[a] > ne[b];
                 for (int c = 0; c < 3; ++c)
@@ -921,7 +925,7 @@ int set_tiles(hx3d_engine *eng, int policy)
     // orders up to 7, a launch of each size measured 0.6% slower than the large tiles alone)
     std::vector<int> small, big;
     for (int e = 0; e < K; ++e) (ne[e] <= hxm::TILE_SMALL ? small : big).push_back(e);
-    if (!big.empty()) {
+    if (sizes == 1 || (sizes == 0 && !big.empty())) {
         big.insert(big.end(), small.begin(), small.end());
         std::sort(big.begin(), big.end());
         small.clear();

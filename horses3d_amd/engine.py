@@ -483,8 +483,11 @@ class Engine:
         self._chk(load_library().hx3d_autotune(self.h, int(reps), C.byref(ch), _p(ms)), "hx3d_autotune")
         return ch.value, (ms if ch.value >= 0 else None)
 
-    def set_tiling(self, policy):
-        self._chk(load_library().hx3d_set_tiling(self.h, int(policy)), "hx3d_set_tiling")
+    def set_tiling(self, policy, sizes=0):
+        """Mixed orders: element tiling = packing policy (0 first-fit decreasing, 1 order-triple
+        groups, 2 one element per tile) and tile sizes (0 256-thread tiles when every element
+        fits, else 512; 1 512-thread tiles; 2 both, each element in the smaller that fits)."""
+        self._chk(load_library().hx3d_set_tiling(self.h, int(policy) + 4 * int(sizes)), "hx3d_set_tiling")
 
     def tiling(self):
         pol, nt = C.c_int(), C.c_int()

@@ -202,13 +202,16 @@ long hx3d_engine_device_bytes(const hx3d_engine *eng);
 /* Free and total memory of the current HIP device (admission by free HBM). */
 int hx3d_device_memory(size_t *free_bytes, size_t *total_bytes);
 /* No counterpart in the reference (its elements run one at a time).  Mixed orders: the element
- * kernels are persistent, one launch per stage over tiles of elements (one thread per node, at
- * most 512 nodes a tile).  The tiling (which elements share a block) is a packing policy:
- * 0 first-fit decreasing by size, 1 elements of one order triple together, 2 one element per
- * tile.  hx3d_autotune times reps evaluations of the resident state's time derivative under each
- * policy (ms[p] per evaluation when ms != NULL, 3 entries), keeps the fastest and returns it in
- * *choice; results do not depend on the policy.  A uniform-order engine has one tiling:
- * *choice = -1.  hx3d_set_tiling / hx3d_get_tiling set and read the policy (mixed orders). */
+ * kernels are persistent, one launch per stage over tiles of elements (one thread per node, tiles
+ * of 256 or 512 threads).  The tiling (which elements share a block) is a policy, packing +
+ * 4 * sizes.  Packing: 0 first-fit decreasing by size, 1 elements of one order triple together,
+ * 2 one element per tile.  Sizes: 0 256-thread tiles when every element has at most 256 nodes,
+ * else 512-thread tiles (the default); 1 512-thread tiles; 2 256-thread tiles for the elements
+ * that fit and 512-thread tiles for the others.  hx3d_autotune times reps evaluations of the
+ * resident state's time derivative under each packing (sizes 0; ms[p] per evaluation when
+ * ms != NULL, 3 entries), keeps the fastest and returns it in *choice; results do not depend on
+ * the policy.  A uniform-order engine has one tiling: *choice = -1.  hx3d_set_tiling /
+ * hx3d_get_tiling set and read the policy (mixed orders). */
 int hx3d_autotune(hx3d_engine *eng, int reps, int *choice, double *ms);
 int hx3d_set_tiling(hx3d_engine *eng, int policy);
 int hx3d_get_tiling(const hx3d_engine *eng, int *policy, int *ntiles);

@@ -212,22 +212,24 @@ def test_mixed_orders_box_matches_oracle(inviscid, averaging, riemann, les):
 @pytest.mark.parametrize("inviscid", ["standard", "split-form"])
 def test_mixed_tilings_agree_bit_for_bit(inviscid):
     """The persistent element kernels of mixed orders give the same bits under every element
-    tiling (first-fit decreasing, order-triple groups, one element per tile): QDot and two
-    device-dt steps; hx3d_autotune keeps one of them and leaves the resident state's QDot."""
+    tiling (first-fit decreasing, order-triple groups, one element per tile) and tile sizes (512
+    threads for all; 256-thread tiles for the elements that fit and 512 for the others, a launch
+    of each size): QDot and two device-dt steps (the stage time advanced once per stage);
+    hx3d_autotune keeps one packing and leaves the resident state's QDot."""
     case = mesh.periodic_box_mixed(5, mesh.random_orders(125, 1, 7, 11))
     Q0 = mesh.taylor_green(case, 0.2, noise=1e-3, seed=12)
     kw = dict(mach=0.2, reynolds=1600.0, riemann="lxf" if inviscid == "split-form" else "roe", inviscid=inviscid,
               averaging="pirozzoli" if inviscid == "split-form" else "standard", les="wale", les_c=None)
     eng = engine_for(case, kw)
     out = []
-    for pol in (0, 1, 2):
-        eng.set_tiling(pol)
-        assert eng.tiling()[0] == pol
+    for pol, sizes in ((0, 0), (1, 0), (2, 0), (0, 2), (1, 2), (2, 2), (0, 1)):
+        eng.set_tiling(pol, sizes)
+        assert eng.tiling()[0] == pol + 4 * sizes
         eng.set_state(Q0)
         qd = eng.qdot()
         t = eng.advance(2, 0.4, 0.4, t=0.0)
         out.append((qd, eng.get_state(), t))
-        print("tiling", pol, "tiles", eng.tiling()[1], flush=True)
+        print("tiling", pol, "sizes", sizes, "tiles", eng.tiling()[1], flush=True)
     for qd, Q, t in out[1:]:
         assert np.array_equal(qd, out[0][0]) and np.array_equal(Q, out[0][1]) and t == out[0][2]
     eng.set_state(Q0)
