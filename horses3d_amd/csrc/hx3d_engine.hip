@@ -470,6 +470,7 @@ struct hx3d_engine {
     int4 *dFnode = nullptr;
     // mixed orders (p-anisotropic elements, mortar faces)
     bool mixed = false;
+    bool t2x = false;     // uniform orders: the face kernel forms t2 = n x t1 (see tangent2_is_cross)
     bool gauss = false;   // Gauss nodes (one order: hxg kernels; per-element orders: the mixed path's)
     long FSm = 0;
     int *dN3 = nullptr, *dEfoff = nullptr, *dEofn = nullptr;
@@ -595,9 +596,14 @@ template <int N>
 void launch_face(hx3d_engine *eng, const hxk::DtReduce &dtr)
 {
     const int nb = grid_for((int)((eng->FS + hxk::FACE_TPB - 1) / hxk::FACE_TPB));
-    hipLaunchKernelGGL((hxk::k_face<N>), dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ, eng->dVgrad,
-                       eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
-                       1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
+    if (eng->t2x)
+        hipLaunchKernelGGL((hxk::k_face<N, true>), dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ,
+                           eng->dVgrad, eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
+                           1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
+    else
+        hipLaunchKernelGGL((hxk::k_face<N, false>), dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ,
+                           eng->dVgrad, eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
+                           1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
 }
 
 template <int N, int FORM, int EPB>
@@ -1017,6 +1023,22 @@ int set_tiles(hx3d_engine *eng, int policy)
     return HX3D_OK;
 }
 
+// Whether the stored second face tangent is n x t1 with vCross's arithmetic
+// (MappedGeometry.f90:745, :807-817) at every face node, bit for bit (fgeo[c*FS + q]: n at c = 0..2,
+// t1 at 3..5, t2 at 6..8).  The reference's geometry and the mesh front end's are; a descriptor
+// with other tangents keeps them, read by the face kernel.
+bool tangent2_is_cross(const std::vector<double> &fgeo, long FS)
+{
+    for (long q = 0; q < FS; ++q) {
+        const double *n0 = &fgeo[q], *t = &fgeo[3 * FS + q], *t2 = &fgeo[6 * FS + q];
+        const double n[3] = {n0[0], n0[FS], n0[2 * FS]}, u[3] = {t[0], t[FS], t[2 * FS]};
+        const double x[3] = {n[1] * u[2] - u[1] * n[2], n[2] * u[0] - u[2] * n[0], n[0] * u[1] - u[0] * n[1]};
+        for (int c = 0; c < 3; ++c)
+            if (std::memcmp(&x[c], &t2[c * FS], sizeof(double)) != 0) return false;
+    }
+    return true;
+}
+
 // The waves of the mortar face kernels (hxm::face_lane): a face of more than 32 nodes alone, a
 // smaller one with the next face of the same shape (face orders and both sides' orders in the
 // face orientation), in face order at the second face of each pair; faces left without a
@@ -1430,6 +1452,7 @@ int hx3d_engine_create(const hx3d_case_desc *d, hx3d_engine **out)
             }
         fdelta[f] = std::sqrt(surf / nf);   // SD.f90:1378
     }
+    eng->t2x = tangent2_is_cross(fgeo, FS);
     // ---- device -------------------------------------------------------------------
     int rc;
     if ((rc = dalloc(eng, &eng->dQ, (size_t)NV * S)) || (rc = dalloc(eng, &eng->dG, (size_t)NV * S)) ||

@@ -95,6 +95,30 @@ def test_box16_n4_curved_matches_oracle(inviscid, averaging, riemann, les):
     eng.close()
 
 
+def test_face_tangent_read_when_not_the_cross_product():
+    """The face kernel forms the second tangent as n x t1 only where the descriptor's t2 is that
+    product bit for bit (the reference's geometry, MappedGeometry.f90:745); with t2 moved by one
+    ulp at every face node it reads the given t2, and the Lax-Friedrichs QDot still matches the
+    oracle on the same geometry."""
+    case = mesh.periodic_box(3, 3, amplitude=0.1, phase_seed=4)
+    n, t1, t2 = (np.asarray(case[k]).reshape(-1, 3) for k in ("normal", "t1", "t2"))
+    cross = np.stack([n[:, 1] * t1[:, 2] - t1[:, 1] * n[:, 2], n[:, 2] * t1[:, 0] - t1[:, 2] * n[:, 0],
+                      n[:, 0] * t1[:, 1] - t1[:, 0] * n[:, 1]], 1)
+    assert np.array_equal(cross.view(np.int64), t2.view(np.int64))
+    moved = dict(case)
+    moved["t2"] = np.nextafter(t2, np.inf).reshape(np.shape(case["t2"]))
+    Q = mesh.taylor_green(case, 0.2, noise=1e-3, seed=5)
+    kw = dict(mach=0.2, reynolds=1000.0, riemann="lxf", inviscid="split-form", averaging="pirozzoli",
+              les=None, les_c=None)
+    for c in (case, moved):
+        eng, ora = engine_for(c, kw), oracle_for(c, kw)
+        eng.set_state(Q)
+        err = po.normwise_error(eng.qdot(), ora.qdot(Q))
+        print(err, flush=True)
+        assert err.max() <= TOL
+        eng.close()
+
+
 @pytest.mark.parametrize("N,averaging,riemann,les", [
     (5, "pirozzoli", "roe", "wale"),
     (6, "pirozzoli", "lxf", None),
