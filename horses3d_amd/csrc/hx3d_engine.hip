@@ -1043,7 +1043,8 @@ bool tangent2_is_cross(const std::vector<double> &fgeo, long FS)
 // smaller one with the next face of the same shape (face orders and both sides' orders in the
 // face orientation), in face order at the second face of each pair; faces left without a
 // partner alone at the end.  Every face is in exactly one wave, and each face's results depend
-// only on its own data, so any grouping gives the same bits.
+// only on its own data, so any grouping gives the same bits.  Built with HX3D_FACE_PAIRS=0,
+// every face takes a wave of its own (for comparison).
 #ifndef HX3D_FACE_PAIRS
 #define HX3D_FACE_PAIRS 1
 #endif

@@ -491,6 +491,8 @@ __host__ __device__ constexpr int pair_slot(int n1, int lo, int hi)
 #ifndef HX3D_VOL_WAVES
 #define HX3D_VOL_WAVES 3
 #endif
+// Split form at N = 5 without the pair table (each node evaluates its own row); built with
+// HX3D_ROW5=0, the pair table as at the other orders (for comparison).
 #ifndef HX3D_ROW5
 #define HX3D_ROW5 1
 #endif
