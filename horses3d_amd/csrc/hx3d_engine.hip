@@ -596,14 +596,14 @@ template <int N>
 void launch_face(hx3d_engine *eng, const hxk::DtReduce &dtr)
 {
     const int nb = grid_for((int)((eng->FS + hxk::FACE_TPB - 1) / hxk::FACE_TPB));
-    if (eng->t2x)
-        hipLaunchKernelGGL((hxk::k_face<N, true>), dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ,
-                           eng->dVgrad, eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
-                           1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
-    else
-        hipLaunchKernelGGL((hxk::k_face<N, false>), dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ,
-                           eng->dVgrad, eng->dFgeo, eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES,
-                           1.0 / eng->ops.w[0], eng->ph, dtr, eng->mc);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(hxk::FACE_TPB), 0, eng->stream, eng->dQ, eng->dVgrad, eng->dFgeo,
+                           eng->dFnode, eng->dFdelta, eng->dFe, eng->FS, eng->S, eng->ES, 1.0 / eng->ops.w[0],
+                           eng->ph, dtr, eng->mc);
+    };
+    if (eng->ph.riemann != 1) go(hxk::k_face<N, false, false>);   // Roe: no tangents
+    else if (eng->t2x) go(hxk::k_face<N, true, true>);
+    else go(hxk::k_face<N, false, true>);
 }
 
 template <int N, int FORM, int EPB>

@@ -314,8 +314,9 @@ struct DtReduce {
 // threads (face nodes) per block of k_face (128 and 512 measured 12% and 18% slower at 16^3 x N=4)
 constexpr int FACE_TPB = 256;
 // T2X: the stored second tangent is n x t1 (vCross, MappedGeometry.f90:745, :807-817) bit for
-// bit at every face node (checked on the host), so it is formed here instead of read.
-template <int N, bool T2X>
+// bit at every face node (checked on the host), so it is formed here instead of read.  LXF: the
+// engine's Riemann solver is Lax-Friedrichs, the one that reads the tangents (Roe does not).
+template <int N, bool T2X, bool LXF>
 __global__ void __launch_bounds__(FACE_TPB)
 __attribute__((amdgpu_waves_per_eu(HX3D_FACE_WAVES)))
 k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const double *__restrict__ fgeo,
@@ -352,8 +353,8 @@ k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const dou
     for (int v = 0; v < NV; ++v) { qL[v] = ldg(Q + v * S, gx); qR[v] = ldg(Q + v * S, gy); }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        t1[c] = ldg(fgeo + (3 + c) * FS, fq);
-        t2[c] = T2X ? 0.0 : ldg(fgeo + (6 + c) * FS, fq);
+        t1[c] = LXF ? ldg(fgeo + (3 + c) * FS, fq) : 0.0;
+        t2[c] = (LXF && !T2X) ? ldg(fgeo + (6 + c) * FS, fq) : 0.0;
     }
 #pragma unroll
     for (int c = 0; c < 12; ++c) { vgL[c] = ldg(vgrad + c * S, gx); vgR[c] = ldg(vgrad + c * S, gy); }
@@ -391,13 +392,16 @@ k_face(const double *__restrict__ Q, const double *__restrict__ vgrad, const dou
                       0.5 * (FL[v][2] + FR[v][2]) * nrm[2];
     }
     double inv[NV];
-    if (T2X) {
-        t2[0] = nrm[1] * t1[2] - t1[1] * nrm[2];
-        t2[1] = nrm[2] * t1[0] - t1[2] * nrm[0];
-        t2[2] = nrm[0] * t1[1] - t1[0] * nrm[1];
+    if (LXF) {
+        if (T2X) {
+            t2[0] = nrm[1] * t1[2] - t1[1] * nrm[2];
+            t2[1] = nrm[2] * t1[0] - t1[2] * nrm[0];
+            t2[2] = nrm[0] * t1[1] - t1[0] * nrm[1];
+        }
+        hx::lxf_flux(ph, qL, qR, nrm, t1, t2, inv);
+    } else {
+        hx::roe_flux(ph, qL, qR, nrm, inv);
     }
-    if (ph.riemann == 1) hx::lxf_flux(ph, qL, qR, nrm, t1, t2, inv);
-    else hx::roe_flux(ph, qL, qR, nrm, inv);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         const double fb = ((inv[v] - visc[v]) * jac) * b0;
